@@ -1,0 +1,203 @@
+/*
+ * brkga_hip.h - C ABI of the MI355X (gfx950) BRKGA / NSGA-II engine.
+ *
+ * The engine runs the whole generation step of the reference's BRKGA solver
+ * on the GPU: key generation, decode + fitness, partition by fitness, elite
+ * copy, biased crossover, mutants and island ring migration, plus NSGA-II
+ * ranking / crowding / survivor selection.  Populations stay resident in
+ * device memory; only results cross this boundary.
+ *
+ * Conventions
+ *   - Every function returns a brkga_status and never throws.  On failure
+ *     brkga_last_error() (thread-local) describes the cause.
+ *   - `stream` arguments are hipStream_t values passed as void*; NULL means
+ *     the null stream of the engine's device.  Functions that return host
+ *     data synchronise the stream they use.
+ *   - Pointers named d_* are device pointers owned by the caller; all other
+ *     pointers are host memory owned by the caller.  Problem data is copied
+ *     into engine-owned device memory at create time.
+ *   - Handles are destroyed in reverse order of creation (populations,
+ *     batches and servers before their problems, problems before the engine).
+ *   - One engine per host thread, or external synchronisation; the server
+ *     functions are thread-safe.
+ *
+ * Reference interfaces replaced (paths relative to the reference's brkga/):
+ *   core/config.hpp       BRKGAConfig<T> (plug-in base, evaluate_population_gpu)
+ *   core/population.hpp   Population<T>
+ *   core/solver.hpp       Solver<T> generation loops and island model
+ *   core/gpu_population.cuh  GPUPopulation<T> (device-resident population API)
+ *   configs/<problem>_config.hpp  TSPConfig, KnapsackConfig, TSPJConfig, MultiTSPConfig
+ */
+#ifndef BRKGA_HIP_H
+#define BRKGA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    BRKGA_OK = 0,
+    BRKGA_ERR_INVALID_ARG = 1, /* bad sizes / parameters (config.hpp:159-179 is_valid) */
+    BRKGA_ERR_OOM = 2,         /* device allocation failed or over budget */
+    BRKGA_ERR_DEVICE = 3,      /* HIP runtime error */
+    BRKGA_ERR_BUSY = 4,        /* result not ready yet */
+    BRKGA_ERR_DEADLINE = 5,    /* request retired without completing */
+    BRKGA_ERR_NOT_FOUND = 6,   /* unknown request id */
+    BRKGA_ERR_NO_DEVICE = 7    /* no usable GPU */
+} brkga_status;
+
+/* Problem kinds (decoders).  BRKGA runs the single-objective kinds,
+ * NSGA-II the bi-objective ones. */
+typedef enum {
+    BRKGA_TSP_COORDS = 0,       /* configs/tsp_config.hpp (Euclidean, coordinates) */
+    BRKGA_TSP_MATRIX = 1,       /* configs/tsp_config.hpp:38-61 (n x n matrix) */
+    BRKGA_KNAPSACK_THRESHOLD = 2, /* configs/knapsack_config.hpp:43-75 */
+    BRKGA_KNAPSACK_GREEDY = 3,  /* greedy fill in key order (no reference counterpart) */
+    BRKGA_TSPJ = 4,             /* configs/tspj_config.hpp:42-65, 555-642 */
+    BRKGA_BITSP_COORDS = 5,     /* configs/multi_tsp_config.hpp, two coordinate sets */
+    BRKGA_BITSP_MATRIX = 6      /* configs/multi_tsp_config.hpp:25-52 (two matrices) */
+} brkga_problem_kind;
+
+typedef struct brkga_engine_s* brkga_engine_t;
+typedef struct brkga_problem_s* brkga_problem_t;
+typedef struct brkga_pop_s* brkga_pop_t;
+
+/* GA parameters: BRKGAConfig<T> fields (core/config.hpp:22-26) plus the
+ * island / migration / seed settings the reference hard-codes
+ * (core/solver.hpp:164-167). */
+typedef struct {
+    int32_t population_size;    /* total over all islands */
+    int32_t elite_size;         /* total over all islands (BRKGA) */
+    int32_t mutant_size;        /* total over all islands (BRKGA) */
+    double elite_prob;          /* rho: P(gene from elite parent); NSGA-II crossover coin */
+    int32_t num_islands;        /* >= 1; population_size must divide evenly */
+    int32_t migration_interval; /* generations between ring migrations (0 = never) */
+    int32_t num_migrants;       /* top-k rows sent to the next island, k <= elite per island */
+    uint64_t seed;              /* Philox key */
+} brkga_params;
+
+/* ------------------------------------------------------------ engine ---- */
+
+/* Version string of the engine library. */
+const char* brkga_version(void);
+
+/* Thread-local description of the last failure on this thread. */
+const char* brkga_last_error(void);
+
+/* Number of visible HIP devices (0 when none). */
+brkga_status brkga_device_count(int32_t* count);
+
+/* Binds an engine to one device.  Replaces Solver::initialize_gpu_system /
+ * detect_gpus (core/solver.hpp:1307-1394). */
+brkga_status brkga_engine_create(int32_t device, brkga_engine_t* out);
+brkga_status brkga_engine_destroy(brkga_engine_t engine);
+
+/* ----------------------------------------------------------- problems --- */
+
+/* Euclidean TSP over n cities from FP32 coordinates xy[n][2]; fitness is
+ * the closed-tour length, minimised.  Same decoder and edge formula as
+ * TSPConfig (configs/tsp_config.hpp:180-225, 359-371), without the n x n
+ * matrix. */
+brkga_status brkga_problem_create_tsp_coords(brkga_engine_t engine, const float* xy, int32_t n,
+                                             brkga_problem_t* out);
+
+/* TSP over an explicit distance matrix dist[n][n] (TSPConfig ctor,
+ * configs/tsp_config.hpp:38-61). */
+brkga_status brkga_problem_create_tsp_matrix(brkga_engine_t engine, const float* dist, int32_t n,
+                                             brkga_problem_t* out);
+
+/* 0/1 knapsack with integer weights/values, maximised.  greedy = 0: the
+ * reference's threshold decoder (key > 0.5, penalty 1000 per unit over
+ * capacity; configs/knapsack_config.hpp:43-75).  greedy = 1: pack items in
+ * ascending key order when they fit; exact integer value. */
+brkga_status brkga_problem_create_knapsack(brkga_engine_t engine, const int32_t* weights, const int32_t* values,
+                                           int32_t n, int64_t capacity, int32_t greedy, brkga_problem_t* out);
+
+/* TSPJ from travel[dim][dim] and jobs[dim][dim] (row/column 0 = depot,
+ * dim = n + 1); chromosome = n city keys then n job keys; makespan
+ * minimised (TSPJConfig, configs/tspj_config.hpp:42-65, 555-642). */
+brkga_status brkga_problem_create_tspj(brkga_engine_t engine, const float* travel, const float* jobs,
+                                       int32_t dim, brkga_problem_t* out);
+
+/* Bi-objective TSP (MultiTSPConfig, configs/multi_tsp_config.hpp:25-103):
+ * depot 0 + n cities; objective m is the depot-to-depot tour length in
+ * metric m.  _coords: two coordinate sets xy0/xy1 of n+1 points each;
+ * _matrix: two matrices of dimension dim = n + 1. */
+brkga_status brkga_problem_create_bitsp_coords(brkga_engine_t engine, const float* xy0, const float* xy1,
+                                               int32_t n, brkga_problem_t* out);
+brkga_status brkga_problem_create_bitsp_matrix(brkga_engine_t engine, const float* m0, const float* m1,
+                                               int32_t dim, brkga_problem_t* out);
+
+/* Kind, chromosome length, objectives and direction of a problem. */
+brkga_status brkga_problem_info(brkga_problem_t problem, int32_t* kind, int32_t* chrom_len, int32_t* num_obj,
+                                int32_t* maximize);
+brkga_status brkga_problem_destroy(brkga_problem_t problem);
+
+/* Decode + fitness of pop_size caller-owned device rows d_keys[pop][chrom_len];
+ * writes d_fitness[pop][num_obj] (the layout of population.hpp:609-624).
+ * Replaces BRKGAConfig::evaluate_population_gpu (core/config.hpp:50-53)
+ * and the configs' fitness kernels; asynchronous on `stream`. */
+brkga_status brkga_evaluate(brkga_problem_t problem, const float* d_keys, float* d_fitness, int32_t pop_size,
+                            int32_t chrom_len, void* stream);
+
+/* Decoded solutions d_decoded[pop][chrom_len] (int32): TSP / bi-TSP city
+ * order (0-based), knapsack 0/1 selection per item, TSPJ 1-based city ids
+ * then 1-based job ids.  Replaces BRKGAConfig::decoder (core/config.hpp:34). */
+brkga_status brkga_decode(brkga_problem_t problem, const float* d_keys, int32_t* d_decoded, int32_t pop_size,
+                          int32_t chrom_len, void* stream);
+
+/* -------------------------------------------------------- populations --- */
+
+/* Device-resident population: ping-pong key buffers [P][ld], fitness,
+ * rank order per island (and NSGA-II rank / crowding).  Replaces
+ * Population<T> (core/population.hpp:72-100) and GPUPopulation<T>
+ * (core/gpu_population.cuh:52-120).  Validation follows
+ * BRKGAConfig::is_valid (core/config.hpp:159-179). */
+brkga_status brkga_population_create(brkga_problem_t problem, const brkga_params* params, brkga_pop_t* out);
+brkga_status brkga_population_destroy(brkga_pop_t pop);
+
+/* Generation 0: Philox keys, evaluation and partition
+ * (Population::initialize, core/population.hpp:107-124). */
+brkga_status brkga_population_initialize(brkga_pop_t pop, void* stream);
+
+/* `generations` generation steps, one launch sequence per step
+ * (Solver::evolve_generation, core/solver.hpp:213-231 / 378-476). */
+brkga_status brkga_evolve(brkga_pop_t pop, int32_t generations, void* stream);
+
+/* Same steps replayed from a captured HIP graph of `generations` steps
+ * (the graph is built on first use and cached per length). */
+brkga_status brkga_evolve_graph(brkga_pop_t pop, int32_t generations, void* stream);
+
+/* Completed generation steps since initialize. */
+brkga_status brkga_population_generation(brkga_pop_t pop, int32_t* generation);
+
+/* Copies the current population to host (GPUPopulation::sync_to_host):
+ * keys[P][chrom_len], fitness[P][num_obj], order[P] (per island, the local
+ * slot of each rank), rank[P] and crowding[P] (NSGA-II only).  Any pointer
+ * may be NULL.  Synchronises `stream`. */
+brkga_status brkga_population_read(brkga_pop_t pop, float* keys, float* fitness, int32_t* order, int32_t* rank,
+                                   float* crowding, void* stream);
+
+/* Replaces the keys (GPUPopulation::sync_from_host) and re-evaluates /
+ * re-partitions them as a fresh generation 0. */
+brkga_status brkga_population_write(brkga_pop_t pop, const float* keys, void* stream);
+
+/* Best individual over all islands (Population::get_best,
+ * Solver::get_best_individual): fitness[num_obj], keys[chrom_len] and the
+ * decoded solution[chrom_len]; any pointer may be NULL.  NSGA-II: the
+ * rank-0 member with the largest crowding distance
+ * (population.hpp update_best_from_front). */
+brkga_status brkga_get_best(brkga_pop_t pop, float* fitness, float* keys, int32_t* decoded, void* stream);
+
+/* Raw device buffers of the current generation (row stride *ld floats). */
+brkga_status brkga_population_device_buffers(brkga_pop_t pop, float** d_keys, float** d_fitness,
+                                             int32_t** d_order, int32_t* ld);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BRKGA_HIP_H */

@@ -1,0 +1,316 @@
+"""Python (ctypes) binding of the MI355X BRKGA engine C ABI (include/brkga_hip.h).
+
+The engine itself is the HIP library ``libbrkga_hip.so`` next to this file;
+this module only marshals numpy arrays and handles.  There is no CPU
+fallback: if the library is missing or cannot be loaded, every call raises.
+
+Object model (mirrors the reference's ``Solver<T>`` / ``Population<T>`` /
+``BRKGAConfig<T>``, core/solver.hpp, core/population.hpp, core/config.hpp):
+
+    eng = Engine(0)
+    prob = Problem.tsp_coords(eng, xy)          # configs/tsp_config.hpp
+    pop = Population(prob, Params(population_size=1024, elite_size=152, ...))
+    pop.initialize(); pop.evolve(100)
+    fitness, keys, tour = pop.best()
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbrkga_hip.so")
+
+OK = 0
+ERR_INVALID_ARG = 1
+ERR_OOM = 2
+ERR_DEVICE = 3
+ERR_BUSY = 4
+ERR_DEADLINE = 5
+ERR_NOT_FOUND = 6
+ERR_NO_DEVICE = 7
+
+TSP_COORDS = 0
+TSP_MATRIX = 1
+KNAPSACK_THRESHOLD = 2
+KNAPSACK_GREEDY = 3
+TSPJ = 4
+BITSP_COORDS = 5
+BITSP_MATRIX = 6
+
+
+class BrkgaError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"brkga status {status}: {message}")
+        self.status = status
+
+
+class CParams(ctypes.Structure):
+    _fields_ = [
+        ("population_size", ctypes.c_int32),
+        ("elite_size", ctypes.c_int32),
+        ("mutant_size", ctypes.c_int32),
+        ("elite_prob", ctypes.c_double),
+        ("num_islands", ctypes.c_int32),
+        ("migration_interval", ctypes.c_int32),
+        ("num_migrants", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+@dataclass
+class Params:
+    """GA parameters (BRKGAConfig fields, core/config.hpp:22-26, plus the
+    island settings core/solver.hpp:164-167 hard-codes)."""
+
+    population_size: int = 100
+    elite_size: int = 20
+    mutant_size: int = 10
+    elite_prob: float = 0.7
+    num_islands: int = 1
+    migration_interval: int = 0
+    num_migrants: int = 0
+    seed: int = 0
+
+    def to_c(self) -> CParams:
+        return CParams(self.population_size, self.elite_size, self.mutant_size, float(self.elite_prob),
+                       self.num_islands, self.migration_interval, self.num_migrants,
+                       self.seed & 0xFFFFFFFFFFFFFFFF)
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_PF = ctypes.POINTER(ctypes.c_float)
+_PI = ctypes.POINTER(ctypes.c_int32)
+
+_SIGNATURES = {
+    "brkga_version": ([], ctypes.c_char_p),
+    "brkga_last_error": ([], ctypes.c_char_p),
+    "brkga_device_count": ([_PI], ctypes.c_int),
+    "brkga_engine_create": ([_I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_engine_destroy": ([_P], ctypes.c_int),
+    "brkga_problem_create_tsp_coords": ([_P, _PF, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_problem_create_tsp_matrix": ([_P, _PF, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_problem_create_knapsack": ([_P, _PI, _PI, _I32, ctypes.c_int64, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_problem_create_tspj": ([_P, _PF, _PF, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_problem_create_bitsp_coords": ([_P, _PF, _PF, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_problem_create_bitsp_matrix": ([_P, _PF, _PF, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_problem_info": ([_P, _PI, _PI, _PI, _PI], ctypes.c_int),
+    "brkga_problem_destroy": ([_P], ctypes.c_int),
+    "brkga_evaluate": ([_P, _P, _P, _I32, _I32, _P], ctypes.c_int),
+    "brkga_decode": ([_P, _P, _P, _I32, _I32, _P], ctypes.c_int),
+    "brkga_population_create": ([_P, ctypes.POINTER(CParams), ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_population_destroy": ([_P], ctypes.c_int),
+    "brkga_population_initialize": ([_P, _P], ctypes.c_int),
+    "brkga_evolve": ([_P, _I32, _P], ctypes.c_int),
+    "brkga_evolve_graph": ([_P, _I32, _P], ctypes.c_int),
+    "brkga_population_generation": ([_P, _PI], ctypes.c_int),
+    "brkga_population_read": ([_P, _PF, _PF, _PI, _PI, _PF, _P], ctypes.c_int),
+    "brkga_population_write": ([_P, _PF, _P], ctypes.c_int),
+    "brkga_get_best": ([_P, _PF, _PF, _PI, _P], ctypes.c_int),
+    "brkga_population_device_buffers": ([_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P), _PI],
+                                        ctypes.c_int),
+}
+
+
+def exported_symbols():
+    """Names the C ABI exports (every function declared in include/brkga_hip.h)."""
+    return list(_SIGNATURES)
+
+
+def lib():
+    """Loads libbrkga_hip.so; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP engine library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(status: int):
+    if status != OK:
+        msg = lib().brkga_last_error().decode(errors="replace")
+        raise BrkgaError(status, msg)
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(_PF)
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(_PI)
+
+
+def device_count() -> int:
+    c = ctypes.c_int32(0)
+    _check(lib().brkga_device_count(ctypes.byref(c)))
+    return c.value
+
+
+class Engine:
+    """One engine per device (Solver::initialize_gpu_system, core/solver.hpp:1307-1394)."""
+
+    def __init__(self, device: int = 0):
+        h = _P()
+        _check(lib().brkga_engine_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            lib().brkga_engine_destroy(self.handle)
+            self.handle = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Problem:
+    """A problem plug-in with engine-owned device data (BRKGAConfig subclasses)."""
+
+    def __init__(self, engine: Engine, handle):
+        self.engine = engine
+        self.handle = handle
+        kind, L, nobj, mx = (ctypes.c_int32() for _ in range(4))
+        _check(lib().brkga_problem_info(handle, ctypes.byref(kind), ctypes.byref(L), ctypes.byref(nobj),
+                                        ctypes.byref(mx)))
+        self.kind, self.chrom_len, self.num_obj, self.maximize = kind.value, L.value, nobj.value, mx.value
+
+    @staticmethod
+    def _make(engine, fn, *args):
+        h = _P()
+        _check(fn(engine.handle, *args, ctypes.byref(h)))
+        return Problem(engine, h)
+
+    @classmethod
+    def tsp_coords(cls, engine, xy):
+        xy = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1, 2)
+        p = cls._make(engine, lib().brkga_problem_create_tsp_coords, _fptr(xy), xy.shape[0])
+        p._data = (xy,)
+        return p
+
+    @classmethod
+    def tsp_matrix(cls, engine, dist):
+        dist = np.ascontiguousarray(dist, dtype=np.float32)
+        n = dist.shape[0]
+        return cls._make(engine, lib().brkga_problem_create_tsp_matrix, _fptr(dist), n)
+
+    @classmethod
+    def knapsack(cls, engine, weights, values, capacity, greedy=True):
+        w = np.ascontiguousarray(weights, dtype=np.int32)
+        v = np.ascontiguousarray(values, dtype=np.int32)
+        return cls._make(engine, lib().brkga_problem_create_knapsack, _iptr(w), _iptr(v), w.shape[0],
+                         int(capacity), 1 if greedy else 0)
+
+    @classmethod
+    def tspj(cls, engine, travel, jobs):
+        t = np.ascontiguousarray(travel, dtype=np.float32)
+        j = np.ascontiguousarray(jobs, dtype=np.float32)
+        return cls._make(engine, lib().brkga_problem_create_tspj, _fptr(t), _fptr(j), t.shape[0])
+
+    @classmethod
+    def bitsp_coords(cls, engine, xy0, xy1):
+        a = np.ascontiguousarray(xy0, dtype=np.float32).reshape(-1, 2)
+        b = np.ascontiguousarray(xy1, dtype=np.float32).reshape(-1, 2)
+        return cls._make(engine, lib().brkga_problem_create_bitsp_coords, _fptr(a), _fptr(b), a.shape[0] - 1)
+
+    @classmethod
+    def bitsp_matrix(cls, engine, m0, m1):
+        a = np.ascontiguousarray(m0, dtype=np.float32)
+        b = np.ascontiguousarray(m1, dtype=np.float32)
+        return cls._make(engine, lib().brkga_problem_create_bitsp_matrix, _fptr(a), _fptr(b), a.shape[0])
+
+    def evaluate_device(self, d_keys: int, d_fitness: int, pop_size: int, stream: int = 0):
+        """evaluate_population_gpu on caller-owned device pointers (ints)."""
+        _check(lib().brkga_evaluate(self.handle, d_keys, d_fitness, pop_size, self.chrom_len, stream or None))
+
+    def decode_device(self, d_keys: int, d_decoded: int, pop_size: int, stream: int = 0):
+        _check(lib().brkga_decode(self.handle, d_keys, d_decoded, pop_size, self.chrom_len, stream or None))
+
+    def close(self):
+        if self.handle:
+            lib().brkga_problem_destroy(self.handle)
+            self.handle = _P()
+
+
+class Population:
+    """Device-resident population (Population<T> / GPUPopulation<T>)."""
+
+    def __init__(self, problem: Problem, params: Params):
+        self.problem = problem
+        self.params = params
+        self._cp = params.to_c()
+        h = _P()
+        _check(lib().brkga_population_create(problem.handle, ctypes.byref(self._cp), ctypes.byref(h)))
+        self.handle = h
+
+    @property
+    def size(self):
+        return self.params.population_size
+
+    def initialize(self, stream: int = 0):
+        _check(lib().brkga_population_initialize(self.handle, stream or None))
+
+    def evolve(self, generations: int, graph: bool = False, stream: int = 0):
+        fn = lib().brkga_evolve_graph if graph else lib().brkga_evolve
+        _check(fn(self.handle, generations, stream or None))
+
+    @property
+    def generation(self) -> int:
+        g = ctypes.c_int32()
+        _check(lib().brkga_population_generation(self.handle, ctypes.byref(g)))
+        return g.value
+
+    def read(self, stream: int = 0):
+        P, L, M = self.size, self.problem.chrom_len, self.problem.num_obj
+        keys = np.empty((P, L), np.float32)
+        fit = np.empty((P, M), np.float32)
+        order = np.empty(P, np.int32)
+        rank = np.full(P, -1, np.int32)
+        crowd = np.zeros(P, np.float32)
+        _check(lib().brkga_population_read(self.handle, _fptr(keys), _fptr(fit), _iptr(order), _iptr(rank),
+                                           _fptr(crowd), stream or None))
+        out = {"keys": keys, "fitness": fit if M > 1 else fit[:, 0], "order": order}
+        if M > 1:
+            out["rank"] = rank
+            out["crowding"] = crowd
+        return out
+
+    def write(self, keys, stream: int = 0):
+        keys = np.ascontiguousarray(keys, dtype=np.float32)
+        assert keys.shape == (self.size, self.problem.chrom_len)
+        _check(lib().brkga_population_write(self.handle, _fptr(keys), stream or None))
+
+    def best(self, stream: int = 0):
+        M, L = self.problem.num_obj, self.problem.chrom_len
+        fit = np.empty(M, np.float32)
+        keys = np.empty(L, np.float32)
+        dec = np.empty(L, np.int32)
+        _check(lib().brkga_get_best(self.handle, _fptr(fit), _fptr(keys), _iptr(dec), stream or None))
+        return (fit if M > 1 else float(fit[0])), keys, dec
+
+    def close(self):
+        if self.handle:
+            lib().brkga_population_destroy(self.handle)
+            self.handle = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

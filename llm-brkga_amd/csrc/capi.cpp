@@ -1,0 +1,777 @@
+// capi.cpp - the C ABI (include/brkga_hip.h): engine, problems, resident
+// populations.  Host logic only; every kernel lives in kernels_*.hip.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "capi_internal.hpp"
+#include "device_common.hpp"
+#include "launch.hpp"
+
+namespace brkga {
+
+namespace {
+thread_local std::string g_last_error;
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+constexpr int kMaxGenes = 16000;   // fixed-point sums hold 2^14 terms
+constexpr int kMaxIslandPop = 16384;
+constexpr int kMaxNsgaPop = 2048;  // parents + children index in 12 bits
+
+double max_abs(const float* a, size_t n)
+{
+    double m = 0.0;
+    for (size_t i = 0; i < n; ++i) m = std::max(m, std::fabs((double)a[i]));
+    return m;
+}
+
+double coords_diameter(const float* xy, int count)
+{
+    if (count <= 0) return 0.0;
+    double x0 = xy[0], x1 = xy[0], y0 = xy[1], y1 = xy[1];
+    for (int i = 1; i < count; ++i) {
+        x0 = std::min(x0, (double)xy[2 * i]);
+        x1 = std::max(x1, (double)xy[2 * i]);
+        y0 = std::min(y0, (double)xy[2 * i + 1]);
+        y1 = std::max(y1, (double)xy[2 * i + 1]);
+    }
+    const double dx = x1 - x0, dy = y1 - y0;
+    return std::sqrt(dx * dx + dy * dy) * (1.0 + 1e-6);
+}
+
+bool all_finite(const float* a, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(a[i])) return false;
+    return true;
+}
+
+void set_scales(ProblemDesc* d, int s0, int s1)
+{
+    d->scale0 = std::ldexp(1.0f, s0);
+    d->inv0 = std::ldexp(1.0f, -s0);
+    d->scale1 = std::ldexp(1.0f, s1);
+    d->inv1 = std::ldexp(1.0f, -s1);
+}
+
+brkga_status upload(brkga_problem_s* pr, const void* host, size_t bytes, const void** dst)
+{
+    void* p = nullptr;
+    brkga_status st = device_alloc(&p, bytes, "problem data");
+    if (st != BRKGA_OK) return st;
+    pr->owned.push_back(p);
+    hipError_t e = hipMemcpy(p, host, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_error(e, "hipMemcpy(problem data)");
+    *dst = p;
+    return BRKGA_OK;
+}
+
+void free_all(std::vector<void*>& v)
+{
+    for (void* p : v) (void)hipFree(p);
+    v.clear();
+}
+
+brkga_status check_lds(const brkga_engine_s* eng, int kind, int n, int L)
+{
+    const int bytes = decode_layout(kind, n, L, sort_buckets(n)).bytes;
+    if ((size_t)bytes > eng->lds_limit)
+        return set_error(BRKGA_ERR_INVALID_ARG, "chromosome too long: decode needs " + std::to_string(bytes) +
+                                                    " bytes of LDS, the device allows " +
+                                                    std::to_string(eng->lds_limit));
+    return BRKGA_OK;
+}
+
+brkga_status new_problem(brkga_engine_t engine, int kind, int n, brkga_problem_s** out)
+{
+    if (!engine || !out) return set_error(BRKGA_ERR_INVALID_ARG, "null engine or output handle");
+    if (n < 1 || n > kMaxGenes)
+        return set_error(BRKGA_ERR_INVALID_ARG, "problem size must be in [1, " + std::to_string(kMaxGenes) + "]");
+    auto* pr = new brkga_problem_s();
+    pr->engine = engine;
+    std::memset(&pr->desc, 0, sizeof(pr->desc));
+    pr->desc.kind = kind;
+    pr->desc.n = n;
+    pr->desc.chrom_len = kind == KIND_TSPJ ? 2 * n : n;
+    pr->desc.num_obj = (kind == KIND_BITSP_COORDS || kind == KIND_BITSP_MATRIX) ? 2 : 1;
+    pr->desc.maximize = (kind == KIND_KNAP_THRESHOLD || kind == KIND_KNAP_GREEDY) ? 1 : 0;
+    brkga_status st = check_lds(engine, kind, n, pr->desc.chrom_len);
+    if (st != BRKGA_OK) {
+        delete pr;
+        return st;
+    }
+    *out = pr;
+    return BRKGA_OK;
+}
+
+brkga_status finish_problem(brkga_problem_s* pr, brkga_status st, brkga_problem_t* out)
+{
+    if (st != BRKGA_OK) {
+        free_all(pr->owned);
+        delete pr;
+        return st;
+    }
+    *out = pr;
+    return BRKGA_OK;
+}
+
+}  // namespace
+
+brkga_status set_error(brkga_status st, const std::string& msg)
+{
+    g_last_error = msg;
+    return st;
+}
+
+brkga_status hip_error(hipError_t e, const char* what)
+{
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return set_error(BRKGA_ERR_OOM, std::string(what) + ": " + hipGetErrorString(e));
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice)
+        return set_error(BRKGA_ERR_NO_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    return set_error(BRKGA_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+brkga_status device_alloc(void** p, size_t bytes, const char* what)
+{
+    *p = nullptr;
+    hipError_t e = hipMalloc(p, bytes < 16 ? 16 : bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+            return set_error(BRKGA_ERR_OOM, std::string("out of device memory allocating ") + what);
+        return hip_error(e, what);
+    }
+    return BRKGA_OK;
+}
+
+// Shift S with |term| * 2^S <= 2^48 for every term below max_abs_term.
+int fixed_shift(double max_abs_term)
+{
+    if (!(max_abs_term > 0.0) || !std::isfinite(max_abs_term)) return 30;
+    int e = 0;
+    std::frexp(max_abs_term, &e);
+    return std::max(-60, std::min(60, 48 - e));
+}
+
+brkga_status fill_inst_desc(const brkga_problem_s* pr, const brkga_params* q, InstDesc* d, int* nsga2)
+{
+    if (!pr || !q || !d) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
+    std::memset(d, 0, sizeof(*d));
+    const ProblemDesc& p = pr->desc;
+    const int mo = p.num_obj > 1;
+    *nsga2 = mo;
+    const int P = q->population_size;
+    if (P <= 0) return set_error(BRKGA_ERR_INVALID_ARG, "population_size must be > 0");
+    if (!(q->elite_prob >= 0.0 && q->elite_prob <= 1.0))
+        return set_error(BRKGA_ERR_INVALID_ARG, "elite_prob must be in [0, 1]");
+    d->prob = p;
+    d->L = p.chrom_len;
+    d->ld = round_up_int(p.chrom_len, 32);
+    d->pop = P;
+    d->seed_lo = (uint32_t)q->seed;
+    d->seed_hi = (uint32_t)(q->seed >> 32);
+    d->rho = (float)q->elite_prob;
+    d->nb = sort_buckets(p.n);
+    d->gen_limit = INT_MAX;
+    if (mo) {
+        if (q->num_islands > 1) return set_error(BRKGA_ERR_INVALID_ARG, "NSGA-II runs a single population");
+        if (P > kMaxNsgaPop)
+            return set_error(BRKGA_ERR_INVALID_ARG, "NSGA-II population_size must be <= " +
+                                                        std::to_string(kMaxNsgaPop));
+        d->islands = 1;
+        d->Pi = P;
+        d->mut_rate = (float)(5.0 / (double)p.chrom_len);
+        return BRKGA_OK;
+    }
+    const int I = q->num_islands < 1 ? 1 : q->num_islands;
+    if (P % I != 0) return set_error(BRKGA_ERR_INVALID_ARG, "population_size must be a multiple of num_islands");
+    const int Pi = P / I, Ei = q->elite_size / I, Mi = q->mutant_size / I;
+    if (Pi > kMaxIslandPop)
+        return set_error(BRKGA_ERR_INVALID_ARG, "island population must be <= " + std::to_string(kMaxIslandPop));
+    if (Ei <= 0) return set_error(BRKGA_ERR_INVALID_ARG, "elite_size per island must be > 0");
+    if (Mi < 0) return set_error(BRKGA_ERR_INVALID_ARG, "mutant_size must be >= 0");
+    if (Ei + Mi > Pi) return set_error(BRKGA_ERR_INVALID_ARG, "elite_size + mutant_size exceeds population_size");
+    if (Ei >= Pi && Pi - Ei - Mi > 0) return set_error(BRKGA_ERR_INVALID_ARG, "no non-elite parents");
+    if (q->migration_interval < 0 || q->num_migrants < 0)
+        return set_error(BRKGA_ERR_INVALID_ARG, "migration settings must be >= 0");
+    if (I > 1 && q->num_migrants > Ei)
+        return set_error(BRKGA_ERR_INVALID_ARG, "num_migrants must not exceed the elite per island");
+    d->islands = I;
+    d->Pi = Pi;
+    d->Ei = Ei;
+    d->Mi = Mi;
+    d->mig_interval = q->migration_interval;
+    d->mig_k = I > 1 ? q->num_migrants : 0;
+    return BRKGA_OK;
+}
+
+brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, std::vector<void*>* owned)
+{
+    const size_t rows = nsga2 ? (size_t)2 * d->pop : (size_t)d->pop;
+    const int M = d->prob.num_obj;
+    for (int b = 0; b < 2; ++b) {
+        void* p;
+        brkga_status st;
+        if ((st = device_alloc(&p, rows * d->ld * sizeof(float), "population keys")) != BRKGA_OK) return st;
+        owned->push_back(p);
+        d->keys[b] = (float*)p;
+        if ((st = device_alloc(&p, rows * M * sizeof(float), "fitness")) != BRKGA_OK) return st;
+        owned->push_back(p);
+        d->fit[b] = (float*)p;
+        if ((st = device_alloc(&p, rows * sizeof(int), "rank order")) != BRKGA_OK) return st;
+        owned->push_back(p);
+        d->perm[b] = (int*)p;
+        if (nsga2) {
+            if ((st = device_alloc(&p, rows * sizeof(int), "front ranks")) != BRKGA_OK) return st;
+            owned->push_back(p);
+            d->rank[b] = (int*)p;
+            if ((st = device_alloc(&p, rows * sizeof(float), "crowding")) != BRKGA_OK) return st;
+            owned->push_back(p);
+            d->crowd[b] = (float*)p;
+        }
+    }
+    void* c;
+    brkga_status st = device_alloc(&c, 2 * sizeof(int), "counters");
+    if (st != BRKGA_OK) return st;
+    owned->push_back(c);
+    d->gen = (int*)c;
+    d->done = (int*)c + 1;
+    // the padding columns are never produced by the RNG paths of every
+    // kernel; clear them once so copies stay deterministic
+    for (int b = 0; b < 2; ++b) {
+        hipError_t e = hipMemset(d->keys[b], 0, rows * d->ld * sizeof(float));
+        if (e != hipSuccess) return hip_error(e, "hipMemset(keys)");
+    }
+    hipError_t e = hipMemset(c, 0, 2 * sizeof(int));
+    if (e != hipSuccess) return hip_error(e, "hipMemset(counters)");
+    return BRKGA_OK;
+}
+
+int step_lds_bytes(const InstDesc& d)
+{
+    return decode_layout(d.prob.kind, d.prob.n, d.L, d.nb).bytes;
+}
+
+int part_lds_bytes(const InstDesc& d)
+{
+    return pow2_at_least(d.Pi) * 8;
+}
+
+hipError_t enqueue_init(int kind, int nsga2, int block, int step_lds, int part_block, int part_lds,
+                        const InstDesc* d_descs, int num_inst, int total_slots, int total_segs, int max_pop,
+                        hipStream_t s)
+{
+    hipError_t e;
+    if (nsga2) {
+        if ((e = launch_nsga2_offspring(kind, block, d_descs, num_inst, total_slots, step_lds, 1, s)) != hipSuccess)
+            return e;
+        return launch_nsga2_select(d_descs, num_inst, 1, max_pop, s);
+    }
+    if ((e = launch_brkga_step(kind, block, d_descs, num_inst, total_slots, step_lds, 1, s)) != hipSuccess) return e;
+    return launch_partition(part_block, d_descs, num_inst, total_segs, part_lds, 1, s);
+}
+
+hipError_t enqueue_step(int kind, int nsga2, int block, int step_lds, int part_block, int part_lds,
+                        const InstDesc* d_descs, int num_inst, int total_slots, int total_segs, int max_pop,
+                        hipStream_t s)
+{
+    hipError_t e;
+    if (nsga2) {
+        if ((e = launch_nsga2_offspring(kind, block, d_descs, num_inst, total_slots, step_lds, 0, s)) != hipSuccess)
+            return e;
+        if ((e = launch_nsga2_select(d_descs, num_inst, 0, max_pop, s)) != hipSuccess) return e;
+        return launch_nsga2_gather(d_descs, num_inst, total_slots, s);
+    }
+    if ((e = launch_brkga_step(kind, block, d_descs, num_inst, total_slots, step_lds, 0, s)) != hipSuccess) return e;
+    return launch_partition(part_block, d_descs, num_inst, total_segs, part_lds, 0, s);
+}
+
+}  // namespace brkga
+
+using namespace brkga;
+
+namespace {
+
+hipStream_t as_stream(void* s)
+{
+    return reinterpret_cast<hipStream_t>(s);
+}
+
+brkga_status pop_step(brkga_pop_t pop, int init, hipStream_t s)
+{
+    const InstDesc& d = pop->host;
+    hipError_t e = init ? enqueue_init(d.prob.kind, pop->nsga2, pop->block, pop->step_lds, pop->part_block,
+                                       pop->part_lds, pop->d_desc, 1, d.pop, d.islands, d.pop, s)
+                        : enqueue_step(d.prob.kind, pop->nsga2, pop->block, pop->step_lds, pop->part_block,
+                                       pop->part_lds, pop->d_desc, 1, d.pop, d.islands, d.pop, s);
+    if (e != hipSuccess) return hip_error(e, "kernel launch");
+    return BRKGA_OK;
+}
+
+brkga_status read_gen(brkga_pop_t pop, hipStream_t s, int* gen)
+{
+    BRKGA_HIP_TRY(hipStreamSynchronize(s));
+    BRKGA_HIP_TRY(hipMemcpy(gen, pop->host.gen, sizeof(int), hipMemcpyDeviceToHost));
+    return BRKGA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* brkga_version(void)
+{
+    return "brkga-mi355x 0.1.0 (gfx950)";
+}
+
+const char* brkga_last_error(void)
+{
+    return g_last_error.c_str();
+}
+
+brkga_status brkga_device_count(int32_t* count)
+{
+    if (!count) return set_error(BRKGA_ERR_INVALID_ARG, "null count");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c = 0;
+    }
+    *count = c;
+    return BRKGA_OK;
+}
+
+brkga_status brkga_engine_create(int32_t device, brkga_engine_t* out)
+{
+    if (!out) return set_error(BRKGA_ERR_INVALID_ARG, "null output handle");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return set_error(BRKGA_ERR_NO_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= count) return set_error(BRKGA_ERR_INVALID_ARG, "device index out of range");
+    DeviceGuard g(device);
+    int lds = 0, lds_optin = 0, cus = 0;
+    BRKGA_HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    if (hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, device) != hipSuccess) {
+        (void)hipGetLastError();
+        lds_optin = 0;
+    }
+    BRKGA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    auto* eng = new brkga_engine_s();
+    eng->device = device;
+    eng->lds_limit = (size_t)std::max(lds, lds_optin);
+    eng->cu_count = cus;
+    *out = eng;
+    return BRKGA_OK;
+}
+
+brkga_status brkga_engine_destroy(brkga_engine_t engine)
+{
+    delete engine;
+    return BRKGA_OK;
+}
+
+brkga_status brkga_problem_create_tsp_coords(brkga_engine_t engine, const float* xy, int32_t n,
+                                             brkga_problem_t* out)
+{
+    if (!xy) return set_error(BRKGA_ERR_INVALID_ARG, "null coordinates");
+    if (n > 0 && !all_finite(xy, (size_t)2 * n)) return set_error(BRKGA_ERR_INVALID_ARG, "non-finite coordinates");
+    brkga_problem_s* pr;
+    brkga_status st = new_problem(engine, KIND_TSP_COORDS, n, &pr);
+    if (st != BRKGA_OK) return st;
+    DeviceGuard g(engine->device);
+    pr->desc.dim = n;
+    set_scales(&pr->desc, fixed_shift(coords_diameter(xy, n)), 30);
+    st = upload(pr, xy, sizeof(float) * 2 * (size_t)n, (const void**)&pr->desc.a);
+    return finish_problem(pr, st, out);
+}
+
+brkga_status brkga_problem_create_tsp_matrix(brkga_engine_t engine, const float* dist, int32_t n,
+                                             brkga_problem_t* out)
+{
+    if (!dist) return set_error(BRKGA_ERR_INVALID_ARG, "null matrix");
+    if (n > 0 && !all_finite(dist, (size_t)n * n)) return set_error(BRKGA_ERR_INVALID_ARG, "non-finite distances");
+    brkga_problem_s* pr;
+    brkga_status st = new_problem(engine, KIND_TSP_MATRIX, n, &pr);
+    if (st != BRKGA_OK) return st;
+    DeviceGuard g(engine->device);
+    pr->desc.dim = n;
+    set_scales(&pr->desc, fixed_shift(max_abs(dist, (size_t)n * n)), 30);
+    st = upload(pr, dist, sizeof(float) * (size_t)n * n, (const void**)&pr->desc.a);
+    return finish_problem(pr, st, out);
+}
+
+brkga_status brkga_problem_create_knapsack(brkga_engine_t engine, const int32_t* weights, const int32_t* values,
+                                           int32_t n, int64_t capacity, int32_t greedy, brkga_problem_t* out)
+{
+    if (!weights || !values) return set_error(BRKGA_ERR_INVALID_ARG, "null weights or values");
+    brkga_problem_s* pr;
+    brkga_status st = new_problem(engine, greedy ? KIND_KNAP_GREEDY : KIND_KNAP_THRESHOLD, n, &pr);
+    if (st != BRKGA_OK) return st;
+    DeviceGuard g(engine->device);
+    pr->desc.dim = n;
+    pr->desc.cap = capacity;
+    set_scales(&pr->desc, 0, 0);
+    st = upload(pr, weights, sizeof(int32_t) * (size_t)n, (const void**)&pr->desc.w);
+    if (st == BRKGA_OK) st = upload(pr, values, sizeof(int32_t) * (size_t)n, (const void**)&pr->desc.v);
+    return finish_problem(pr, st, out);
+}
+
+brkga_status brkga_problem_create_tspj(brkga_engine_t engine, const float* travel, const float* jobs, int32_t dim,
+                                       brkga_problem_t* out)
+{
+    if (!travel || !jobs) return set_error(BRKGA_ERR_INVALID_ARG, "null matrices");
+    if (dim < 2) return set_error(BRKGA_ERR_INVALID_ARG, "TSPJ needs dim >= 2 (depot + one city)");
+    const size_t cells = (size_t)dim * dim;
+    if (!all_finite(travel, cells) || !all_finite(jobs, cells))
+        return set_error(BRKGA_ERR_INVALID_ARG, "non-finite matrix entries");
+    brkga_problem_s* pr;
+    brkga_status st = new_problem(engine, KIND_TSPJ, dim - 1, &pr);
+    if (st != BRKGA_OK) return st;
+    DeviceGuard g(engine->device);
+    pr->desc.dim = dim;
+    set_scales(&pr->desc, fixed_shift(std::max(max_abs(travel, cells), max_abs(jobs, cells))), 30);
+    st = upload(pr, travel, sizeof(float) * cells, (const void**)&pr->desc.a);
+    if (st == BRKGA_OK) st = upload(pr, jobs, sizeof(float) * cells, (const void**)&pr->desc.b);
+    return finish_problem(pr, st, out);
+}
+
+brkga_status brkga_problem_create_bitsp_coords(brkga_engine_t engine, const float* xy0, const float* xy1, int32_t n,
+                                               brkga_problem_t* out)
+{
+    if (!xy0 || !xy1) return set_error(BRKGA_ERR_INVALID_ARG, "null coordinates");
+    if (n > 0 && (!all_finite(xy0, (size_t)2 * (n + 1)) || !all_finite(xy1, (size_t)2 * (n + 1))))
+        return set_error(BRKGA_ERR_INVALID_ARG, "non-finite coordinates");
+    brkga_problem_s* pr;
+    brkga_status st = new_problem(engine, KIND_BITSP_COORDS, n, &pr);
+    if (st != BRKGA_OK) return st;
+    DeviceGuard g(engine->device);
+    pr->desc.dim = n + 1;
+    set_scales(&pr->desc, fixed_shift(coords_diameter(xy0, n + 1)), fixed_shift(coords_diameter(xy1, n + 1)));
+    st = upload(pr, xy0, sizeof(float) * 2 * (size_t)(n + 1), (const void**)&pr->desc.a);
+    if (st == BRKGA_OK) st = upload(pr, xy1, sizeof(float) * 2 * (size_t)(n + 1), (const void**)&pr->desc.b);
+    return finish_problem(pr, st, out);
+}
+
+brkga_status brkga_problem_create_bitsp_matrix(brkga_engine_t engine, const float* m0, const float* m1, int32_t dim,
+                                               brkga_problem_t* out)
+{
+    if (!m0 || !m1) return set_error(BRKGA_ERR_INVALID_ARG, "null matrices");
+    if (dim < 2) return set_error(BRKGA_ERR_INVALID_ARG, "bi-objective TSP needs dim >= 2 (depot + one city)");
+    const size_t cells = (size_t)dim * dim;
+    if (!all_finite(m0, cells) || !all_finite(m1, cells))
+        return set_error(BRKGA_ERR_INVALID_ARG, "non-finite matrix entries");
+    brkga_problem_s* pr;
+    brkga_status st = new_problem(engine, KIND_BITSP_MATRIX, dim - 1, &pr);
+    if (st != BRKGA_OK) return st;
+    DeviceGuard g(engine->device);
+    pr->desc.dim = dim;
+    set_scales(&pr->desc, fixed_shift(max_abs(m0, cells)), fixed_shift(max_abs(m1, cells)));
+    st = upload(pr, m0, sizeof(float) * cells, (const void**)&pr->desc.a);
+    if (st == BRKGA_OK) st = upload(pr, m1, sizeof(float) * cells, (const void**)&pr->desc.b);
+    return finish_problem(pr, st, out);
+}
+
+brkga_status brkga_problem_info(brkga_problem_t problem, int32_t* kind, int32_t* chrom_len, int32_t* num_obj,
+                                int32_t* maximize)
+{
+    if (!problem) return set_error(BRKGA_ERR_INVALID_ARG, "null problem");
+    if (kind) *kind = problem->desc.kind;
+    if (chrom_len) *chrom_len = problem->desc.chrom_len;
+    if (num_obj) *num_obj = problem->desc.num_obj;
+    if (maximize) *maximize = problem->desc.maximize;
+    return BRKGA_OK;
+}
+
+brkga_status brkga_problem_destroy(brkga_problem_t problem)
+{
+    if (!problem) return BRKGA_OK;
+    DeviceGuard g(problem->engine->device);
+    free_all(problem->owned);
+    delete problem;
+    return BRKGA_OK;
+}
+
+static brkga_status evaluate_impl(brkga_problem_t problem, const float* d_keys, float* d_fitness, int* d_decoded,
+                                  int32_t pop_size, int32_t chrom_len, void* stream)
+{
+    if (!problem || !d_keys) return set_error(BRKGA_ERR_INVALID_ARG, "null problem or keys");
+    if (pop_size < 0) return set_error(BRKGA_ERR_INVALID_ARG, "negative pop_size");
+    if (chrom_len != problem->desc.chrom_len)
+        return set_error(BRKGA_ERR_INVALID_ARG, "chrom_len " + std::to_string(chrom_len) + " does not match the problem (" +
+                                                    std::to_string(problem->desc.chrom_len) + ")");
+    if (pop_size == 0) return BRKGA_OK;
+    DeviceGuard g(problem->engine->device);
+    hipError_t e = launch_evaluate(problem->desc, d_keys, chrom_len, pop_size, d_fitness, d_decoded,
+                                   default_block(problem->desc.n), as_stream(stream));
+    if (e != hipSuccess) return hip_error(e, "evaluate launch");
+    return BRKGA_OK;
+}
+
+brkga_status brkga_evaluate(brkga_problem_t problem, const float* d_keys, float* d_fitness, int32_t pop_size,
+                            int32_t chrom_len, void* stream)
+{
+    if (!d_fitness) return set_error(BRKGA_ERR_INVALID_ARG, "null fitness buffer");
+    return evaluate_impl(problem, d_keys, d_fitness, nullptr, pop_size, chrom_len, stream);
+}
+
+brkga_status brkga_decode(brkga_problem_t problem, const float* d_keys, int32_t* d_decoded, int32_t pop_size,
+                          int32_t chrom_len, void* stream)
+{
+    if (!d_decoded) return set_error(BRKGA_ERR_INVALID_ARG, "null decoded buffer");
+    return evaluate_impl(problem, d_keys, nullptr, d_decoded, pop_size, chrom_len, stream);
+}
+
+brkga_status brkga_population_create(brkga_problem_t problem, const brkga_params* params, brkga_pop_t* out)
+{
+    if (!problem || !params || !out) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    auto* pop = new brkga_pop_s();
+    pop->problem = problem;
+    pop->params = *params;
+    pop->d_desc = nullptr;
+    pop->capture_stream = nullptr;
+    DeviceGuard g(problem->engine->device);
+    brkga_status st = fill_inst_desc(problem, params, &pop->host, &pop->nsga2);
+    if (st == BRKGA_OK) st = alloc_inst_buffers(&pop->host, pop->nsga2, &pop->owned);
+    void* p = nullptr;
+    if (st == BRKGA_OK && (st = device_alloc(&p, sizeof(InstDesc), "descriptor")) == BRKGA_OK) {
+        pop->owned.push_back(p);
+        pop->d_desc = (InstDesc*)p;
+    }
+    if (st == BRKGA_OK && (st = device_alloc(&p, sizeof(int) * pop->host.L, "decode row")) == BRKGA_OK) {
+        pop->owned.push_back(p);
+        pop->d_decode_row = (int*)p;
+    }
+    if (st == BRKGA_OK && (st = device_alloc(&p, sizeof(float) * 2, "fitness row")) == BRKGA_OK) {
+        pop->owned.push_back(p);
+        pop->d_fit_row = (float*)p;
+    }
+    if (st == BRKGA_OK) {
+        const int n = problem->desc.n;
+        pop->block = default_block(n);
+        pop->step_lds = step_lds_bytes(pop->host);
+        pop->part_block = partition_block(pop->host.Pi);
+        pop->part_lds = part_lds_bytes(pop->host);
+        if ((size_t)pop->part_lds > problem->engine->lds_limit)
+            st = set_error(BRKGA_ERR_INVALID_ARG, "island population too large for the LDS partition");
+    }
+    if (st == BRKGA_OK) {
+        hipError_t e = hipMemcpy(pop->d_desc, &pop->host, sizeof(InstDesc), hipMemcpyHostToDevice);
+        if (e != hipSuccess) st = hip_error(e, "hipMemcpy(descriptor)");
+    }
+    if (st == BRKGA_OK) {
+        hipError_t e = hipStreamCreateWithFlags(&pop->capture_stream, hipStreamNonBlocking);
+        if (e != hipSuccess) st = hip_error(e, "hipStreamCreate");
+    }
+    if (st != BRKGA_OK) {
+        if (pop->capture_stream) (void)hipStreamDestroy(pop->capture_stream);
+        free_all(pop->owned);
+        delete pop;
+        return st;
+    }
+    *out = pop;
+    return BRKGA_OK;
+}
+
+brkga_status brkga_population_destroy(brkga_pop_t pop)
+{
+    if (!pop) return BRKGA_OK;
+    DeviceGuard g(pop->problem->engine->device);
+    (void)hipDeviceSynchronize();
+    for (auto& kv : pop->graphs) (void)hipGraphExecDestroy(kv.second);
+    if (pop->capture_stream) (void)hipStreamDestroy(pop->capture_stream);
+    free_all(pop->owned);
+    delete pop;
+    return BRKGA_OK;
+}
+
+brkga_status brkga_population_initialize(brkga_pop_t pop, void* stream)
+{
+    if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
+    DeviceGuard g(pop->problem->engine->device);
+    hipStream_t s = as_stream(stream);
+    BRKGA_HIP_TRY(hipMemsetAsync(pop->host.gen, 0, 2 * sizeof(int), s));
+    return pop_step(pop, 1, s);
+}
+
+brkga_status brkga_evolve(brkga_pop_t pop, int32_t generations, void* stream)
+{
+    if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
+    if (generations < 0) return set_error(BRKGA_ERR_INVALID_ARG, "negative generation count");
+    DeviceGuard g(pop->problem->engine->device);
+    hipStream_t s = as_stream(stream);
+    for (int i = 0; i < generations; ++i) {
+        brkga_status st = pop_step(pop, 0, s);
+        if (st != BRKGA_OK) return st;
+    }
+    return BRKGA_OK;
+}
+
+brkga_status brkga_evolve_graph(brkga_pop_t pop, int32_t generations, void* stream)
+{
+    if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
+    if (generations < 0) return set_error(BRKGA_ERR_INVALID_ARG, "negative generation count");
+    if (generations == 0) return BRKGA_OK;
+    std::lock_guard<std::mutex> lock(pop->mu);
+    DeviceGuard g(pop->problem->engine->device);
+    auto it = pop->graphs.find(generations);
+    if (it == pop->graphs.end()) {
+        hipGraph_t graph = nullptr;
+        BRKGA_HIP_TRY(hipStreamBeginCapture(pop->capture_stream, hipStreamCaptureModeThreadLocal));
+        brkga_status st = BRKGA_OK;
+        for (int i = 0; i < generations && st == BRKGA_OK; ++i) st = pop_step(pop, 0, pop->capture_stream);
+        hipError_t e = hipStreamEndCapture(pop->capture_stream, &graph);
+        if (st != BRKGA_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return hip_error(e, "hipGraphInstantiate");
+        it = pop->graphs.emplace(generations, exec).first;
+    }
+    BRKGA_HIP_TRY(hipGraphLaunch(it->second, as_stream(stream)));
+    return BRKGA_OK;
+}
+
+brkga_status brkga_population_generation(brkga_pop_t pop, int32_t* generation)
+{
+    if (!pop || !generation) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
+    DeviceGuard g(pop->problem->engine->device);
+    int gen = 0;
+    brkga_status st = read_gen(pop, nullptr, &gen);
+    if (st != BRKGA_OK) return st;
+    *generation = gen;
+    return BRKGA_OK;
+}
+
+brkga_status brkga_population_read(brkga_pop_t pop, float* keys, float* fitness, int32_t* order, int32_t* rank,
+                                   float* crowding, void* stream)
+{
+    if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
+    DeviceGuard g(pop->problem->engine->device);
+    hipStream_t s = as_stream(stream);
+    int gen = 0;
+    brkga_status st = read_gen(pop, s, &gen);
+    if (st != BRKGA_OK) return st;
+    const InstDesc& d = pop->host;
+    const int b = gen & 1;
+    const size_t P = (size_t)d.pop;
+    if (keys)
+        BRKGA_HIP_TRY(hipMemcpy2D(keys, sizeof(float) * d.L, d.keys[b], sizeof(float) * d.ld, sizeof(float) * d.L, P,
+                                  hipMemcpyDeviceToHost));
+    if (fitness)
+        BRKGA_HIP_TRY(hipMemcpy(fitness, d.fit[b], sizeof(float) * P * d.prob.num_obj, hipMemcpyDeviceToHost));
+    if (order) BRKGA_HIP_TRY(hipMemcpy(order, d.perm[b], sizeof(int) * P, hipMemcpyDeviceToHost));
+    if (pop->nsga2) {
+        if (rank) BRKGA_HIP_TRY(hipMemcpy(rank, d.rank[b], sizeof(int) * P, hipMemcpyDeviceToHost));
+        if (crowding) BRKGA_HIP_TRY(hipMemcpy(crowding, d.crowd[b], sizeof(float) * P, hipMemcpyDeviceToHost));
+    }
+    return BRKGA_OK;
+}
+
+brkga_status brkga_population_write(brkga_pop_t pop, const float* keys, void* stream)
+{
+    if (!pop || !keys) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
+    DeviceGuard g(pop->problem->engine->device);
+    hipStream_t s = as_stream(stream);
+    const InstDesc& d = pop->host;
+    BRKGA_HIP_TRY(hipStreamSynchronize(s));
+    BRKGA_HIP_TRY(hipMemcpy2D(d.keys[0], sizeof(float) * d.ld, keys, sizeof(float) * d.L, sizeof(float) * d.L,
+                              (size_t)d.pop, hipMemcpyHostToDevice));
+    BRKGA_HIP_TRY(hipMemsetAsync(d.gen, 0, 2 * sizeof(int), s));
+    hipError_t e = launch_evaluate(d.prob, d.keys[0], d.ld, d.pop, d.fit[0], nullptr, pop->block, s);
+    if (e != hipSuccess) return hip_error(e, "evaluate launch");
+    e = pop->nsga2 ? launch_nsga2_select(pop->d_desc, 1, 1, d.pop, s)
+                   : launch_partition(pop->part_block, pop->d_desc, 1, d.islands, pop->part_lds, 1, s);
+    if (e != hipSuccess) return hip_error(e, "partition launch");
+    return BRKGA_OK;
+}
+
+brkga_status brkga_get_best(brkga_pop_t pop, float* fitness, float* keys, int32_t* decoded, void* stream)
+{
+    if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
+    DeviceGuard g(pop->problem->engine->device);
+    hipStream_t s = as_stream(stream);
+    int gen = 0;
+    brkga_status st = read_gen(pop, s, &gen);
+    if (st != BRKGA_OK) return st;
+    const InstDesc& d = pop->host;
+    const int b = gen & 1;
+    const int P = d.pop, M = d.prob.num_obj;
+    std::vector<float> fit((size_t)P * M);
+    BRKGA_HIP_TRY(hipMemcpy(fit.data(), d.fit[b], sizeof(float) * fit.size(), hipMemcpyDeviceToHost));
+    int best = -1;
+    if (pop->nsga2) {
+        std::vector<int> rank(P);
+        std::vector<float> crowd(P);
+        BRKGA_HIP_TRY(hipMemcpy(rank.data(), d.rank[b], sizeof(int) * P, hipMemcpyDeviceToHost));
+        BRKGA_HIP_TRY(hipMemcpy(crowd.data(), d.crowd[b], sizeof(float) * P, hipMemcpyDeviceToHost));
+        for (int i = 0; i < P; ++i)
+            if (rank[i] == 0 && (best < 0 || crowd[i] > crowd[best])) best = i;
+    } else {
+        std::vector<int> perm(P);
+        BRKGA_HIP_TRY(hipMemcpy(perm.data(), d.perm[b], sizeof(int) * P, hipMemcpyDeviceToHost));
+        for (int i = 0; i < d.islands; ++i) {
+            const int cand = i * d.Pi + perm[(size_t)i * d.Pi];
+            if (best < 0) {
+                best = cand;
+                continue;
+            }
+            const float a = fit[cand], c = fit[best];
+            if (d.prob.maximize ? a > c : a < c) best = cand;
+        }
+    }
+    if (best < 0) return set_error(BRKGA_ERR_DEVICE, "no best individual found");
+    if (fitness)
+        for (int m = 0; m < M; ++m) fitness[m] = fit[(size_t)best * M + m];
+    const float* row = d.keys[b] + (size_t)best * d.ld;
+    if (keys) BRKGA_HIP_TRY(hipMemcpy(keys, row, sizeof(float) * d.L, hipMemcpyDeviceToHost));
+    if (decoded) {
+        hipError_t e = launch_evaluate(d.prob, row, d.ld, 1, pop->d_fit_row, pop->d_decode_row, pop->block, s);
+        if (e != hipSuccess) return hip_error(e, "decode launch");
+        BRKGA_HIP_TRY(hipStreamSynchronize(s));
+        BRKGA_HIP_TRY(hipMemcpy(decoded, pop->d_decode_row, sizeof(int) * d.L, hipMemcpyDeviceToHost));
+    }
+    return BRKGA_OK;
+}
+
+brkga_status brkga_population_device_buffers(brkga_pop_t pop, float** d_keys, float** d_fitness, int32_t** d_order,
+                                             int32_t* ld)
+{
+    if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
+    DeviceGuard g(pop->problem->engine->device);
+    int gen = 0;
+    brkga_status st = read_gen(pop, nullptr, &gen);
+    if (st != BRKGA_OK) return st;
+    const int b = gen & 1;
+    if (d_keys) *d_keys = pop->host.keys[b];
+    if (d_fitness) *d_fitness = pop->host.fit[b];
+    if (d_order) *d_order = pop->host.perm[b];
+    if (ld) *ld = pop->host.ld;
+    return BRKGA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// capi_internal.hpp - host-side objects behind the C ABI handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/brkga_hip.h"
+#include "engine_types.hpp"
+
+namespace brkga {
+
+brkga_status set_error(brkga_status st, const std::string& msg);
+brkga_status hip_error(hipError_t e, const char* what);
+
+#define BRKGA_HIP_TRY(expr)                                          \
+    do {                                                             \
+        hipError_t brkga_e_ = (expr);                                \
+        if (brkga_e_ != hipSuccess) return ::brkga::hip_error(brkga_e_, #expr); \
+    } while (0)
+
+// Device allocation that reports OOM distinctly from other failures.
+brkga_status device_alloc(void** p, size_t bytes, const char* what);
+
+int fixed_shift(double max_abs_term);
+
+}  // namespace brkga
+
+struct brkga_engine_s {
+    int device;
+    size_t lds_limit;  // bytes of LDS one workgroup may use
+    int cu_count;
+};
+
+struct brkga_problem_s {
+    brkga_engine_s* engine;
+    brkga::ProblemDesc desc;  // device pointers
+    std::vector<void*> owned;
+};
+
+struct brkga_pop_s {
+    brkga_problem_s* problem;
+    brkga_params params;
+    brkga::InstDesc host;     // descriptor mirror
+    brkga::InstDesc* d_desc;  // descriptor in device memory
+    std::vector<void*> owned;
+    int nsga2;
+    int block;                // workgroup size of the step kernel
+    int step_lds;
+    int part_block;
+    int part_lds;
+    int* d_decode_row;        // scratch for get_best decode
+    float* d_fit_row;
+    hipStream_t capture_stream;
+    std::map<int, hipGraphExec_t> graphs;
+    std::mutex mu;
+};
+
+namespace brkga {
+
+// Validates params against a problem and fills `d` (pointers excluded).
+brkga_status fill_inst_desc(const brkga_problem_s* pr, const brkga_params* q, InstDesc* d, int* nsga2);
+// Allocates the ping-pong buffers and counters described by d.
+brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, std::vector<void*>* owned);
+
+// Enqueues generation 0 for instances (same kind) whose descriptors are in
+// device memory.
+hipError_t enqueue_init(int kind, int nsga2, int block, int step_lds, int part_block, int part_lds,
+                        const InstDesc* d_descs, int num_inst, int total_slots, int total_segs, int max_pop,
+                        hipStream_t s);
+// Enqueues one generation step for the same instances.
+hipError_t enqueue_step(int kind, int nsga2, int block, int step_lds, int part_block, int part_lds,
+                        const InstDesc* d_descs, int num_inst, int total_slots, int total_segs, int max_pop,
+                        hipStream_t s);
+
+int step_lds_bytes(const InstDesc& d);
+int part_lds_bytes(const InstDesc& d);
+
+}  // namespace brkga

@@ -1,0 +1,163 @@
+// decoders.hpp - per-individual decode + fitness, one workgroup per
+// individual, keys already staged in LDS.
+//
+// Reference decoders replaced here (paths relative to the reference's brkga/):
+//   TSP          configs/tsp_config.hpp:180-225 (+ GPU kernel 384-437, which
+//                truncates at 128 cities; not reproduced)
+//   knapsack     configs/knapsack_config.hpp:43-75, knapsack_gpu_config.hpp:242-268
+//   TSPJ         configs/tspj_config.hpp:555-642 (+ GPU kernel 874-982)
+//   bi-obj TSP   configs/multi_tsp_config.hpp:54-103
+// Every edge / term is FP32 as in the reference; sums are order-independent
+// int64 fixed point converted once to FP32, so the result does not depend on
+// the block size.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace brkga {
+
+template <int KIND, int BLOCK>
+__device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout& lay, int nb,
+                             float* fit_out, int* decoded)
+{
+    const int tid = threadIdx.x;
+    const int n = p.n;
+    float* keys = reinterpret_cast<float*>(lds + lay.keys_off);
+    int* perm = reinterpret_cast<int*>(lds + lay.perm_off);
+    long long* red = reinterpret_cast<long long*>(lds + lay.red_off);
+    SortScratch ss;
+    ss.cnt = reinterpret_cast<int*>(lds + lay.cnt_off);
+    ss.items = reinterpret_cast<int*>(lds + lay.items_off);
+    ss.red = reinterpret_cast<int*>(lds + lay.red_off);
+
+    if constexpr (KIND == KIND_TSP_COORDS || KIND == KIND_TSP_MATRIX) {
+        block_argsort<BLOCK>(keys, n, nb, ss, perm);
+        long long acc = 0;
+        const float2* xy = reinterpret_cast<const float2*>(p.a);
+        for (int q = tid; q < n; q += BLOCK) {
+            const int a = perm[q];
+            const int b = perm[q + 1 < n ? q + 1 : 0];
+            float d;
+            if constexpr (KIND == KIND_TSP_COORDS) {
+                const float2 ca = xy[a], cb = xy[b];
+                d = edge_xy(ca.x, ca.y, cb.x, cb.y);
+            } else {
+                d = p.a[(size_t)a * n + b];
+            }
+            acc += fx_term(d, p.scale0);
+        }
+        acc = block_sum_i64<BLOCK>(acc, red);
+        if (tid == 0 && fit_out) fit_out[0] = fx_to_float(acc, p.inv0);
+        if (decoded)
+            for (int q = tid; q < n; q += BLOCK) decoded[q] = perm[q];
+    } else if constexpr (KIND == KIND_KNAP_THRESHOLD) {
+        long long tw = 0, tv = 0;
+        for (int i = tid; i < n; i += BLOCK) {
+            const int take = keys[i] > 0.5f;
+            if (take) {
+                tw += p.w[i];
+                tv += p.v[i];
+            }
+            if (decoded) decoded[i] = take;
+        }
+        tw = block_sum_i64<BLOCK>(tw, red);
+        tv = block_sum_i64<BLOCK>(tv, red);
+        if (tid == 0 && fit_out) {
+            const long long f = tw > p.cap ? tv - 1000LL * (tw - p.cap) : tv;
+            fit_out[0] = __ll2float_rn(f);
+        }
+    } else if constexpr (KIND == KIND_KNAP_GREEDY) {
+        block_argsort<BLOCK>(keys, n, nb, ss, perm);
+        // Wave 0 walks the key order 64 items at a time: the first open lane
+        // whose weight fits is packed, open lanes before it are rejected.
+        if (tid < kWave) {
+            const int lane = tid;
+            long long rem = p.cap, val = 0;
+            for (int base = 0; base < n; base += kWave) {
+                const int q = base + lane;
+                const bool valid = q < n;
+                const int item = valid ? perm[q] : 0;
+                const long long w = valid ? (long long)p.w[item] : 0;
+                const long long v = valid ? (long long)p.v[item] : 0;
+                bool open = valid;
+                bool taken = false;
+                while (true) {
+                    const unsigned long long fits = __ballot(open && w <= rem);
+                    if (fits == 0ull) break;
+                    const int f = __builtin_ctzll(fits);
+                    rem -= __shfl(w, f, kWave);
+                    val += __shfl(v, f, kWave);
+                    taken = taken || lane == f;
+                    open = open && lane > f;
+                }
+                if (decoded && valid) decoded[item] = taken ? 1 : 0;
+            }
+            if (lane == 0 && fit_out) fit_out[0] = __ll2float_rn(val);
+        }
+    } else if constexpr (KIND == KIND_TSPJ) {
+        int* tour = perm;
+        int* jobs = perm + n;
+        block_argsort<BLOCK>(keys, n, nb, ss, tour);
+        block_argsort<BLOCK>(keys + n, n, nb, ss, jobs);
+        const int dim = p.dim;
+        const int chunk = (n + BLOCK - 1) / BLOCK;
+        const int lo = tid * chunk;
+        const int hi = lo + chunk < n ? lo + chunk : n;
+        long long local = 0;
+        for (int q = lo; q < hi; ++q) {
+            const int prev = q == 0 ? 0 : tour[q - 1] + 1;
+            local += fx_term(p.a[(size_t)prev * dim + tour[q] + 1], p.scale0);
+        }
+        long long t = block_inclusive_scan_i64<BLOCK>(local, red) - local;
+        long long best = 0;
+        for (int q = lo; q < hi; ++q) {
+            const int city = tour[q] + 1;
+            const int prev = q == 0 ? 0 : tour[q - 1] + 1;
+            t += fx_term(p.a[(size_t)prev * dim + city], p.scale0);
+            const long long done = t + fx_term(p.b[(size_t)city * dim + jobs[q] + 1], p.scale0);
+            best = done > best ? done : best;
+            if (q == n - 1) {
+                const long long total = t + fx_term(p.a[(size_t)city * dim], p.scale0);
+                best = total > best ? total : best;
+            }
+        }
+        best = block_max_i64<BLOCK>(best, red);
+        if (tid == 0 && fit_out) fit_out[0] = fx_to_float(best, p.inv0);
+        if (decoded)
+            for (int q = tid; q < n; q += BLOCK) {
+                decoded[q] = tour[q] + 1;
+                decoded[n + q] = jobs[q] + 1;
+            }
+    } else if constexpr (KIND == KIND_BITSP_COORDS || KIND == KIND_BITSP_MATRIX) {
+        block_argsort<BLOCK>(keys, n, nb, ss, perm);
+        const int dim = p.dim;
+        long long acc0 = 0, acc1 = 0;
+        for (int q = tid; q <= n; q += BLOCK) {
+            const int from = q == 0 ? 0 : perm[q - 1] + 1;
+            const int to = q == n ? 0 : perm[q] + 1;
+            float d0, d1;
+            if constexpr (KIND == KIND_BITSP_COORDS) {
+                const float2* xa = reinterpret_cast<const float2*>(p.a);
+                const float2* xb = reinterpret_cast<const float2*>(p.b);
+                const float2 a0 = xa[from], a1 = xa[to], b0 = xb[from], b1 = xb[to];
+                d0 = edge_xy(a0.x, a0.y, a1.x, a1.y);
+                d1 = edge_xy(b0.x, b0.y, b1.x, b1.y);
+            } else {
+                d0 = p.a[(size_t)from * dim + to];
+                d1 = p.b[(size_t)from * dim + to];
+            }
+            acc0 += fx_term(d0, p.scale0);
+            acc1 += fx_term(d1, p.scale1);
+        }
+        acc0 = block_sum_i64<BLOCK>(acc0, red);
+        acc1 = block_sum_i64<BLOCK>(acc1, red);
+        if (tid == 0 && fit_out) {
+            fit_out[0] = fx_to_float(acc0, p.inv0);
+            fit_out[1] = fx_to_float(acc1, p.inv1);
+        }
+        if (decoded)
+            for (int q = tid; q < n; q += BLOCK) decoded[q] = perm[q];
+    }
+}
+
+}  // namespace brkga

@@ -1,0 +1,98 @@
+// engine_types.hpp - plain-old-data descriptors shared by host code and
+// kernels.  Kernels receive these by value or read them from device memory,
+// so every field is a scalar or a raw device pointer.
+#pragma once
+
+#include <stdint.h>
+
+namespace brkga {
+
+// Problem kinds; the numbering is part of the C ABI (brkga_hip.h).
+enum ProblemKind : int {
+    KIND_TSP_COORDS = 0,      // configs/tsp_config.hpp (Euclidean, from coordinates)
+    KIND_TSP_MATRIX = 1,      // configs/tsp_config.hpp (explicit n x n matrix)
+    KIND_KNAP_THRESHOLD = 2,  // configs/knapsack_config.hpp:43-75
+    KIND_KNAP_GREEDY = 3,     // greedy fill in key order (no reference counterpart)
+    KIND_TSPJ = 4,            // configs/tspj_config.hpp:555-642
+    KIND_BITSP_COORDS = 5,    // configs/multi_tsp_config.hpp, two coordinate sets
+    KIND_BITSP_MATRIX = 6,    // configs/multi_tsp_config.hpp, two matrices
+    KIND_COUNT = 7
+};
+
+enum RngTag : uint32_t {
+    TAG_PARENTS = 0,
+    TAG_COINS = 1,
+    TAG_MUTANT = 2,
+    TAG_INIT = 3,
+    TAG_MUTCOIN = 4
+};
+
+struct ProblemDesc {
+    int kind;
+    int n;          // cities / items (a depot, where there is one, is extra)
+    int chrom_len;  // keys per individual (TSPJ: 2n)
+    int num_obj;    // 1, or 2 for the bi-objective TSP
+    int dim;        // rows of a / b (coords count or matrix dimension)
+    int maximize;   // 1: larger fitness is better (knapsack)
+    const float* a; // coords [dim][2] or matrix [dim][dim]
+    const float* b; // second matrix / coordinate set
+    const int* w;   // knapsack weights
+    const int* v;   // knapsack values
+    long long cap;  // knapsack capacity
+    float scale0, inv0, scale1, inv1;  // fixed-point 2^S and 2^-S per objective
+};
+
+// One resident population (a BRKGA run of `islands` islands, or one NSGA-II
+// population) as seen by the generation kernels.  Buffers ping-pong on the
+// parity of the step counter *gen: step t reads [t & 1] and writes
+// [(t + 1) & 1].
+struct InstDesc {
+    ProblemDesc prob;
+    float* keys[2];   // [pop][ld]
+    float* fit[2];    // [pop][num_obj]
+    int* perm[2];     // [islands][Pi]: local slot of each rank
+    int* rank[2];     // NSGA-II front index per slot
+    float* crowd[2];  // NSGA-II crowding distance per slot
+    int* gen;         // completed generation steps
+    int* done;        // arrival counter of the per-island partition blocks
+    int L, ld;
+    int pop, Pi, Ei, Mi, islands;
+    int mig_interval, mig_k;
+    uint32_t seed_lo, seed_hi;
+    float rho;
+    float mut_rate;   // NSGA-II per-gene resampling probability
+    int nb;           // argsort buckets for n keys
+    int slot_begin;   // first block of this instance in a step launch
+    int seg_begin;    // first block of this instance in a partition launch
+    int gen_limit;    // steps are skipped once *gen >= gen_limit
+};
+
+__host__ __device__ inline int round_up_int(int v, int m)
+{
+    return (v + m - 1) / m * m;
+}
+
+// LDS carve-up of one decode block.
+struct DecodeLayout {
+    int keys_off, perm_off, items_off, cnt_off, red_off, bytes;
+};
+
+__host__ __device__ inline DecodeLayout decode_layout(int kind, int n, int L, int nb)
+{
+    DecodeLayout d;
+    int off = 0;
+    d.red_off = off;
+    off += 16 * 8;
+    d.keys_off = off;
+    off += round_up_int(L, 32) * 4;
+    d.perm_off = off;
+    off += (kind == KIND_TSPJ ? 2 * n : n) * 4;
+    d.items_off = off;
+    off += n * 4;
+    d.cnt_off = off;
+    off += (nb + 1) * 4;
+    d.bytes = round_up_int(off, 16);
+    return d;
+}
+
+}  // namespace brkga

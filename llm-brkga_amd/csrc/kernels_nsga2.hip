@@ -1,0 +1,426 @@
+// kernels_nsga2.hip - NSGA-II generation on the GPU (two objectives).
+//
+// The reference runs NSGA-II on the host only (core/population.hpp:334-517,
+// core/solver.hpp:751-1091).  Per generation the engine runs:
+//   k_nsga2_offspring  one workgroup per child: two binary tournaments,
+//                      uniform crossover, per-gene resampling, decode of the
+//                      child's two objectives from LDS;
+//   k_nsga2_select     one workgroup per population over parents + children:
+//                      domination counts and front peeling, crowding
+//                      distance per front, survivor order;
+//   k_nsga2_gather     one workgroup per survivor row: copy into the next
+//                      buffer; the last block advances the step counter.
+// Buffers: keys[b] holds 2N rows (parents [0,N), children [N,2N)).
+#include "launch.hpp"
+#include "decoders.hpp"
+
+namespace brkga {
+
+namespace {
+
+__device__ __forceinline__ int find_inst(const InstDesc* __restrict__ d, int num, int b, int per_block_pop)
+{
+    // instances are laid out with slot_begin = prefix of their populations
+    int lo = 0, hi = num - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d[mid].slot_begin <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    (void)per_block_pop;
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t order_key_asc(float f)
+{
+    if (f != f) return 0xFFFFFFFFu;
+    if (f == 0.0f) f = 0.0f;
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ bool dominates2(float a0, float a1, float b0, float b1)
+{
+    // individual.hpp:99-114 for two objectives (minimisation)
+    if (a0 > b0 || a1 > b1) return false;
+    return a0 < b0 || a1 < b1;
+}
+
+template <int BLOCK>
+__device__ void bitonic_u64(unsigned long long* sk, int n)
+{
+    for (int k = 2; k <= n; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < (n >> 1); i += BLOCK) {
+                const int lo = 2 * j * (i / j) + (i % j);
+                const int hi = lo + j;
+                const bool up = (lo & k) == 0;
+                const unsigned long long x = sk[lo], y = sk[hi];
+                if ((x > y) == up) {
+                    sk[lo] = y;
+                    sk[hi] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace
+
+template <int KIND, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_nsga2_offspring(const InstDesc* __restrict__ descs, int num_inst, int init)
+{
+    extern __shared__ __align__(16) char lds[];
+    const int b = blockIdx.x;
+    const InstDesc& d = descs[find_inst(descs, num_inst, b, 0)];
+    const int c = b - d.slot_begin;  // child (or, at init, parent) index
+    const int N = d.pop;
+    if (c >= N) return;
+    const int t = init ? 0 : *d.gen;
+    if (!init && t >= d.gen_limit) return;
+    const int cur = t & 1;
+    const int row = init ? c : N + c;
+    const int ld = d.ld, L = d.L;
+    const int nq = (L + 3) >> 2;
+    const int buf = init ? 0 : cur;
+    float4* out = reinterpret_cast<float4*>(d.keys[buf] + (size_t)row * ld);
+    const int nb = d.nb;
+    const DecodeLayout lay = decode_layout(KIND, d.prob.n, L, nb);
+    float4* kl = reinterpret_cast<float4*>(lds + lay.keys_off);
+
+    if (init) {
+        for (int q = threadIdx.x; q < nq; q += BLOCK) {
+            const U4 r = rng_block(d.seed_lo, d.seed_hi, 0, 0, c, TAG_INIT, q);
+            const float4 x = make_float4(unit_from_u32(r.x), unit_from_u32(r.y), unit_from_u32(r.z),
+                                         unit_from_u32(r.w));
+            out[q] = x;
+            kl[q] = x;
+        }
+    } else {
+        const uint32_t g = (uint32_t)t + 1u;
+        const int* rank = d.rank[cur];
+        const float* crowd = d.crowd[cur];
+        const U4 pr = rng_block(d.seed_lo, d.seed_hi, 0, g, c, TAG_PARENTS, 0);
+        // binary tournaments (solver.hpp:1042-1056)
+        auto tour = [&](int a, int bb) {
+            if (rank[a] < rank[bb]) return a;
+            if (rank[bb] < rank[a]) return bb;
+            if (crowd[a] > crowd[bb]) return a;
+            return bb;
+        };
+        const int p1 = tour((int)(pr.x % (uint32_t)N), (int)(pr.y % (uint32_t)N));
+        const int p2 = tour((int)(pr.z % (uint32_t)N), (int)(pr.w % (uint32_t)N));
+        const float4* a4 = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)p1 * ld);
+        const float4* b4 = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)p2 * ld);
+        const float rho = d.rho, mut = d.mut_rate;
+        for (int q = threadIdx.x; q < nq; q += BLOCK) {
+            const U4 rc = rng_block(d.seed_lo, d.seed_hi, 0, g, c, TAG_COINS, q);
+            const U4 rm = rng_block(d.seed_lo, d.seed_hi, 0, g, c, TAG_MUTCOIN, q);
+            const U4 rk = rng_block(d.seed_lo, d.seed_hi, 0, g, c, TAG_MUTANT, q);
+            const float4 pa = a4[q], pb = b4[q];
+            float4 x;
+            x.x = unit_from_u32(rc.x) < rho ? pa.x : pb.x;
+            x.y = unit_from_u32(rc.y) < rho ? pa.y : pb.y;
+            x.z = unit_from_u32(rc.z) < rho ? pa.z : pb.z;
+            x.w = unit_from_u32(rc.w) < rho ? pa.w : pb.w;
+            x.x = unit_from_u32(rm.x) < mut ? unit_from_u32(rk.x) : x.x;
+            x.y = unit_from_u32(rm.y) < mut ? unit_from_u32(rk.y) : x.y;
+            x.z = unit_from_u32(rm.z) < mut ? unit_from_u32(rk.z) : x.z;
+            x.w = unit_from_u32(rm.w) < mut ? unit_from_u32(rk.w) : x.w;
+            out[q] = x;
+            kl[q] = x;
+        }
+    }
+    __syncthreads();
+    decode_block<KIND, BLOCK>(d.prob, lds, lay, nb, d.fit[buf] + (size_t)row * 2, nullptr);
+}
+
+// LDS layout of the selection block for C = up to 2N individuals.
+struct SelectLayout {
+    int o0, o1, rk, cnt, lst, cr, fst, fs, sk, bytes;
+};
+
+__host__ __device__ inline SelectLayout select_layout(int C)
+{
+    SelectLayout s;
+    const int Cp = pow2_at_least(C < 2 ? 2 : C);
+    int off = 0;
+    s.o0 = off; off += C * 4;
+    s.o1 = off; off += C * 4;
+    s.rk = off; off += C * 4;
+    s.cnt = off; off += C * 4;
+    s.lst = off; off += (C + 1) * 4;
+    s.cr = off; off += C * 4;
+    s.fst = off; off += (C + 2) * 4;
+    s.fs = off; off += (C + 2) * 4;
+    off = round_up_int(off, 8);
+    s.sk = off; off += Cp * 8;
+    s.bytes = round_up_int(off, 16);
+    return s;
+}
+
+constexpr int kSelectBlock = 1024;
+
+__global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* __restrict__ descs, int init)
+{
+    constexpr int B = kSelectBlock;
+    extern __shared__ __align__(16) char lds[];
+    const InstDesc& d = descs[blockIdx.x];
+    const int N = d.pop;
+    const int t = init ? 0 : *d.gen;
+    if (!init && t >= d.gen_limit) return;
+    const int cur = init ? 0 : (t & 1);
+    const int C = init ? N : 2 * N;
+    const int Cp = pow2_at_least(C < 2 ? 2 : C);
+    const SelectLayout L = select_layout(C);
+    float* o0 = reinterpret_cast<float*>(lds + L.o0);
+    float* o1 = reinterpret_cast<float*>(lds + L.o1);
+    int* rk = reinterpret_cast<int*>(lds + L.rk);
+    int* cnt = reinterpret_cast<int*>(lds + L.cnt);
+    int* lst = reinterpret_cast<int*>(lds + L.lst);  // [0] = size, then members
+    float* cr = reinterpret_cast<float*>(lds + L.cr);
+    int* fst = reinterpret_cast<int*>(lds + L.fst);  // first sorted position of each front
+    int* fs = reinterpret_cast<int*>(lds + L.fs);    // front sizes
+    unsigned long long* sk = reinterpret_cast<unsigned long long*>(lds + L.sk);
+
+    const float* fit = d.fit[cur];
+    for (int i = threadIdx.x; i < C; i += B) {
+        o0[i] = fit[2 * i];
+        o1[i] = fit[2 * i + 1];
+        rk[i] = -1;
+    }
+    __syncthreads();
+    // domination counts (population.hpp:409-440)
+    for (int i = threadIdx.x; i < C; i += B) {
+        const float b0 = o0[i], b1 = o1[i];
+        int c = 0;
+        for (int j = 0; j < C; ++j) c += dominates2(o0[j], o1[j], b0, b1) ? 1 : 0;
+        cnt[i] = c;
+        if (c == 0) rk[i] = 0;
+    }
+    __syncthreads();
+    // front peeling (population.hpp:334-369)
+    int r = 0;
+    while (true) {
+        if (threadIdx.x == 0) lst[0] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < C; i += B)
+            if (rk[i] == r) lst[1 + atomicAdd(&lst[0], 1)] = i;
+        __syncthreads();
+        const int m = lst[0];
+        if (m == 0) break;
+        if (threadIdx.x == 0) fs[r] = m;
+        for (int q = threadIdx.x; q < C; q += B) {
+            if (rk[q] >= 0) continue;
+            const float b0 = o0[q], b1 = o1[q];
+            int dec = 0;
+            for (int a = 0; a < m; ++a) {
+                const int p = lst[1 + a];
+                dec += dominates2(o0[p], o1[p], b0, b1) ? 1 : 0;
+            }
+            if (dec) {
+                cnt[q] -= dec;
+                if (cnt[q] == 0) rk[q] = r + 1;
+            }
+        }
+        ++r;
+        __syncthreads();
+    }
+    const int nfront = r;
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int f = 0; f < nfront; ++f) {
+            fst[f] = acc;
+            acc += fs[f];
+        }
+    }
+    // crowding distance (population.hpp:453-489); per objective the front
+    // is ordered by (objective, index)
+    for (int i = threadIdx.x; i < C; i += B) cr[i] = 0.0f;
+    __syncthreads();
+    for (int m = 0; m < 2; ++m) {
+        const float* om = m ? o1 : o0;
+        for (int i = threadIdx.x; i < Cp; i += B)
+            sk[i] = i < C ? ((unsigned long long)rk[i] << 44) | ((unsigned long long)order_key_asc(om[i]) << 12) |
+                                (unsigned long long)i
+                          : ~0ull;
+        __syncthreads();
+        bitonic_u64<B>(sk, Cp);
+        for (int q = threadIdx.x; q < C; q += B) {
+            const int e = (int)(sk[q] & 0xFFFull);
+            const int re = rk[e];
+            const int size = fs[re];
+            if (size <= 2) {
+                if (m == 1) cr[e] = INFINITY;
+                continue;
+            }
+            // the front occupies sorted positions [start, start + size)
+            const int start = fst[re];
+            const bool first = q == start;
+            const bool last = q == start + size - 1;
+            const int lo = (int)(sk[start] & 0xFFFull);
+            const int hi = (int)(sk[start + size - 1] & 0xFFFull);
+            const float range = __fsub_rn(om[hi], om[lo]);
+            // boundaries become infinite; interior members add the normalised
+            // gap between their neighbours when the range is positive
+            float acc = cr[e];
+            if (first || last) {
+                acc = INFINITY;
+            } else if (range > 0.0f) {
+                const float gap = __fsub_rn(om[(int)(sk[q + 1] & 0xFFFull)], om[(int)(sk[q - 1] & 0xFFFull)]);
+                acc = __fadd_rn(acc, __fdiv_rn(gap, range));
+            }
+            cr[e] = acc;
+        }
+        __syncthreads();
+    }
+    // after the m = 1 pass, sk holds (rank, obj1, index) order
+    const int nxt = cur ^ 1;
+    if (init) {
+        for (int i = threadIdx.x; i < C; i += B) {
+            d.rank[0][i] = rk[i];
+            d.crowd[0][i] = cr[i];
+        }
+        return;
+    }
+    // survivor order (population.hpp:491-517)
+    int full = 0, cum = 0;
+    for (int f = 0; f < nfront; ++f) {
+        if (cum + fs[f] <= N) {
+            cum += fs[f];
+            full = f + 1;
+        } else {
+            break;
+        }
+    }
+    int* order = d.perm[cur];
+    // whole fronts: position = lower fronts + position in (obj1, index)
+    // order for fronts of more than two, index order otherwise
+    for (int q = threadIdx.x; q < C; q += B) {
+        const int e = (int)(sk[q] & 0xFFFull);
+        const int re = rk[e];
+        if (re >= full) continue;
+        const int base = fst[re];
+        int pos;
+        if (fs[re] > 2) {
+            pos = q - base;
+        } else if (fs[re] == 2) {
+            const int o = (int)(sk[q == base ? q + 1 : q - 1] & 0xFFFull);
+            pos = o < e ? 1 : 0;
+        } else {
+            pos = 0;
+        }
+        const int out = base + pos;
+        order[out] = e;
+        d.rank[nxt][out] = re;
+        d.crowd[nxt][out] = cr[e];
+    }
+    __syncthreads();
+    if (cum < N) {
+        // overflowing front `full`: crowding descending, ties by index
+        for (int i = threadIdx.x; i < Cp; i += B) {
+            unsigned long long key = ~0ull;
+            if (i < C && rk[i] == full) {
+                const uint32_t ok = ~order_key_asc(cr[i]);
+                key = ((unsigned long long)ok << 12) | (unsigned long long)i;
+            }
+            sk[i] = key;
+        }
+        __syncthreads();
+        bitonic_u64<B>(sk, Cp);
+        for (int q = threadIdx.x; q < N - cum; q += B) {
+            const int e = (int)(sk[q] & 0xFFFull);
+            order[cum + q] = e;
+            d.rank[nxt][cum + q] = full;
+            d.crowd[nxt][cum + q] = cr[e];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_nsga2_gather(const InstDesc* __restrict__ descs, int num_inst)
+{
+    const int b = blockIdx.x;
+    const InstDesc& d = descs[find_inst(descs, num_inst, b, 0)];
+    const int i = b - d.slot_begin;
+    const int N = d.pop;
+    if (i >= N) return;
+    const int t = *d.gen;
+    if (t >= d.gen_limit) return;
+    const int cur = t & 1, nxt = cur ^ 1;
+    const int src = d.perm[cur][i];
+    const int nq = (d.L + 3) >> 2;
+    const float4* in = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)src * d.ld);
+    float4* out = reinterpret_cast<float4*>(d.keys[nxt] + (size_t)i * d.ld);
+    for (int q = threadIdx.x; q < nq; q += 256) out[q] = in[q];
+    if (threadIdx.x < 2) d.fit[nxt][2 * i + threadIdx.x] = d.fit[cur][2 * src + threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int old = atomicAdd(d.done, 1);
+        if (old == N - 1) {
+            *d.done = 0;
+            *d.gen = t + 1;
+        }
+    }
+}
+
+// ------------------------------------------------------------- launchers --
+
+namespace {
+
+template <int KIND, int BLOCK>
+hipError_t off_one(const InstDesc* d, int num, int blocks, int lds, int init, hipStream_t s)
+{
+    if (lds > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_nsga2_offspring<KIND, BLOCK>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    hipLaunchKernelGGL((k_nsga2_offspring<KIND, BLOCK>), dim3(blocks), dim3(BLOCK), lds, s, d, num, init);
+    return hipGetLastError();
+}
+
+template <int KIND>
+hipError_t off_kind(int block, const InstDesc* d, int num, int blocks, int lds, int init, hipStream_t s)
+{
+    switch (block) {
+    case 64: return off_one<KIND, 64>(d, num, blocks, lds, init, s);
+    case 128: return off_one<KIND, 128>(d, num, blocks, lds, init, s);
+    case 256: return off_one<KIND, 256>(d, num, blocks, lds, init, s);
+    case 512: return off_one<KIND, 512>(d, num, blocks, lds, init, s);
+    case 1024: return off_one<KIND, 1024>(d, num, blocks, lds, init, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_nsga2_offspring(int kind, int block, const InstDesc* d_descs, int num_inst, int blocks, int lds,
+                                  int init, hipStream_t s)
+{
+    if (blocks <= 0) return hipSuccess;
+    switch (kind) {
+    case KIND_BITSP_COORDS: return off_kind<KIND_BITSP_COORDS>(block, d_descs, num_inst, blocks, lds, init, s);
+    case KIND_BITSP_MATRIX: return off_kind<KIND_BITSP_MATRIX>(block, d_descs, num_inst, blocks, lds, init, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_nsga2_select(const InstDesc* d_descs, int num_inst, int init, int max_pop, hipStream_t s)
+{
+    if (num_inst <= 0) return hipSuccess;
+    const int lds = select_layout(init ? max_pop : 2 * max_pop).bytes;
+    if (lds > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_nsga2_select),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    hipLaunchKernelGGL(k_nsga2_select, dim3(num_inst), dim3(kSelectBlock), lds, s, d_descs, init);
+    return hipGetLastError();
+}
+
+hipError_t launch_nsga2_gather(const InstDesc* d_descs, int num_inst, int total_rows, hipStream_t s)
+{
+    if (total_rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nsga2_gather, dim3(total_rows), dim3(256), 0, s, d_descs, num_inst);
+    return hipGetLastError();
+}
+
+}  // namespace brkga

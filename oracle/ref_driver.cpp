@@ -1,0 +1,82 @@
+// ref_driver.cpp - C entry points over the reference's own CUDA-free headers.
+//
+// TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile (target `ref`) from
+// the reference sources where they lie (REF_DIR, default the read-only
+// reference checkout) into oracle/_ref/libref.so.  Tests use it to pin the
+// oracle's restatement of:
+//   KnapsackConfig<float>::calculate_knapsack_fitness  configs/knapsack_config.hpp:43-62
+//   KnapsackConfig<float>::decode_to_solution          configs/knapsack_config.hpp:64-75
+//   MultiTSPConfig<float> objectives + decode_tour     configs/multi_tsp_config.hpp:54-103
+//   Individual<float>::dominates                       core/individual.hpp:99-114
+// Nothing here is part of the engine.
+#include "configs/knapsack_config.hpp"
+#include "configs/multi_tsp_config.hpp"
+#include "core/individual.hpp"
+
+#include <cstdint>
+#include <vector>
+
+namespace {
+
+Individual<float> make_individual(const float* keys, int n)
+{
+    Individual<float> ind(n);
+    for (int i = 0; i < n; ++i) ind.chromosomes[0][i] = keys[i];
+    return ind;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Fitness and 0/1 selection of `count` key rows of length n.
+void ref_knapsack_threshold(const float* w, const float* v, int n, float cap,
+                            const float* keys, int count, float* fitness, int32_t* sel)
+{
+    std::vector<float> wv(w, w + n), vv(v, v + n);
+    KnapsackConfig<float> cfg(wv, vv, cap);
+    for (int r = 0; r < count; ++r) {
+        auto ind = make_individual(keys + (size_t)r * n, n);
+        fitness[r] = cfg.fitness_function(ind);
+        auto dec = cfg.decoder(ind);
+        for (int i = 0; i < n; ++i) sel[(size_t)r * n + i] = dec[0][i] > 0.5f ? 1 : 0;
+    }
+}
+
+// Objectives [count][2] and 0-based city order [count][n] for matrices of
+// dimension dim = n + 1 (row/column 0 is the depot).
+void ref_multitsp(const float* dist, const float* time, int dim,
+                  const float* keys, int count, float* obj, int32_t* perm)
+{
+    std::vector<std::vector<float>> d(dim, std::vector<float>(dim)), t(dim, std::vector<float>(dim));
+    for (int i = 0; i < dim; ++i)
+        for (int j = 0; j < dim; ++j) {
+            d[i][j] = dist[(size_t)i * dim + j];
+            t[i][j] = time[(size_t)i * dim + j];
+        }
+    MultiTSPConfig<float> cfg(d, t);
+    int n = dim - 1;
+    for (int r = 0; r < count; ++r) {
+        auto ind = make_individual(keys + (size_t)r * n, n);
+        obj[2 * r] = cfg.objective_functions[0](ind);
+        obj[2 * r + 1] = cfg.objective_functions[1](ind);
+        auto tour = cfg.decode_tour(ind);
+        for (int i = 0; i < n; ++i) perm[(size_t)r * n + i] = tour[i];
+    }
+}
+
+// out[i*count + j] = 1 if individual i dominates individual j.
+void ref_dominates(const float* obj, int count, int nobj, int32_t* out)
+{
+    std::vector<Individual<float>> pop;
+    pop.reserve(count);
+    for (int i = 0; i < count; ++i) {
+        Individual<float> ind(std::vector<int>{1}, nobj);
+        for (int m = 0; m < nobj; ++m) ind.objectives[m] = obj[(size_t)i * nobj + m];
+        pop.push_back(ind);
+    }
+    for (int i = 0; i < count; ++i)
+        for (int j = 0; j < count; ++j) out[(size_t)i * count + j] = pop[i].dominates(pop[j]) ? 1 : 0;
+}
+
+}  // extern "C"
