@@ -196,6 +196,15 @@ brkga_status brkga_get_best(brkga_pop_t pop, float* fitness, float* keys, int32_
 brkga_status brkga_population_device_buffers(brkga_pop_t pop, float** d_keys, float** d_fitness,
                                              int32_t** d_order, int32_t* ld);
 
+/* Runs `generations` plain generation steps with HIP events recorded on
+ * `stream` around every kernel launch and returns the mean duration of the
+ * generation kernel (BRKGA step / NSGA-II offspring) and of the closing
+ * kernels (partition / NSGA-II select + gather), in microseconds.
+ * Replaces the reference's PerformanceStats timers (core/solver.hpp:143-157,
+ * 406-438), which time synchronous launches with std::chrono. */
+brkga_status brkga_population_profile(brkga_pop_t pop, int32_t generations, double* step_us, double* close_us,
+                                      void* stream);
+
 #ifdef __cplusplus
 }
 #endif

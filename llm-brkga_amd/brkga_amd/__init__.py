@@ -116,6 +116,8 @@ _SIGNATURES = {
     "brkga_get_best": ([_P, _PF, _PF, _PI, _P], ctypes.c_int),
     "brkga_population_device_buffers": ([_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P), _PI],
                                         ctypes.c_int),
+    "brkga_population_profile": ([_P, _I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _P],
+                                 ctypes.c_int),
 }
 
 
@@ -303,6 +305,14 @@ class Population:
         dec = np.empty(L, np.int32)
         _check(lib().brkga_get_best(self.handle, _fptr(fit), _fptr(keys), _iptr(dec), stream or None))
         return (fit if M > 1 else float(fit[0])), keys, dec
+
+    def profile(self, generations: int, stream: int = 0):
+        """Mean (generation kernel, closing kernels) durations in microseconds,
+        from HIP events around each launch of `generations` plain steps."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().brkga_population_profile(self.handle, generations, ctypes.byref(a), ctypes.byref(b),
+                                              stream or None))
+        return a.value, b.value
 
     def close(self):
         if self.handle:

@@ -774,4 +774,60 @@ brkga_status brkga_population_device_buffers(brkga_pop_t pop, float** d_keys, fl
     return BRKGA_OK;
 }
 
+brkga_status brkga_population_profile(brkga_pop_t pop, int32_t generations, double* step_us, double* close_us,
+                                      void* stream)
+{
+    if (!pop || generations <= 0) return set_error(BRKGA_ERR_INVALID_ARG, "null population or no generations");
+    DeviceGuard g(pop->problem->engine->device);
+    hipStream_t s = as_stream(stream);
+    const InstDesc& d = pop->host;
+    constexpr int kChunk = 64;
+    hipEvent_t ev[3 * kChunk];
+    for (int i = 0; i < 3 * kChunk; ++i) BRKGA_HIP_TRY(hipEventCreate(&ev[i]));
+    double sum_step = 0.0, sum_close = 0.0;
+    brkga_status st = BRKGA_OK;
+    for (int done = 0; done < generations && st == BRKGA_OK;) {
+        const int m = std::min(kChunk, generations - done);
+        for (int i = 0; i < m && st == BRKGA_OK; ++i) {
+            hipError_t e = hipEventRecord(ev[3 * i], s);
+            if (e == hipSuccess) {
+                if (pop->nsga2) {
+                    e = launch_nsga2_offspring(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, 0, s);
+                } else {
+                    e = launch_brkga_step(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, 0, s);
+                }
+            }
+            if (e == hipSuccess) e = hipEventRecord(ev[3 * i + 1], s);
+            if (e == hipSuccess) {
+                if (pop->nsga2) {
+                    e = launch_nsga2_select(pop->d_desc, 1, 0, d.pop, s);
+                    if (e == hipSuccess) e = launch_nsga2_gather(pop->d_desc, 1, d.pop, s);
+                } else {
+                    e = launch_partition(pop->part_block, pop->d_desc, 1, d.islands, pop->part_lds, 0, s);
+                }
+            }
+            if (e == hipSuccess) e = hipEventRecord(ev[3 * i + 2], s);
+            if (e != hipSuccess) st = hip_error(e, "profiled launch");
+        }
+        if (st == BRKGA_OK) {
+            hipError_t e = hipEventSynchronize(ev[3 * (m - 1) + 2]);
+            if (e != hipSuccess) st = hip_error(e, "hipEventSynchronize");
+        }
+        for (int i = 0; i < m && st == BRKGA_OK; ++i) {
+            float a = 0.f, b = 0.f;
+            hipError_t e = hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]);
+            if (e != hipSuccess) st = hip_error(e, "hipEventElapsedTime");
+            sum_step += a;
+            sum_close += b;
+        }
+        done += m;
+    }
+    for (int i = 0; i < 3 * kChunk; ++i) (void)hipEventDestroy(ev[i]);
+    if (st != BRKGA_OK) return st;
+    if (step_us) *step_us = 1000.0 * sum_step / generations;
+    if (close_us) *close_us = 1000.0 * sum_close / generations;
+    return BRKGA_OK;
+}
+
 }  // extern "C"

@@ -1,0 +1,212 @@
+"""GPU parity: the HIP engine, called through its C ABI, against the oracle.
+
+Bar: keys, rank orders, decoded permutations / item sets and integer
+fitness bit-exact; FP32 tour lengths / makespans / objectives bit-exact to the
+oracle (both use the same FP32 edges and order-independent fixed-point sums),
+which itself stays within 1e-6 relative of FP64 (test_oracle_decode.py).
+"""
+import numpy as np
+import pytest
+import torch
+
+import brkga_amd
+import pyoracle
+from brkga_amd import workloads
+from conftest import random_keys
+
+pytestmark = pytest.mark.gpu
+
+
+def make_problem(engine, kind, n, inst=0):
+    """(engine problem, oracle problem) over identical data."""
+    if kind == "tsp":
+        xy = workloads.tsp_coords(inst, n, 10000.0)
+        return brkga_amd.Problem.tsp_coords(engine, xy), pyoracle.Problem.tsp_coords(xy)
+    if kind == "tsp_matrix":
+        d = workloads.euclid_matrix(workloads.tsp_coords(inst, n))
+        return brkga_amd.Problem.tsp_matrix(engine, d), pyoracle.Problem.tsp_matrix(d)
+    if kind in ("knap_greedy", "knap_threshold"):
+        w, v, cap = workloads.knapsack(inst, n)
+        g = kind == "knap_greedy"
+        return brkga_amd.Problem.knapsack(engine, w, v, cap, g), pyoracle.Problem.knapsack(w, v, cap, g)
+    if kind == "tspj":
+        t, j = workloads.tspj(inst, n)
+        return brkga_amd.Problem.tspj(engine, t, j), pyoracle.Problem.tspj(t, j)
+    if kind == "bitsp":
+        a, b = workloads.bitsp_coords(inst, n)
+        return brkga_amd.Problem.bitsp_coords(engine, a, b), pyoracle.Problem.bitsp_coords(a, b)
+    if kind == "bitsp_matrix":
+        a, b = workloads.bitsp_coords(inst, n)
+        ma, mb = workloads.euclid_matrix(a), workloads.euclid_matrix(b)
+        return brkga_amd.Problem.bitsp_matrix(engine, ma, mb), pyoracle.Problem.bitsp_matrix(ma, mb)
+    raise ValueError(kind)
+
+
+def gpu_evaluate(prob, keys):
+    k = torch.from_numpy(keys).cuda()
+    f = torch.empty((keys.shape[0], prob.num_obj), dtype=torch.float32, device="cuda")
+    d = torch.empty(keys.shape, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    prob.evaluate_device(k.data_ptr(), f.data_ptr(), keys.shape[0])
+    prob.decode_device(k.data_ptr(), d.data_ptr(), keys.shape[0])
+    torch.cuda.synchronize()
+    f = f.cpu().numpy()
+    return (f[:, 0] if prob.num_obj == 1 else f), d.cpu().numpy()
+
+
+EVAL_CASES = [("tsp", n) for n in (1, 2, 3, 52, 127, 128, 129, 1000, 5000)] + [
+    ("tsp_matrix", 1), ("tsp_matrix", 60), ("tsp_matrix", 1000),
+    ("knap_greedy", 1), ("knap_greedy", 63), ("knap_greedy", 500), ("knap_greedy", 2000),
+    ("knap_threshold", 1), ("knap_threshold", 500), ("knap_threshold", 2000),
+    ("tspj", 1), ("tspj", 2), ("tspj", 200), ("tspj", 1000),
+    ("bitsp", 1), ("bitsp", 50), ("bitsp", 500), ("bitsp_matrix", 50)]
+
+
+@pytest.mark.parametrize("kind,n", EVAL_CASES)
+def test_evaluate_and_decode_parity(engine, kind, n):
+    print(f"evaluate parity {kind} n={n}", flush=True)
+    gp, op = make_problem(engine, kind, n, inst=n)
+    L = op.chrom_len
+    for ties in (False, True):
+        keys = random_keys(n * 3 + ties, 48, L, ties=ties)
+        gf, gd = gpu_evaluate(gp, keys)
+        of, od = op.evaluate(keys)
+        assert np.array_equal(gd, od), f"decode mismatch ({kind}, n={n}, ties={ties})"
+        assert np.array_equal(gf, of), f"fitness mismatch ({kind}, n={n}, ties={ties})"
+
+
+def test_evaluate_rejects_wrong_chrom_len(engine):
+    gp, _ = make_problem(engine, "tsp", 10)
+    k = torch.zeros((2, 11), device="cuda")
+    f = torch.zeros(2, device="cuda")
+    st = brkga_amd.lib().brkga_evaluate(gp.handle, k.data_ptr(), f.data_ptr(), 2, 11, None)
+    assert st == brkga_amd.ERR_INVALID_ARG
+
+
+def compare_brkga(pop, ost, ctx):
+    r = pop.read()
+    assert np.array_equal(r["keys"], ost.keys), f"keys differ {ctx}"
+    assert np.array_equal(r["fitness"], ost.fitness), f"fitness differs {ctx}"
+    assert np.array_equal(r["order"], ost.order), f"rank order differs {ctx}"
+
+
+BRKGA_CASES = [
+    # kind, n, P, E, M, rho, islands, interval, k, generations
+    ("tsp", 52, 64, 12, 6, 0.7, 1, 0, 0, 12),
+    ("tsp", 100, 4 * 32, 4 * 6, 4 * 5, 0.7, 4, 3, 2, 10),
+    ("tsp_matrix", 40, 48, 9, 5, 0.65, 1, 0, 0, 8),
+    ("knap_greedy", 300, 64, 12, 6, 0.7, 1, 0, 0, 8),
+    ("knap_threshold", 120, 64, 12, 6, 0.7, 2, 2, 3, 8),
+    ("tspj", 60, 64, 12, 6, 0.75, 1, 0, 0, 8),
+]
+
+
+@pytest.mark.parametrize("case", BRKGA_CASES, ids=lambda c: f"{c[0]}-n{c[1]}-P{c[2]}-I{c[6]}")
+def test_brkga_trajectory_parity(engine, case):
+    kind, n, P, E, M, rho, I, iv, k, G = case
+    print(f"brkga trajectory {case}", flush=True)
+    gp, op = make_problem(engine, kind, n, inst=1)
+    seed = 0xC0FFEE + n
+    pop = brkga_amd.Population(gp, brkga_amd.Params(P, E, M, rho, I, iv, k, seed))
+    pop.initialize()
+    ost = pyoracle.BrkgaState(op, pyoracle.params(P, E, M, rho, I, iv, k, seed, op.maximize))
+    compare_brkga(pop, ost, "after init")
+    for t in range(G):
+        pop.evolve(1)
+        ost.step()
+        compare_brkga(pop, ost, f"after step {t}")
+    assert pop.generation == G
+
+
+def test_full_size_tsp_islands_parity(engine):
+    """n=1000, P=1024 as 4 islands x 256, E=M=38 per island, ring migration
+    of the top 2 every 50 generations: bit-exact over 52 generations, which
+    include the migration at step 50."""
+    print("full-size parity n=1000 P=1024", flush=True)
+    xy = workloads.tsp_coords(0, 1000, 10000.0)
+    gp = brkga_amd.Problem.tsp_coords(engine, xy)
+    op = pyoracle.Problem.tsp_coords(xy)
+    prm = brkga_amd.Params(1024, 152, 152, 0.7, 4, 50, 2, 2024)
+    pop = brkga_amd.Population(gp, prm)
+    pop.initialize()
+    ost = pyoracle.BrkgaState(op, pyoracle.params(1024, 152, 152, 0.7, 4, 50, 2, 2024))
+    compare_brkga(pop, ost, "init")
+    for t in range(52):
+        pop.evolve(1)
+        ost.step()
+        if t in (0, 1, 49, 50, 51):
+            compare_brkga(pop, ost, f"step {t}")
+    compare_brkga(pop, ost, "end")
+
+
+def test_graph_replay_matches_plain(engine):
+    gp, _ = make_problem(engine, "tsp", 200, inst=3)
+    prm = brkga_amd.Params(4 * 64, 4 * 10, 4 * 10, 0.7, 4, 4, 2, 77)
+    a = brkga_amd.Population(gp, prm)
+    b = brkga_amd.Population(gp, prm)
+    a.initialize()
+    b.initialize()
+    a.evolve(15)
+    ra = a.read()
+    # replay a 5-generation graph three times with eager work in between
+    for _ in range(3):
+        b.evolve(5, graph=True)
+        torch.cuda.synchronize()
+        _ = b.read()
+    rb = b.read()
+    for key in ("keys", "fitness", "order"):
+        assert np.array_equal(ra[key], rb[key])
+    assert b.generation == 15
+
+
+def test_best_and_write(engine):
+    gp, op = make_problem(engine, "tsp", 80, inst=4)
+    prm = brkga_amd.Params(64, 12, 6, 0.7, 2, 0, 0, 5)
+    pop = brkga_amd.Population(gp, prm)
+    pop.initialize()
+    pop.evolve(5)
+    f, k, d = pop.best()
+    of, od = op.evaluate(k.reshape(1, -1))
+    assert f == of[0] and np.array_equal(d, od[0])
+    r = pop.read()
+    assert f == r["fitness"].min()
+    # write: arbitrary keys become generation 0 of the oracle's partition
+    keys = random_keys(9, 64, 80)
+    pop.write(keys)
+    r = pop.read()
+    fit, _ = op.evaluate(keys)
+    assert np.array_equal(r["keys"], keys) and np.array_equal(r["fitness"], fit)
+    assert np.array_equal(r["order"], pyoracle.partition(fit, 2, 32, 0))
+    assert pop.generation == 0
+
+
+def test_invalid_params_rejected(engine):
+    gp, _ = make_problem(engine, "tsp", 20)
+    for prm in (brkga_amd.Params(100, 0, 10), brkga_amd.Params(100, 60, 50), brkga_amd.Params(10, 2, 1, 0.7, 3),
+                brkga_amd.Params(100, 20, 10, 1.5), brkga_amd.Params(64, 8, 8, 0.7, 2, 5, 5)):
+        with pytest.raises(brkga_amd.BrkgaError) as e:
+            brkga_amd.Population(gp, prm)
+        assert e.value.status == brkga_amd.ERR_INVALID_ARG
+
+
+def compare_nsga(pop, ost, ctx):
+    r = pop.read()
+    assert np.array_equal(r["keys"], ost.keys), f"keys differ {ctx}"
+    assert np.array_equal(r["fitness"], ost.obj), f"objectives differ {ctx}"
+    assert np.array_equal(r["rank"], ost.rank), f"ranks differ {ctx}"
+    assert np.array_equal(r["crowding"], ost.crowd), f"crowding differs {ctx}"
+
+
+@pytest.mark.parametrize("n,N,G", [(20, 32, 10), (120, 512, 4), (50, 64, 30)])
+def test_nsga2_trajectory_parity(engine, n, N, G):
+    print(f"nsga2 trajectory n={n} N={N}", flush=True)
+    gp, op = make_problem(engine, "bitsp", n, inst=2)
+    seed = 31337 + n
+    pop = brkga_amd.Population(gp, brkga_amd.Params(N, 0, 0, 0.7, 1, 0, 0, seed))
+    pop.initialize()
+    ost = pyoracle.Nsga2State(op, pyoracle.params(N, 0, 0, 0.7, seed=seed))
+    compare_nsga(pop, ost, "init")
+    for t in range(G):
+        pop.evolve(1)
+        ost.step()
+        compare_nsga(pop, ost, f"step {t}")
