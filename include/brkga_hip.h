@@ -149,6 +149,15 @@ brkga_status brkga_evaluate(brkga_problem_t problem, const float* d_keys, float*
 brkga_status brkga_decode(brkga_problem_t problem, const float* d_keys, int32_t* d_decoded, int32_t pop_size,
                           int32_t chrom_len, void* stream);
 
+/* Host-buffer form of brkga_evaluate + brkga_decode for a few rows: copies
+ * keys[count][chrom_len] to the device, decodes and evaluates them there and
+ * copies fitness[count][num_obj] / decoded[count][chrom_len] back (either
+ * may be NULL).  Backs the host-side fitness_function / decoder callbacks of
+ * the configs (core/config.hpp:33-34), which the reference evaluates with
+ * its CPU code.  Synchronous. */
+brkga_status brkga_evaluate_host(brkga_problem_t problem, const float* keys, int32_t count, float* fitness,
+                                 int32_t* decoded);
+
 /* -------------------------------------------------------- populations --- */
 
 /* Device-resident population: ping-pong key buffers [P][ld], fitness,

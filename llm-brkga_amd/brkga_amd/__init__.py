@@ -105,6 +105,7 @@ _SIGNATURES = {
     "brkga_problem_destroy": ([_P], ctypes.c_int),
     "brkga_evaluate": ([_P, _P, _P, _I32, _I32, _P], ctypes.c_int),
     "brkga_decode": ([_P, _P, _P, _I32, _I32, _P], ctypes.c_int),
+    "brkga_evaluate_host": ([_P, _PF, _I32, _PF, _PI], ctypes.c_int),
     "brkga_population_create": ([_P, ctypes.POINTER(CParams), ctypes.POINTER(_P)], ctypes.c_int),
     "brkga_population_destroy": ([_P], ctypes.c_int),
     "brkga_population_initialize": ([_P, _P], ctypes.c_int),
@@ -240,6 +241,14 @@ class Problem:
     def evaluate_device(self, d_keys: int, d_fitness: int, pop_size: int, stream: int = 0):
         """evaluate_population_gpu on caller-owned device pointers (ints)."""
         _check(lib().brkga_evaluate(self.handle, d_keys, d_fitness, pop_size, self.chrom_len, stream or None))
+
+    def evaluate(self, keys):
+        """Fitness [rows(, num_obj)] and decoded solutions [rows][chrom_len] of host keys."""
+        keys = np.ascontiguousarray(keys, dtype=np.float32).reshape(-1, self.chrom_len)
+        fit = np.empty((keys.shape[0], self.num_obj), np.float32)
+        dec = np.empty(keys.shape, np.int32)
+        _check(lib().brkga_evaluate_host(self.handle, _fptr(keys), keys.shape[0], _fptr(fit), _iptr(dec)))
+        return (fit[:, 0] if self.num_obj == 1 else fit), dec
 
     def decode_device(self, d_keys: int, d_decoded: int, pop_size: int, stream: int = 0):
         _check(lib().brkga_decode(self.handle, d_keys, d_decoded, pop_size, self.chrom_len, stream or None))

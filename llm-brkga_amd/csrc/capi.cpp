@@ -541,6 +541,32 @@ brkga_status brkga_decode(brkga_problem_t problem, const float* d_keys, int32_t*
     return evaluate_impl(problem, d_keys, nullptr, d_decoded, pop_size, chrom_len, stream);
 }
 
+brkga_status brkga_evaluate_host(brkga_problem_t problem, const float* keys, int32_t count, float* fitness,
+                                 int32_t* decoded)
+{
+    if (!problem || !keys || count < 0) return set_error(BRKGA_ERR_INVALID_ARG, "null problem / keys or negative count");
+    if (count == 0) return BRKGA_OK;
+    DeviceGuard g(problem->engine->device);
+    const ProblemDesc& p = problem->desc;
+    const size_t nk = (size_t)count * p.chrom_len;
+    void *dk = nullptr, *df = nullptr, *dd = nullptr;
+    brkga_status st = device_alloc(&dk, nk * sizeof(float), "keys");
+    if (st == BRKGA_OK) st = device_alloc(&df, (size_t)count * p.num_obj * sizeof(float), "fitness");
+    if (st == BRKGA_OK) st = device_alloc(&dd, nk * sizeof(int), "decoded");
+    hipError_t e = hipSuccess;
+    if (st == BRKGA_OK) e = hipMemcpy(dk, keys, nk * sizeof(float), hipMemcpyHostToDevice);
+    if (st == BRKGA_OK && e == hipSuccess)
+        e = launch_evaluate(p, (const float*)dk, p.chrom_len, count, (float*)df, (int*)dd, default_block(p.n), nullptr);
+    if (st == BRKGA_OK && e == hipSuccess) e = hipDeviceSynchronize();
+    if (st == BRKGA_OK && e == hipSuccess && fitness)
+        e = hipMemcpy(fitness, df, (size_t)count * p.num_obj * sizeof(float), hipMemcpyDeviceToHost);
+    if (st == BRKGA_OK && e == hipSuccess && decoded) e = hipMemcpy(decoded, dd, nk * sizeof(int), hipMemcpyDeviceToHost);
+    if (st == BRKGA_OK && e != hipSuccess) st = hip_error(e, "evaluate_host");
+    for (void* q : {dk, df, dd})
+        if (q) (void)hipFree(q);
+    return st;
+}
+
 brkga_status brkga_population_create(brkga_problem_t problem, const brkga_params* params, brkga_pop_t* out)
 {
     if (!problem || !params || !out) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");

@@ -75,13 +75,45 @@ __device__ __forceinline__ float fx_to_float(long long acc, float inv_scale)
     return __fmul_rn(__ll2float_rn(acc), inv_scale);
 }
 
+// Correctly rounded FP32 square root of a finite x >= 0.  The hardware
+// estimate is within one ulp; the exact nearest float is found by comparing
+// x with the squared midpoints to the neighbouring floats in FP64 (a 25-bit
+// midpoint squares exactly in 53 bits, and no midpoint square equals a
+// float, so there are no ties).
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+    float y = __builtin_sqrtf(x);
+    if (!(x > 0.0f) || !(x < 3.0e38f)) return y;
+    const double xd = (double)x;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const float up = __uint_as_float(__float_as_uint(y) + 1u);
+        const double mu = 0.5 * ((double)y + (double)up);
+        if (mu * mu < xd) {
+            y = up;
+            continue;
+        }
+        const float dn = __uint_as_float(__float_as_uint(y) - 1u);
+        const double md = 0.5 * ((double)y + (double)dn);
+        if (md * md > xd) {
+            y = dn;
+            continue;
+        }
+        break;
+    }
+    return y;
+}
+
 // Euclidean edge in FP32 with separately rounded products and sum
 // (configs/tsp_config.hpp:366-368), correctly rounded square root.
 __device__ __forceinline__ float edge_xy(float xa, float ya, float xb, float yb)
 {
-    const float dx = __fsub_rn(xa, xb);
-    const float dy = __fsub_rn(ya, yb);
-    return __fsqrt_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)));
+#pragma clang fp contract(off)
+    const float dx = xa - xb;
+    const float dy = ya - yb;
+    const float sx = dx * dx;
+    const float sy = dy * dy;
+    return sqrt_rn(sx + sy);
 }
 
 // ------------------------------------------------------------ wave / block --
