@@ -247,7 +247,8 @@ brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, std::vector<void*>* owne
         }
     }
     void* c;
-    brkga_status st = device_alloc(&c, 2 * sizeof(int), "counters");
+    const size_t counters = 2;
+    brkga_status st = device_alloc(&c, counters * sizeof(int), "counters");
     if (st != BRKGA_OK) return st;
     owned->push_back(c);
     d->gen = (int*)c;
@@ -258,48 +259,9 @@ brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, std::vector<void*>* owne
         hipError_t e = hipMemset(d->keys[b], 0, rows * d->ld * sizeof(float));
         if (e != hipSuccess) return hip_error(e, "hipMemset(keys)");
     }
-    hipError_t e = hipMemset(c, 0, 2 * sizeof(int));
+    hipError_t e = hipMemset(c, 0, counters * sizeof(int));
     if (e != hipSuccess) return hip_error(e, "hipMemset(counters)");
     return BRKGA_OK;
-}
-
-int step_lds_bytes(const InstDesc& d)
-{
-    return decode_layout(d.prob.kind, d.prob.n, d.L, d.nb).bytes;
-}
-
-int part_lds_bytes(const InstDesc& d)
-{
-    return pow2_at_least(d.Pi) * 8;
-}
-
-hipError_t enqueue_init(int kind, int nsga2, int block, int step_lds, int part_block, int part_lds,
-                        const InstDesc* d_descs, int num_inst, int total_slots, int total_segs, int max_pop,
-                        hipStream_t s)
-{
-    hipError_t e;
-    if (nsga2) {
-        if ((e = launch_nsga2_offspring(kind, block, d_descs, num_inst, total_slots, step_lds, 1, s)) != hipSuccess)
-            return e;
-        return launch_nsga2_select(d_descs, num_inst, 1, max_pop, s);
-    }
-    if ((e = launch_brkga_step(kind, block, d_descs, num_inst, total_slots, step_lds, 1, s)) != hipSuccess) return e;
-    return launch_partition(part_block, d_descs, num_inst, total_segs, part_lds, 1, s);
-}
-
-hipError_t enqueue_step(int kind, int nsga2, int block, int step_lds, int part_block, int part_lds,
-                        const InstDesc* d_descs, int num_inst, int total_slots, int total_segs, int max_pop,
-                        hipStream_t s)
-{
-    hipError_t e;
-    if (nsga2) {
-        if ((e = launch_nsga2_offspring(kind, block, d_descs, num_inst, total_slots, step_lds, 0, s)) != hipSuccess)
-            return e;
-        if ((e = launch_nsga2_select(d_descs, num_inst, 0, max_pop, s)) != hipSuccess) return e;
-        return launch_nsga2_gather(d_descs, num_inst, total_slots, s);
-    }
-    if ((e = launch_brkga_step(kind, block, d_descs, num_inst, total_slots, step_lds, 0, s)) != hipSuccess) return e;
-    return launch_partition(part_block, d_descs, num_inst, total_segs, part_lds, 0, s);
 }
 
 }  // namespace brkga
@@ -313,13 +275,20 @@ hipStream_t as_stream(void* s)
     return reinterpret_cast<hipStream_t>(s);
 }
 
-brkga_status pop_step(brkga_pop_t pop, int init, hipStream_t s)
+// Enqueues generation 0 (init) or one step whose parent generation has
+// buffer parity `cur`.
+brkga_status pop_step(brkga_pop_t pop, int init, int cur, hipStream_t s)
 {
     const InstDesc& d = pop->host;
-    hipError_t e = init ? enqueue_init(d.prob.kind, pop->nsga2, pop->block, pop->step_lds, pop->part_block,
-                                       pop->part_lds, pop->d_desc, 1, d.pop, d.islands, d.pop, s)
-                        : enqueue_step(d.prob.kind, pop->nsga2, pop->block, pop->step_lds, pop->part_block,
-                                       pop->part_lds, pop->d_desc, 1, d.pop, d.islands, d.pop, s);
+    hipError_t e;
+    if (pop->nsga2) {
+        e = launch_nsga2_offspring(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, init, s);
+        if (e == hipSuccess) e = launch_nsga2_select(pop->d_desc, 1, init, d.pop, s);
+        if (e == hipSuccess && !init) e = launch_nsga2_gather(pop->d_desc, 1, d.pop, s);
+    } else {
+        e = launch_step_one(d, pop->block, cur, init, s);
+        if (e == hipSuccess) e = launch_partition_one(d, init ? 0 : cur ^ 1, init ? 0 : 1, s);
+    }
     if (e != hipSuccess) return hip_error(e, "kernel launch");
     return BRKGA_OK;
 }
@@ -595,11 +564,12 @@ brkga_status brkga_population_create(brkga_problem_t problem, const brkga_params
     if (st == BRKGA_OK) {
         const int n = problem->desc.n;
         pop->block = default_block(n);
-        pop->step_lds = step_lds_bytes(pop->host);
-        pop->part_block = partition_block(pop->host.Pi);
-        pop->part_lds = part_lds_bytes(pop->host);
-        if ((size_t)pop->part_lds > problem->engine->lds_limit)
-            st = set_error(BRKGA_ERR_INVALID_ARG, "island population too large for the LDS partition");
+        pop->host_t = 0;
+        pop->step_lds = pop->nsga2 ? decode_layout(problem->desc.kind, n, pop->host.L, pop->host.nb).bytes
+                                   : step_lds_bytes(pop->host);
+        if ((size_t)partition_lds_bytes(pop->host.Pi) > problem->engine->lds_limit ||
+            (size_t)pop->step_lds > problem->engine->lds_limit)
+            st = set_error(BRKGA_ERR_INVALID_ARG, "population or chromosome too large for the LDS kernels");
     }
     if (st == BRKGA_OK) {
         hipError_t e = hipMemcpy(pop->d_desc, &pop->host, sizeof(InstDesc), hipMemcpyHostToDevice);
@@ -637,7 +607,8 @@ brkga_status brkga_population_initialize(brkga_pop_t pop, void* stream)
     DeviceGuard g(pop->problem->engine->device);
     hipStream_t s = as_stream(stream);
     BRKGA_HIP_TRY(hipMemsetAsync(pop->host.gen, 0, 2 * sizeof(int), s));
-    return pop_step(pop, 1, s);
+    pop->host_t = 0;
+    return pop_step(pop, 1, 0, s);
 }
 
 brkga_status brkga_evolve(brkga_pop_t pop, int32_t generations, void* stream)
@@ -647,8 +618,9 @@ brkga_status brkga_evolve(brkga_pop_t pop, int32_t generations, void* stream)
     DeviceGuard g(pop->problem->engine->device);
     hipStream_t s = as_stream(stream);
     for (int i = 0; i < generations; ++i) {
-        brkga_status st = pop_step(pop, 0, s);
+        brkga_status st = pop_step(pop, 0, pop->host_t & 1, s);
         if (st != BRKGA_OK) return st;
+        ++pop->host_t;
     }
     return BRKGA_OK;
 }
@@ -660,12 +632,14 @@ brkga_status brkga_evolve_graph(brkga_pop_t pop, int32_t generations, void* stre
     if (generations == 0) return BRKGA_OK;
     std::lock_guard<std::mutex> lock(pop->mu);
     DeviceGuard g(pop->problem->engine->device);
-    auto it = pop->graphs.find(generations);
+    const std::pair<int, int> key(generations, pop->host_t & 1);
+    auto it = pop->graphs.find(key);
     if (it == pop->graphs.end()) {
         hipGraph_t graph = nullptr;
         BRKGA_HIP_TRY(hipStreamBeginCapture(pop->capture_stream, hipStreamCaptureModeThreadLocal));
         brkga_status st = BRKGA_OK;
-        for (int i = 0; i < generations && st == BRKGA_OK; ++i) st = pop_step(pop, 0, pop->capture_stream);
+        for (int i = 0; i < generations && st == BRKGA_OK; ++i)
+            st = pop_step(pop, 0, (pop->host_t + i) & 1, pop->capture_stream);
         hipError_t e = hipStreamEndCapture(pop->capture_stream, &graph);
         if (st != BRKGA_OK) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -676,9 +650,10 @@ brkga_status brkga_evolve_graph(brkga_pop_t pop, int32_t generations, void* stre
         e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return hip_error(e, "hipGraphInstantiate");
-        it = pop->graphs.emplace(generations, exec).first;
+        it = pop->graphs.emplace(key, exec).first;
     }
     BRKGA_HIP_TRY(hipGraphLaunch(it->second, as_stream(stream)));
+    pop->host_t += generations;
     return BRKGA_OK;
 }
 
@@ -728,10 +703,10 @@ brkga_status brkga_population_write(brkga_pop_t pop, const float* keys, void* st
     BRKGA_HIP_TRY(hipMemcpy2D(d.keys[0], sizeof(float) * d.ld, keys, sizeof(float) * d.L, sizeof(float) * d.L,
                               (size_t)d.pop, hipMemcpyHostToDevice));
     BRKGA_HIP_TRY(hipMemsetAsync(d.gen, 0, 2 * sizeof(int), s));
+    pop->host_t = 0;
     hipError_t e = launch_evaluate(d.prob, d.keys[0], d.ld, d.pop, d.fit[0], nullptr, pop->block, s);
     if (e != hipSuccess) return hip_error(e, "evaluate launch");
-    e = pop->nsga2 ? launch_nsga2_select(pop->d_desc, 1, 1, d.pop, s)
-                   : launch_partition(pop->part_block, pop->d_desc, 1, d.islands, pop->part_lds, 1, s);
+    e = pop->nsga2 ? launch_nsga2_select(pop->d_desc, 1, 1, d.pop, s) : launch_partition_one(d, 0, 0, s);
     if (e != hipSuccess) return hip_error(e, "partition launch");
     return BRKGA_OK;
 }
@@ -815,13 +790,12 @@ brkga_status brkga_population_profile(brkga_pop_t pop, int32_t generations, doub
     for (int done = 0; done < generations && st == BRKGA_OK;) {
         const int m = std::min(kChunk, generations - done);
         for (int i = 0; i < m && st == BRKGA_OK; ++i) {
+            const int cur = pop->host_t & 1;
             hipError_t e = hipEventRecord(ev[3 * i], s);
             if (e == hipSuccess) {
-                if (pop->nsga2) {
-                    e = launch_nsga2_offspring(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, 0, s);
-                } else {
-                    e = launch_brkga_step(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, 0, s);
-                }
+                e = pop->nsga2 ? launch_nsga2_offspring(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds,
+                                                        0, s)
+                               : launch_step_one(d, pop->block, cur, 0, s);
             }
             if (e == hipSuccess) e = hipEventRecord(ev[3 * i + 1], s);
             if (e == hipSuccess) {
@@ -829,9 +803,10 @@ brkga_status brkga_population_profile(brkga_pop_t pop, int32_t generations, doub
                     e = launch_nsga2_select(pop->d_desc, 1, 0, d.pop, s);
                     if (e == hipSuccess) e = launch_nsga2_gather(pop->d_desc, 1, d.pop, s);
                 } else {
-                    e = launch_partition(pop->part_block, pop->d_desc, 1, d.islands, pop->part_lds, 0, s);
+                    e = launch_partition_one(d, cur ^ 1, 1, s);
                 }
             }
+            ++pop->host_t;
             if (e == hipSuccess) e = hipEventRecord(ev[3 * i + 2], s);
             if (e != hipSuccess) st = hip_error(e, "profiled launch");
         }

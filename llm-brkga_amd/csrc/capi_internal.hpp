@@ -49,13 +49,12 @@ struct brkga_pop_s {
     std::vector<void*> owned;
     int nsga2;
     int block;                // workgroup size of the step kernel
-    int step_lds;
-    int part_block;
-    int part_lds;
+    int step_lds;             // LDS bytes of one step block
+    int host_t;               // generation steps enqueued since initialize
     int* d_decode_row;        // scratch for get_best decode
     float* d_fit_row;
     hipStream_t capture_stream;
-    std::map<int, hipGraphExec_t> graphs;
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (steps, start parity)
     std::mutex mu;
 };
 
@@ -65,18 +64,5 @@ namespace brkga {
 brkga_status fill_inst_desc(const brkga_problem_s* pr, const brkga_params* q, InstDesc* d, int* nsga2);
 // Allocates the ping-pong buffers and counters described by d.
 brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, std::vector<void*>* owned);
-
-// Enqueues generation 0 for instances (same kind) whose descriptors are in
-// device memory.
-hipError_t enqueue_init(int kind, int nsga2, int block, int step_lds, int part_block, int part_lds,
-                        const InstDesc* d_descs, int num_inst, int total_slots, int total_segs, int max_pop,
-                        hipStream_t s);
-// Enqueues one generation step for the same instances.
-hipError_t enqueue_step(int kind, int nsga2, int block, int step_lds, int part_block, int part_lds,
-                        const InstDesc* d_descs, int num_inst, int total_slots, int total_segs, int max_pop,
-                        hipStream_t s);
-
-int step_lds_bytes(const InstDesc& d);
-int part_lds_bytes(const InstDesc& d);
 
 }  // namespace brkga

@@ -16,9 +16,13 @@
 
 namespace brkga {
 
-template <int KIND, int BLOCK>
+// CLEARED: the argsort counters were cleared (and a barrier passed) by the
+// caller.  XY_LDS: coordinates were staged in LDS at xy0 / xy1 (coordinate
+// kinds only); otherwise they are read from p.a / p.b.
+template <int KIND, int BLOCK, bool CLEARED = false, bool XY_LDS = false>
 __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout& lay, int nb,
-                             float* fit_out, int* decoded)
+                             float* fit_out, int* decoded, const float2* xy0 = nullptr,
+                             const float2* xy1 = nullptr)
 {
     const int tid = threadIdx.x;
     const int n = p.n;
@@ -27,27 +31,42 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
     long long* red = reinterpret_cast<long long*>(lds + lay.red_off);
     SortScratch ss;
     ss.cnt = reinterpret_cast<int*>(lds + lay.cnt_off);
-    ss.items = reinterpret_cast<int*>(lds + lay.items_off);
-    ss.red = reinterpret_cast<int*>(lds + lay.red_off);
+    ss.items = reinterpret_cast<unsigned long long*>(lds + lay.items_off);
+    ss.red = red;
 
     if constexpr (KIND == KIND_TSP_COORDS || KIND == KIND_TSP_MATRIX) {
-        block_argsort<BLOCK>(keys, n, nb, ss, perm);
-        long long acc = 0;
-        const float2* xy = reinterpret_cast<const float2*>(p.a);
-        for (int q = tid; q < n; q += BLOCK) {
-            const int a = perm[q];
-            const int b = perm[q + 1 < n ? q + 1 : 0];
-            float d;
-            if constexpr (KIND == KIND_TSP_COORDS) {
-                const float2 ca = xy[a], cb = xy[b];
-                d = edge_xy(ca.x, ca.y, cb.x, cb.y);
-            } else {
-                d = p.a[(size_t)a * n + b];
-            }
-            acc += fx_term(d, p.scale0);
+        block_argsort<BLOCK, CLEARED>(keys, n, nb, ss, perm);
+        // thread t sums the edges leaving positions [t*chunk, t*chunk+chunk)
+        const int chunk = (n + BLOCK - 1) / BLOCK;
+        const int p0 = tid * chunk;
+        const int m = p0 < n ? (p0 + chunk < n ? chunk : n - p0) : 0;
+        int pv[kSortItems + 1];
+#pragma unroll
+        for (int k = 0; k <= kSortItems; ++k) {
+            const int p = p0 + k;
+            if (k <= m && m > 0) pv[k] = perm[p < n ? p : p - n];
         }
-        acc = block_sum_i64<BLOCK>(acc, red);
-        if (tid == 0 && fit_out) fit_out[0] = fx_to_float(acc, p.inv0);
+        FxAcc acc;
+        if constexpr (KIND == KIND_TSP_COORDS) {
+            const float2* xy = XY_LDS ? xy0 : reinterpret_cast<const float2*>(p.a);
+            float2 c[kSortItems + 1];
+#pragma unroll
+            for (int k = 0; k <= kSortItems; ++k)
+                if (k <= m && m > 0) c[k] = xy[pv[k]];
+#pragma unroll
+            for (int k = 0; k < kSortItems; ++k)
+                if (k < m) fx_add(acc, edge_xy(c[k].x, c[k].y, c[k + 1].x, c[k + 1].y), p.scale0);
+        } else {
+            float dd[kSortItems];
+#pragma unroll
+            for (int k = 0; k < kSortItems; ++k)
+                if (k < m) dd[k] = p.a[(size_t)pv[k] * n + pv[k + 1]];
+#pragma unroll
+            for (int k = 0; k < kSortItems; ++k)
+                if (k < m) fx_add(acc, dd[k], p.scale0);
+        }
+        const long long tot = block_sum_fx<BLOCK>(fx_value(acc), ss);
+        if (tid == 0 && fit_out) fit_out[0] = fx_to_float(tot, p.inv0);
         if (decoded)
             for (int q = tid; q < n; q += BLOCK) decoded[q] = perm[q];
     } else if constexpr (KIND == KIND_KNAP_THRESHOLD) {
@@ -67,7 +86,7 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
             fit_out[0] = __ll2float_rn(f);
         }
     } else if constexpr (KIND == KIND_KNAP_GREEDY) {
-        block_argsort<BLOCK>(keys, n, nb, ss, perm);
+        block_argsort<BLOCK, CLEARED>(keys, n, nb, ss, perm);
         // Wave 0 walks the key order 64 items at a time: the first open lane
         // whose weight fits is packed, open lanes before it are rejected.
         if (tid < kWave) {
@@ -97,7 +116,7 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
     } else if constexpr (KIND == KIND_TSPJ) {
         int* tour = perm;
         int* jobs = perm + n;
-        block_argsort<BLOCK>(keys, n, nb, ss, tour);
+        block_argsort<BLOCK, CLEARED>(keys, n, nb, ss, tour);
         block_argsort<BLOCK>(keys + n, n, nb, ss, jobs);
         const int dim = p.dim;
         const int chunk = (n + BLOCK - 1) / BLOCK;
@@ -129,7 +148,7 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
                 decoded[n + q] = jobs[q] + 1;
             }
     } else if constexpr (KIND == KIND_BITSP_COORDS || KIND == KIND_BITSP_MATRIX) {
-        block_argsort<BLOCK>(keys, n, nb, ss, perm);
+        block_argsort<BLOCK, CLEARED>(keys, n, nb, ss, perm);
         const int dim = p.dim;
         long long acc0 = 0, acc1 = 0;
         for (int q = tid; q <= n; q += BLOCK) {
@@ -137,8 +156,8 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
             const int to = q == n ? 0 : perm[q] + 1;
             float d0, d1;
             if constexpr (KIND == KIND_BITSP_COORDS) {
-                const float2* xa = reinterpret_cast<const float2*>(p.a);
-                const float2* xb = reinterpret_cast<const float2*>(p.b);
+                const float2* xa = XY_LDS ? xy0 : reinterpret_cast<const float2*>(p.a);
+                const float2* xb = XY_LDS ? xy1 : reinterpret_cast<const float2*>(p.b);
                 const float2 a0 = xa[from], a1 = xa[to], b0 = xb[from], b1 = xb[to];
                 d0 = edge_xy(a0.x, a0.y, a1.x, a1.y);
                 d1 = edge_xy(b0.x, b0.y, b1.x, b1.y);

@@ -30,10 +30,10 @@ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t lo0 = 0xD2511F53u * c0;
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
-        const uint32_t lo1 = 0xCD9E8D57u * c2;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0;  // one v_mad_u64_u32 each
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+        const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
         const uint32_t n0 = hi1 ^ c1 ^ k0;
         const uint32_t n2 = hi0 ^ c3 ^ k1;
         c0 = n0;
@@ -75,32 +75,42 @@ __device__ __forceinline__ float fx_to_float(long long acc, float inv_scale)
     return __fmul_rn(__ll2float_rn(acc), inv_scale);
 }
 
-// Correctly rounded FP32 square root of a finite x >= 0.  The hardware
-// estimate is within one ulp; the exact nearest float is found by comparing
-// x with the squared midpoints to the neighbouring floats in FP64 (a 25-bit
-// midpoint squares exactly in 53 bits, and no midpoint square equals a
-// float, so there are no ties).
+// Correctly rounded FP32 square root of a finite x >= 0.
+//
+// y = hardware estimate (within one ulp).  With u = ulp(y) and u' the ulp
+// below y, the exact residual d = x - y^2 is a multiple of u^2 (x and y^2
+// both are) below 2^25 u^2 in magnitude, so fl(d) = fma(-y, y, x) is exact
+// whenever |d| < 2^24 u^2 and keeps its order against y*u (< 2^24 u^2,
+// exact) otherwise.  sqrt(x) lies above the midpoint y + u/2 iff d > y*u,
+// and below y - u'/2 iff d <= -y*u' (midpoint squares are never floats, so
+// there are no ties).  V_SQRT_F32 is accurate to one ulp, so one correction
+// suffices; inputs whose ulp would be subnormal take the FP64 midpoint test.
 __device__ __forceinline__ float sqrt_rn(float x)
 {
-    float y = __builtin_sqrtf(x);
-    if (!(x > 0.0f) || !(x < 3.0e38f)) return y;
-    const double xd = (double)x;
+    float y = __builtin_amdgcn_sqrtf(x);
+    if (!(x > 0.0f) || !(x < 3.0e38f)) return __builtin_sqrtf(x);
+    if (x < 1.0e-30f) {
+        const double xd = (double)x;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const float up = __uint_as_float(__float_as_uint(y) + 1u);
-        const double mu = 0.5 * ((double)y + (double)up);
-        if (mu * mu < xd) {
-            y = up;
-            continue;
+        for (int i = 0; i < 2; ++i) {
+            const float up = __uint_as_float(__float_as_uint(y) + 1u);
+            const double mu = 0.5 * ((double)y + (double)up);
+            if (mu * mu < xd) {
+                y = up;
+                continue;
+            }
+            const float dn = __uint_as_float(__float_as_uint(y) - 1u);
+            const double md = 0.5 * ((double)y + (double)dn);
+            if (md * md > xd) y = dn;
         }
-        const float dn = __uint_as_float(__float_as_uint(y) - 1u);
-        const double md = 0.5 * ((double)y + (double)dn);
-        if (md * md > xd) {
-            y = dn;
-            continue;
-        }
-        break;
+        return y;
     }
+    const uint32_t b = __float_as_uint(y);
+    const float u = __uint_as_float((b & 0x7F800000u) - (23u << 23));  // ulp(y)
+    const float ud = (b & 0x007FFFFFu) ? u : __uint_as_float((b & 0x7F800000u) - (24u << 23));
+    const float d = __builtin_fmaf(-y, y, x);
+    if (d > y * u) y = __uint_as_float(b + 1u);
+    else if (d <= -(y * ud)) y = __uint_as_float(b - 1u);
     return y;
 }
 
@@ -231,7 +241,34 @@ __device__ __forceinline__ long long block_inclusive_scan_i64(long long v, long 
     return base + inc;
 }
 
+// --------------------------------------------------------- fixed point ----
+
+// value = hi * 2^24 + lo: 32-bit partial sums of at most 127 terms each.
+struct FxAcc {
+    int hi = 0, lo = 0;
+};
+
+// Adds llrint(t * 2^S) for |t * 2^S| <= 2^48: the value splits exactly into
+// a multiple of 2^24 and a remainder below 2^24; only the remainder can need
+// rounding, and only when the whole value is below 2^24 (then hi = 0).
+__device__ __forceinline__ void fx_add(FxAcc& a, float t, float scale)
+{
+    const float x = t * scale;
+    const float h = __builtin_truncf(x * 0x1p-24f);
+    const float l = __builtin_fmaf(-h, 0x1p24f, x);
+    a.hi += (int)h;
+    a.lo += (int)__builtin_rintf(l);
+}
+
+__device__ __forceinline__ long long fx_value(const FxAcc& a)
+{
+    return ((long long)a.hi << 24) + (long long)a.lo;
+}
+
 // ------------------------------------------------------------- argsort ----
+
+// Most keys one thread handles in the block argsort and decoders.
+constexpr int kSortItems = 16;
 
 // Bucket of a key for nb (power of two) value buckets over [0,1); any other
 // value is clamped to the first/last bucket, which keeps the map monotone.
@@ -243,58 +280,173 @@ __device__ __forceinline__ int key_bucket(float k, int nb)
     return b < nb - 1 ? b : nb - 1;
 }
 
+// Order-preserving FP32 -> u32 (-0 == +0, NaN last).
+__device__ __forceinline__ uint32_t key_order(float k)
+{
+    if (k != k) return 0xFFFFFFFFu;
+    if (k == 0.0f) k = 0.0f;
+    const uint32_t u = __float_as_uint(k);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
 struct SortScratch {
-    int* cnt;    // [nb + 1]
-    int* items;  // [n]
-    int* red;    // [BLOCK/64]
+    int* cnt;                   // [nb + 1] bucket counts, then bucket starts
+    unsigned long long* items;  // [n] (key order << 32 | index) in bucket order
+    long long* red;             // [17] per-wave partials; red[16] = block sum
 };
+
+// Clears the bucket counters and the block-sum slot; a barrier must follow.
+template <int BLOCK>
+__device__ __forceinline__ void argsort_clear(const SortScratch& s, int nb)
+{
+    for (int i = threadIdx.x; i <= nb; i += BLOCK) s.cnt[i] = 0;
+    if (threadIdx.x == 0) s.red[16] = 0;
+}
+
+// Exclusive scan of cnt[0..nb) in place (bucket starts), cnt[nb] = n.
+template <int BLOCK>
+__device__ __forceinline__ void scan_bucket_counts(int* cnt, int nb, int n, int* red)
+{
+    constexpr int W = BLOCK / kWave;
+    const int tid = threadIdx.x;
+    const int per = nb >= BLOCK ? nb / BLOCK : 1;
+    const int lo = tid * per;
+    int local = 0;
+    if (lo < nb) {
+        if ((per & 3) == 0) {
+            const int4* c4 = reinterpret_cast<const int4*>(cnt + lo);
+            for (int k = 0; k < (per >> 2); ++k) {
+                const int4 v = c4[k];
+                local += v.x + v.y + v.z + v.w;
+            }
+        } else {
+            for (int k = 0; k < per; ++k) local += cnt[lo + k];
+        }
+    }
+    const int incl = wave_inclusive_scan_i32(local);
+    int base = 0;
+    if (W > 1) {
+        const int wid = tid / kWave;
+        if ((tid & (kWave - 1)) == kWave - 1) red[wid] = incl;
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < W; ++w) base += w < wid ? red[w] : 0;
+    }
+    int run = base + incl - local;
+    if (lo < nb) {
+        for (int k = 0; k < per; ++k) {
+            const int c = cnt[lo + k];
+            cnt[lo + k] = run;
+            run += c;
+        }
+    }
+    if (tid == 0) cnt[nb] = n;
+    __syncthreads();
+}
+
+__device__ __forceinline__ void cswap(unsigned long long& a, unsigned long long& b)
+{
+    const unsigned long long lo = a < b ? a : b;
+    b = a < b ? b : a;
+    a = lo;
+}
 
 // perm[p] = index of the element at position p of the stable ascending order
 // of keys[0..n) (equal keys keep ascending index order, i.e. std::sort on
 // pair<float,int>(key, index) as configs/tsp_config.hpp:209-216 does).
-// keys and perm are LDS arrays; the result is ready after the final barrier.
-template <int BLOCK>
+//
+// Bucket sort: keys are counted into nb value buckets (LDS atomics), the
+// counts are scanned into bucket ranges, every key drops its
+// (order key, index) item into its range, and the first claimant of each
+// bucket sorts that bucket in registers and writes its ranks.  keys and
+// perm live in LDS; the result is ready after the final barrier.
+// CLEARED: the caller already ran argsort_clear and a barrier.
+template <int BLOCK, bool CLEARED = false>
 __device__ void block_argsort(const float* keys, int n, int nb, const SortScratch& s, int* perm)
 {
     const int tid = threadIdx.x;
-    for (int i = tid; i <= nb; i += BLOCK) s.cnt[i] = 0;
-    __syncthreads();
-    for (int e = tid; e < n; e += BLOCK) perm[e] = atomicAdd(&s.cnt[key_bucket(keys[e], nb)], 1);
-    __syncthreads();
-    // exclusive scan of cnt[0..nb) in place, chunked per thread
-    const int chunk = (nb + BLOCK - 1) / BLOCK;
-    const int lo = tid * chunk;
-    int local = 0;
-    for (int i = 0; i < chunk; ++i)
-        if (lo + i < nb) local += s.cnt[lo + i];
-    int total;
-    int run = block_exclusive_scan_i32<BLOCK>(local, s.red, &total);
-    for (int i = 0; i < chunk; ++i) {
-        if (lo + i < nb) {
-            const int c = s.cnt[lo + i];
-            s.cnt[lo + i] = run;
-            run += c;
+    if (!CLEARED) {
+        argsort_clear<BLOCK>(s, nb);
+        __syncthreads();
+    }
+    int bkt[kSortItems], slot[kSortItems];
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int e = tid + i * BLOCK;
+        if (e < n) {
+            bkt[i] = key_bucket(keys[e], nb);
+            slot[i] = atomicAdd(&s.cnt[bkt[i]], 1);
         }
     }
-    if (tid == 0) s.cnt[nb] = n;
     __syncthreads();
-    for (int e = tid; e < n; e += BLOCK) s.items[s.cnt[key_bucket(keys[e], nb)] + perm[e]] = e;
+    scan_bucket_counts<BLOCK>(s.cnt, nb, n, reinterpret_cast<int*>(s.red));
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int e = tid + i * BLOCK;
+        if (e < n)
+            s.items[s.cnt[bkt[i]] + slot[i]] = ((unsigned long long)key_order(keys[e]) << 32) | (unsigned)e;
+    }
     __syncthreads();
-    for (int e = tid; e < n; e += BLOCK) {
-        const float k = keys[e];
-        const int b = key_bucket(k, nb);
-        const int st = s.cnt[b], en = s.cnt[b + 1];
-        int r = st;
-        if (en - st > 1) {
-            for (int q = st; q < en; ++q) {
-                const int j = s.items[q];
-                const float kj = keys[j];
-                r += (kj < k) || (kj == k && j < e);
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int e = tid + i * BLOCK;
+        if (e < n && slot[i] == 0) {
+            const int st = s.cnt[bkt[i]];
+            const int sz = s.cnt[bkt[i] + 1] - st;
+            if (sz == 1) {
+                perm[st] = e;
+            } else if (sz <= 4) {
+                unsigned long long v0 = s.items[st], v1 = s.items[st + 1];
+                unsigned long long v2 = sz > 2 ? s.items[st + 2] : ~0ull;
+                unsigned long long v3 = sz > 3 ? s.items[st + 3] : ~0ull;
+                cswap(v0, v1);
+                cswap(v2, v3);
+                cswap(v0, v2);
+                cswap(v1, v3);
+                cswap(v1, v2);
+                perm[st] = (int)(uint32_t)v0;
+                perm[st + 1] = (int)(uint32_t)v1;
+                if (sz > 2) perm[st + 2] = (int)(uint32_t)v2;
+                if (sz > 3) perm[st + 3] = (int)(uint32_t)v3;
+            } else if (sz <= 8) {
+                unsigned long long v[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) v[k] = k < sz ? s.items[st + k] : ~0ull;
+                // Batcher's odd-even merge network for 8 (19 exchanges)
+                cswap(v[0], v[1]); cswap(v[2], v[3]); cswap(v[4], v[5]); cswap(v[6], v[7]);
+                cswap(v[0], v[2]); cswap(v[1], v[3]); cswap(v[4], v[6]); cswap(v[5], v[7]);
+                cswap(v[1], v[2]); cswap(v[5], v[6]);
+                cswap(v[0], v[4]); cswap(v[1], v[5]); cswap(v[2], v[6]); cswap(v[3], v[7]);
+                cswap(v[2], v[4]); cswap(v[3], v[5]);
+                cswap(v[1], v[2]); cswap(v[3], v[4]); cswap(v[5], v[6]);
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    if (k < sz) perm[st + k] = (int)(uint32_t)v[k];
+            } else {
+                // rare large bucket: rank every member by counting
+                for (int q = st; q < st + sz; ++q) {
+                    const unsigned long long x = s.items[q];
+                    int r = 0;
+                    for (int p = st; p < st + sz; ++p) r += s.items[p] < x;
+                    perm[st + r] = (int)(uint32_t)x;
+                }
             }
         }
-        perm[r] = e;
     }
     __syncthreads();
+}
+
+// Block sum of one int64 per thread; the result is valid in thread 0.
+// Uses s.red[16], cleared by argsort_clear.
+template <int BLOCK>
+__device__ __forceinline__ long long block_sum_fx(long long v, const SortScratch& s)
+{
+    v = wave_sum_i64(v);
+    if (BLOCK == kWave) return v;
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(&s.red[16]), (unsigned long long)v);
+    __syncthreads();
+    return s.red[16];
 }
 
 __host__ __device__ inline int pow2_at_least(int v)

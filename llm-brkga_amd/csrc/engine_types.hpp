@@ -77,22 +77,52 @@ struct DecodeLayout {
     int keys_off, perm_off, items_off, cnt_off, red_off, bytes;
 };
 
+// LDS of a generation-step block: the decode layout, then the staged
+// coordinates, then the staged rank order of the block's island.
+struct StepLayout {
+    DecodeLayout dec;
+    int xy_off, isl_off, bytes;
+};
+
+// Bytes of coordinates a decoder of this kind can stage in LDS.
+__host__ __device__ inline int coords_bytes(int kind, int n)
+{
+    if (kind == KIND_TSP_COORDS) return 8 * n;
+    if (kind == KIND_BITSP_COORDS) return 16 * (n + 1);
+    return 0;
+}
+
 __host__ __device__ inline DecodeLayout decode_layout(int kind, int n, int L, int nb)
 {
     DecodeLayout d;
     int off = 0;
     d.red_off = off;
-    off += 16 * 8;
+    off += 17 * 8;
+    off = round_up_int(off, 16);
+    d.items_off = off;
+    off += n * 8;
+    off = round_up_int(off, 16);
     d.keys_off = off;
     off += round_up_int(L, 32) * 4;
     d.perm_off = off;
-    off += (kind == KIND_TSPJ ? 2 * n : n) * 4;
-    d.items_off = off;
-    off += n * 4;
+    off += round_up_int(kind == KIND_TSPJ ? 2 * n : n, 4) * 4 + 16;
     d.cnt_off = off;
-    off += (nb + 1) * 4;
+    off += (nb + 4) * 4;
     d.bytes = round_up_int(off, 16);
     return d;
+}
+
+// Rank orders longer than this are read from global memory, not staged.
+constexpr int kMaxStagedOrder = 2048;
+
+__host__ __device__ inline StepLayout step_layout(int kind, int n, int L, int nb, int Pi)
+{
+    StepLayout s;
+    s.dec = decode_layout(kind, n, L, nb);
+    s.xy_off = s.dec.bytes;
+    s.isl_off = s.xy_off + round_up_int(coords_bytes(kind, n), 16);
+    s.bytes = s.isl_off + round_up_int(4 * (Pi <= kMaxStagedOrder ? Pi : 0), 16);
+    return s;
 }
 
 }  // namespace brkga

@@ -1,15 +1,22 @@
 // kernels_brkga.hip - BRKGA generation step for gfx950.
 //
-//   k_evaluate      decode + fitness of caller-owned key rows: the
-//                   evaluate_population_gpu hook (core/config.hpp:50-53).
-//   k_brkga_step    one workgroup per child slot: elite / migrant row copy,
-//                   mutant keys from Philox, or biased uniform crossover,
-//                   then decode + fitness of the new row straight from LDS.
-//                   Replaces brkga_generation_kernel (core/cuda_kernels.cuh:88-138)
-//                   fused with the config fitness kernels, and the host-staged
-//                   ring migration (core/solver.hpp:517-558).
-//   k_partition     stable sort of each island's fitness -> rank order;
-//                   replaces thrust::sequence + sort_by_key (core/solver.hpp:416-419).
+//   k_evaluate       decode + fitness of caller-owned key rows: the
+//                    evaluate_population_gpu hook (core/config.hpp:50-53).
+//   k_step_*         one workgroup per child slot: elite / migrant row copy,
+//                    mutant keys from Philox, or biased uniform crossover,
+//                    then decode + fitness of the new row straight from LDS.
+//                    Replaces brkga_generation_kernel (core/cuda_kernels.cuh:88-138)
+//                    fused with the config fitness kernels, and the host-staged
+//                    ring migration (core/solver.hpp:517-558).
+//   k_partition_*    stable sort of each island's fitness -> rank order;
+//                    replaces thrust::sequence + sort_by_key (core/solver.hpp:416-419).
+//
+// The *_one kernels take the population descriptor by value (kernel
+// arguments), the *_many kernels read one descriptor per instance from
+// device memory (batched and served populations).  `cur` is the buffer
+// parity of the parent generation, known to the launcher; the generation
+// counter in device memory only feeds the Philox counter and the migration
+// schedule, so a captured graph can be replayed.
 //
 // Key rows are [pop][ld] FP32 with ld a multiple of 32, so every row starts
 // 128-byte aligned and is streamed as float4.
@@ -50,6 +57,11 @@ __device__ __forceinline__ uint32_t fitness_order_key(float f, int maximize)
     return maximize ? ~u : u;
 }
 
+__device__ __forceinline__ float4 unit4(const U4& r)
+{
+    return make_float4(unit_from_u32(r.x), unit_from_u32(r.y), unit_from_u32(r.z), unit_from_u32(r.w));
+}
+
 template <int KIND, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_evaluate(ProblemDesc p, const float* __restrict__ keys, long long ld,
                                                     int count, float* __restrict__ fit, int* __restrict__ decoded)
@@ -64,135 +76,214 @@ __global__ __launch_bounds__(BLOCK) void k_evaluate(ProblemDesc p, const float* 
     const float* src = keys + (size_t)row * ld;
     for (int j = threadIdx.x; j < L; j += BLOCK) k[j] = src[j];
     __syncthreads();
-    decode_block<KIND, BLOCK>(p, lds, lay, nb, fit + (size_t)row * p.num_obj,
+    decode_block<KIND, BLOCK>(p, lds, lay, nb, fit ? fit + (size_t)row * p.num_obj : nullptr,
                               decoded ? decoded + (size_t)row * L : nullptr);
 }
 
+// One block of a generation step: slot gs of the population described by d.
 template <int KIND, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_brkga_step(const InstDesc* __restrict__ descs, int num_inst, int init)
+__device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, int init, char* lds)
 {
-    extern __shared__ __align__(16) char lds[];
-    const int b = blockIdx.x;
-    const InstDesc& d = descs[find_instance_by_slot(descs, num_inst, b)];
-    const int gs = b - d.slot_begin;
-    if (gs >= d.pop) return;
-    const int t = init ? 0 : *d.gen;
-    if (!init && t >= d.gen_limit) return;
-
+    constexpr bool kXY = KIND == KIND_TSP_COORDS;
     const int Pi = d.Pi, Ei = d.Ei, Mi = d.Mi, I = d.islands;
     const int island = gs / Pi;
     const int s = gs - island * Pi;
-    const int cur = t & 1;
     const int nxt = init ? 0 : cur ^ 1;
     const int M = d.prob.num_obj;
-    const int ld = d.ld, L = d.L;
+    const int ld = d.ld, L = d.L, n = d.prob.n;
     const int nq = (L + 3) >> 2;
     float4* out = reinterpret_cast<float4*>(d.keys[nxt] + (size_t)gs * ld);
     float* fit_out = d.fit[nxt] + (size_t)gs * M;
+    const int* perm_cur = d.perm[cur] + (size_t)island * Pi;
+    const int k = d.mig_k;
+    const bool maybe_migrant = !init && I > 1 && k > 0 && d.mig_interval > 0 && s >= Pi - k;
+
+    // elite slot that no migration can claim: copy row + fitness, no decode
+    if (!init && !maybe_migrant && s < Ei) {
+        const int src = island * Pi + perm_cur[s];
+        const float4* in = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)src * ld);
+        for (int c = threadIdx.x; c < nq; c += BLOCK) out[c] = in[c];
+        if (threadIdx.x < M) fit_out[threadIdx.x] = d.fit[cur][(size_t)src * M + threadIdx.x];
+        return;
+    }
+
+    const int t = init ? 0 : *d.gen;
     const int nb = d.nb;
-    const DecodeLayout lay = decode_layout(KIND, d.prob.n, L, nb);
-    float4* kl = reinterpret_cast<float4*>(lds + lay.keys_off);
+    const StepLayout sl = step_layout(KIND, n, L, nb, Pi);
+    float4* kl = reinterpret_cast<float4*>(lds + sl.dec.keys_off);
+    float2* xy = reinterpret_cast<float2*>(lds + sl.xy_off);
+    int* order = reinterpret_cast<int*>(lds + sl.isl_off);
+    const bool staged = Pi <= kMaxStagedOrder;
+    SortScratch ss;
+    ss.cnt = reinterpret_cast<int*>(lds + sl.dec.cnt_off);
+    ss.red = reinterpret_cast<long long*>(lds + sl.dec.red_off);
+
+    // independent loads first: coordinates (kept in registers until the
+    // keys are built), island rank order; then clear the argsort counters
+    float2 cxy[kXY ? kSortItems : 1];
+    if constexpr (kXY) {
+        const float2* gxy = reinterpret_cast<const float2*>(d.prob.a);
+#pragma unroll
+        for (int k = 0; k < kSortItems; ++k) {
+            const int i = threadIdx.x + k * BLOCK;
+            if (i < n) cxy[k] = gxy[i];
+        }
+    }
+    if (!init && staged)
+        for (int i = threadIdx.x; i < Pi; i += BLOCK) order[i] = perm_cur[i];
+    argsort_clear<BLOCK>(ss, nb);
+    __syncthreads();
 
     if (init) {
         for (int c = threadIdx.x; c < nq; c += BLOCK) {
-            const U4 r = rng_block(d.seed_lo, d.seed_hi, island, 0, s, TAG_INIT, c);
-            const float4 x = make_float4(unit_from_u32(r.x), unit_from_u32(r.y), unit_from_u32(r.z),
-                                         unit_from_u32(r.w));
+            const float4 x = unit4(rng_block(d.seed_lo, d.seed_hi, island, 0, s, TAG_INIT, c));
             out[c] = x;
             kl[c] = x;
         }
     } else {
-        const int* perm = d.perm[cur];
-        const float* keys_in = d.keys[cur];
         const uint32_t g = (uint32_t)t + 1u;
-        const int k = d.mig_k;
-        const bool migrate = I > 1 && d.mig_interval > 0 && k > 0 && t > 0 && (t % d.mig_interval) == 0;
+        const bool migrate = maybe_migrant && t > 0 && (t % d.mig_interval) == 0;
         int src = -1;
-        if (migrate && s >= Pi - k) {
+        if (migrate) {
             const int si = (island + I - 1) % I;
-            src = si * Pi + perm[si * Pi + (s - (Pi - k))];
+            src = si * Pi + d.perm[cur][si * Pi + (s - (Pi - k))];
         } else if (s < Ei) {
-            src = island * Pi + perm[island * Pi + s];
+            src = island * Pi + (staged ? order[s] : perm_cur[s]);
         }
         if (src >= 0) {
-            // elite / migrant: the row and its fitness are copied unchanged
-            const float4* in = reinterpret_cast<const float4*>(keys_in + (size_t)src * ld);
+            const float4* in = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)src * ld);
             for (int c = threadIdx.x; c < nq; c += BLOCK) out[c] = in[c];
             if (threadIdx.x < M) fit_out[threadIdx.x] = d.fit[cur][(size_t)src * M + threadIdx.x];
             return;
         }
         if (s >= Pi - Mi) {
             for (int c = threadIdx.x; c < nq; c += BLOCK) {
-                const U4 r = rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_MUTANT, c);
-                const float4 x = make_float4(unit_from_u32(r.x), unit_from_u32(r.y), unit_from_u32(r.z),
-                                             unit_from_u32(r.w));
+                const float4 x = unit4(rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_MUTANT, c));
                 out[c] = x;
                 kl[c] = x;
             }
         } else {
             const U4 pr = rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_PARENTS, 0);
-            const int ep = island * Pi + perm[island * Pi + (int)(pr.x % (uint32_t)Ei)];
-            const int np = island * Pi + perm[island * Pi + Ei + (int)(pr.y % (uint32_t)(Pi - Ei))];
-            const float4* pe = reinterpret_cast<const float4*>(keys_in + (size_t)ep * ld);
-            const float4* pn = reinterpret_cast<const float4*>(keys_in + (size_t)np * ld);
+            const int ei = (int)(pr.x % (uint32_t)Ei);
+            const int ni = Ei + (int)(pr.y % (uint32_t)(Pi - Ei));
+            const int ep = island * Pi + (staged ? order[ei] : perm_cur[ei]);
+            const int np = island * Pi + (staged ? order[ni] : perm_cur[ni]);
+            const float4* pe = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)ep * ld);
+            const float4* pn = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)np * ld);
             const float rho = d.rho;
             for (int c = threadIdx.x; c < nq; c += BLOCK) {
                 const U4 r = rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_COINS, c);
-                const float4 a = pe[c], bb = pn[c];
+                const float4 a = pe[c], b = pn[c];
                 float4 x;
-                x.x = unit_from_u32(r.x) < rho ? a.x : bb.x;
-                x.y = unit_from_u32(r.y) < rho ? a.y : bb.y;
-                x.z = unit_from_u32(r.z) < rho ? a.z : bb.z;
-                x.w = unit_from_u32(r.w) < rho ? a.w : bb.w;
+                x.x = unit_from_u32(r.x) < rho ? a.x : b.x;
+                x.y = unit_from_u32(r.y) < rho ? a.y : b.y;
+                x.z = unit_from_u32(r.z) < rho ? a.z : b.z;
+                x.w = unit_from_u32(r.w) < rho ? a.w : b.w;
                 out[c] = x;
                 kl[c] = x;
             }
         }
     }
-    __syncthreads();
-    decode_block<KIND, BLOCK>(d.prob, lds, lay, nb, fit_out, nullptr);
-}
-
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_partition(const InstDesc* __restrict__ descs, int num_inst, int init)
-{
-    extern __shared__ __align__(16) char lds[];
-    unsigned long long* sk = reinterpret_cast<unsigned long long*>(lds);
-    const int b = blockIdx.x;
-    const InstDesc& d = descs[find_instance_by_seg(descs, num_inst, b)];
-    const int island = b - d.seg_begin;
-    if (island >= d.islands) return;
-    const int t = init ? 0 : *d.gen;
-    if (!init && t >= d.gen_limit) return;
-    const int buf = init ? 0 : ((t + 1) & 1);
-    const int Pi = d.Pi;
-    const int Pn = pow2_at_least(Pi);
-    const int M = d.prob.num_obj;
-    const float* fit = d.fit[buf] + (size_t)island * Pi * M;
-    const int maximize = d.prob.maximize;
-    for (int i = threadIdx.x; i < Pn; i += BLOCK) {
-        sk[i] = i < Pi ? ((unsigned long long)fitness_order_key(fit[(size_t)i * M], maximize) << 32) | (unsigned)i
-                       : ~0ull;
-    }
-    __syncthreads();
-    for (int k = 2; k <= Pn; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < (Pn >> 1); i += BLOCK) {
-                const int lo = 2 * j * (i / j) + (i % j);
-                const int hi = lo + j;
-                const bool up = (lo & k) == 0;
-                const unsigned long long x = sk[lo], y = sk[hi];
-                if ((x > y) == up) {
-                    sk[lo] = y;
-                    sk[hi] = x;
-                }
-            }
-            __syncthreads();
+    if constexpr (kXY) {
+#pragma unroll
+        for (int k = 0; k < kSortItems; ++k) {
+            const int i = threadIdx.x + k * BLOCK;
+            if (i < n) xy[i] = cxy[k];
         }
     }
-    int* perm = d.perm[buf] + (size_t)island * Pi;
-    for (int i = threadIdx.x; i < Pi; i += BLOCK) perm[i] = (int)(sk[i] & 0xFFFFFFFFull);
+    __syncthreads();
+    decode_block<KIND, BLOCK, true, kXY>(d.prob, lds, sl.dec, nb, fit_out, nullptr, xy, nullptr);
+}
+
+template <int KIND, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_step_one(InstDesc d, int cur, int init)
+{
+    extern __shared__ __align__(16) char lds[];
+    step_body<KIND, BLOCK>(d, blockIdx.x, cur, init, lds);
+}
+
+template <int KIND, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_step_many(const InstDesc* __restrict__ descs, int num_inst, int cur,
+                                                     int init)
+{
+    extern __shared__ __align__(16) char lds[];
+    const InstDesc& d = descs[find_instance_by_slot(descs, num_inst, blockIdx.x)];
+    const int gs = blockIdx.x - d.slot_begin;
+    if (gs >= d.pop) return;
+    int c = cur;
     if (!init) {
+        const int t = *d.gen;
+        if (t >= d.gen_limit) return;
+        if (c < 0) c = t & 1;
+    }
+    step_body<KIND, BLOCK>(d, gs, c, init, lds);
+}
+
+constexpr int kPartBlock = 1024;
+
+// Stable rank order of island `island`'s fitness in buffer `buf`.  Up to
+// kPartBlock individuals: every key (fitness order key, index) is unique,
+// so the rank of a key is the number of smaller keys; G = kPartBlock / Pn
+// threads count one key's rank over interleaved quarters and combine with
+// wave shuffles.  Larger islands use a bitonic network.  With `advance`,
+// the last island block of the population bumps the generation counter.
+__device__ __forceinline__ void partition_body(const InstDesc& d, int island, int buf, int advance, int t,
+                                               char* lds)
+{
+    constexpr int B = kPartBlock;
+    unsigned long long* sk = reinterpret_cast<unsigned long long*>(lds);
+    const int Pi = d.Pi;
+    const int M = d.prob.num_obj;
+    const float* fit = d.fit[buf] + (size_t)island * Pi * M;
+    int* perm = d.perm[buf] + (size_t)island * Pi;
+    const int maximize = d.prob.maximize;
+    const int Pn = pow2_at_least(Pi < 2 ? 2 : Pi);
+    for (int i = threadIdx.x; i < Pn; i += B)
+        sk[i] = i < Pi ? ((unsigned long long)fitness_order_key(fit[(size_t)i * M], maximize) << 32) | (unsigned)i
+                       : ~0ull;
+    __syncthreads();
+    if (Pn <= B) {
+        const int G = B / Pn;  // threads per key, power of two <= 512
+        const int e = threadIdx.x / G, sub = threadIdx.x % G;
+        const unsigned long long mine = sk[e];
+        const ulonglong2* pairs = reinterpret_cast<const ulonglong2*>(sk);
+        int cnt = 0;
+        for (int q = sub; q < (Pn >> 1); q += G) {
+            const ulonglong2 v = pairs[q];
+            cnt += (v.x < mine) + (v.y < mine);
+        }
+        for (int o = 1; o < G && o < kWave; o <<= 1) cnt += __shfl_xor(cnt, o, kWave);
+        if (G > kWave) {
+            // more than a wave per key (Pn < 16): fold the waves through LDS
+            __syncthreads();
+            int* part = reinterpret_cast<int*>(sk + Pn);
+            if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = cnt;
+            __syncthreads();
+            if (sub == 0) {
+                cnt = 0;
+                for (int w = 0; w < G / kWave; ++w) cnt += part[threadIdx.x / kWave + w];
+            }
+        }
+        if (sub == 0 && e < Pi) perm[cnt] = e;
+    } else {
+        for (int k = 2; k <= Pn; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = threadIdx.x; i < (Pn >> 1); i += B) {
+                    const int lo = 2 * j * (i / j) + (i % j);
+                    const int hi = lo + j;
+                    const bool up = (lo & k) == 0;
+                    const unsigned long long x = sk[lo], y = sk[hi];
+                    if ((x > y) == up) {
+                        sk[lo] = y;
+                        sk[hi] = x;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = threadIdx.x; i < Pi; i += B) perm[i] = (int)(sk[i] & 0xFFFFFFFFull);
+    }
+    if (advance) {
         __syncthreads();
         if (threadIdx.x == 0) {
             __threadfence();
@@ -203,6 +294,37 @@ __global__ __launch_bounds__(BLOCK) void k_partition(const InstDesc* __restrict_
             }
         }
     }
+}
+
+__global__ __launch_bounds__(kPartBlock) void k_partition_one(InstDesc d, int buf, int advance)
+{
+    extern __shared__ __align__(16) char lds[];
+    const int t = advance ? *d.gen : 0;
+    partition_body(d, blockIdx.x, buf, advance, t, lds);
+}
+
+__global__ __launch_bounds__(kPartBlock) void k_partition_many(const InstDesc* __restrict__ descs, int num_inst,
+                                                               int buf, int advance)
+{
+    extern __shared__ __align__(16) char lds[];
+    const InstDesc& d = descs[find_instance_by_seg(descs, num_inst, blockIdx.x)];
+    const int island = blockIdx.x - d.seg_begin;
+    if (island >= d.islands) return;
+    int t = 0, b = buf;
+    if (advance) {
+        t = *d.gen;
+        if (t >= d.gen_limit) return;
+        if (b < 0) b = (t + 1) & 1;
+    } else if (b < 0) {
+        b = 0;
+    }
+    partition_body(d, island, b, advance, t, lds);
+}
+
+int partition_lds_bytes(int Pi)
+{
+    const int Pn = pow2_at_least(Pi < 2 ? 2 : Pi);
+    return Pn * 8 + 64;
 }
 
 // ------------------------------------------------------------- launchers --
@@ -242,32 +364,55 @@ hipError_t eval_kind(int block, const ProblemDesc& p, const float* keys, long lo
 }
 
 template <int KIND, int BLOCK>
-hipError_t step_one(const InstDesc* d, int num, int blocks, int lds, int init, hipStream_t s)
+hipError_t step_single(const InstDesc& d, int lds, int cur, int init, hipStream_t s)
 {
-    ensure_lds(k_brkga_step<KIND, BLOCK>, lds);
-    hipLaunchKernelGGL((k_brkga_step<KIND, BLOCK>), dim3(blocks), dim3(BLOCK), lds, s, d, num, init);
+    ensure_lds(k_step_one<KIND, BLOCK>, lds);
+    hipLaunchKernelGGL((k_step_one<KIND, BLOCK>), dim3(d.pop), dim3(BLOCK), lds, s, d, cur, init);
+    return hipGetLastError();
+}
+
+template <int KIND, int BLOCK>
+hipError_t step_multi(const InstDesc* d, int num, int blocks, int lds, int cur, int init, hipStream_t s)
+{
+    ensure_lds(k_step_many<KIND, BLOCK>, lds);
+    hipLaunchKernelGGL((k_step_many<KIND, BLOCK>), dim3(blocks), dim3(BLOCK), lds, s, d, num, cur, init);
     return hipGetLastError();
 }
 
 template <int KIND>
-hipError_t step_kind(int block, const InstDesc* d, int num, int blocks, int lds, int init, hipStream_t s)
+hipError_t step_kind(int block, const InstDesc* one, const InstDesc* many, int num, int blocks, int lds, int cur,
+                     int init, hipStream_t s)
 {
+#define BRKGA_STEP_CASE(B)                                                       \
+    case B:                                                                      \
+        return one ? step_single<KIND, B>(*one, lds, cur, init, s)               \
+                   : step_multi<KIND, B>(many, num, blocks, lds, cur, init, s);
     switch (block) {
-    case 64: return step_one<KIND, 64>(d, num, blocks, lds, init, s);
-    case 128: return step_one<KIND, 128>(d, num, blocks, lds, init, s);
-    case 256: return step_one<KIND, 256>(d, num, blocks, lds, init, s);
-    case 512: return step_one<KIND, 512>(d, num, blocks, lds, init, s);
-    case 1024: return step_one<KIND, 1024>(d, num, blocks, lds, init, s);
+        BRKGA_STEP_CASE(64)
+        BRKGA_STEP_CASE(128)
+        BRKGA_STEP_CASE(256)
+        BRKGA_STEP_CASE(512)
+        BRKGA_STEP_CASE(1024)
     default: return hipErrorInvalidValue;
     }
+#undef BRKGA_STEP_CASE
 }
 
-template <int BLOCK>
-hipError_t part_one(const InstDesc* d, int num, int blocks, int lds, int init, hipStream_t s)
+hipError_t step_dispatch(int kind, int block, const InstDesc* one, const InstDesc* many, int num, int blocks,
+                         int lds, int cur, int init, hipStream_t s)
 {
-    ensure_lds(k_partition<BLOCK>, lds);
-    hipLaunchKernelGGL((k_partition<BLOCK>), dim3(blocks), dim3(BLOCK), lds, s, d, num, init);
-    return hipGetLastError();
+    switch (kind) {
+    case KIND_TSP_COORDS:
+        return step_kind<KIND_TSP_COORDS>(block, one, many, num, blocks, lds, cur, init, s);
+    case KIND_TSP_MATRIX:
+        return step_kind<KIND_TSP_MATRIX>(block, one, many, num, blocks, lds, cur, init, s);
+    case KIND_KNAP_THRESHOLD:
+        return step_kind<KIND_KNAP_THRESHOLD>(block, one, many, num, blocks, lds, cur, init, s);
+    case KIND_KNAP_GREEDY:
+        return step_kind<KIND_KNAP_GREEDY>(block, one, many, num, blocks, lds, cur, init, s);
+    case KIND_TSPJ: return step_kind<KIND_TSPJ>(block, one, many, num, blocks, lds, cur, init, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
@@ -290,33 +435,39 @@ hipError_t launch_evaluate(const ProblemDesc& p, const float* keys, long long ld
     }
 }
 
-hipError_t launch_brkga_step(int kind, int block, const InstDesc* d_descs, int num_inst, int blocks, int lds,
-                             int init, hipStream_t s)
+int step_lds_bytes(const InstDesc& d)
 {
-    if (blocks <= 0) return hipSuccess;
-    switch (kind) {
-    case KIND_TSP_COORDS: return step_kind<KIND_TSP_COORDS>(block, d_descs, num_inst, blocks, lds, init, s);
-    case KIND_TSP_MATRIX: return step_kind<KIND_TSP_MATRIX>(block, d_descs, num_inst, blocks, lds, init, s);
-    case KIND_KNAP_THRESHOLD:
-        return step_kind<KIND_KNAP_THRESHOLD>(block, d_descs, num_inst, blocks, lds, init, s);
-    case KIND_KNAP_GREEDY: return step_kind<KIND_KNAP_GREEDY>(block, d_descs, num_inst, blocks, lds, init, s);
-    case KIND_TSPJ: return step_kind<KIND_TSPJ>(block, d_descs, num_inst, blocks, lds, init, s);
-    default: return hipErrorInvalidValue;
-    }
+    return step_layout(d.prob.kind, d.prob.n, d.L, d.nb, d.Pi).bytes;
 }
 
-hipError_t launch_partition(int block, const InstDesc* d_descs, int num_inst, int blocks, int lds, int init,
-                            hipStream_t s)
+hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s)
+{
+    return step_dispatch(d.prob.kind, block, &d, nullptr, 1, d.pop, step_lds_bytes(d), cur, init, s);
+}
+
+hipError_t launch_step_many(int kind, int block, const InstDesc* d_descs, int num_inst, int blocks, int lds, int cur,
+                            int init, hipStream_t s)
 {
     if (blocks <= 0) return hipSuccess;
-    switch (block) {
-    case 64: return part_one<64>(d_descs, num_inst, blocks, lds, init, s);
-    case 128: return part_one<128>(d_descs, num_inst, blocks, lds, init, s);
-    case 256: return part_one<256>(d_descs, num_inst, blocks, lds, init, s);
-    case 512: return part_one<512>(d_descs, num_inst, blocks, lds, init, s);
-    case 1024: return part_one<1024>(d_descs, num_inst, blocks, lds, init, s);
-    default: return hipErrorInvalidValue;
-    }
+    return step_dispatch(kind, block, nullptr, d_descs, num_inst, blocks, lds, cur, init, s);
+}
+
+hipError_t launch_partition_one(const InstDesc& d, int buf, int advance, hipStream_t s)
+{
+    const int lds = partition_lds_bytes(d.Pi);
+    ensure_lds(k_partition_one, lds);
+    hipLaunchKernelGGL(k_partition_one, dim3(d.islands), dim3(kPartBlock), lds, s, d, buf, advance);
+    return hipGetLastError();
+}
+
+hipError_t launch_partition_many(const InstDesc* d_descs, int num_inst, int blocks, int max_Pi, int buf, int advance,
+                                 hipStream_t s)
+{
+    if (blocks <= 0) return hipSuccess;
+    const int lds = partition_lds_bytes(max_Pi);
+    ensure_lds(k_partition_many, lds);
+    hipLaunchKernelGGL(k_partition_many, dim3(blocks), dim3(kPartBlock), lds, s, d_descs, num_inst, buf, advance);
+    return hipGetLastError();
 }
 
 }  // namespace brkga

@@ -12,16 +12,27 @@ namespace brkga {
 hipError_t launch_evaluate(const ProblemDesc& p, const float* keys, long long ld, int count, float* fit,
                            int* decoded, int block, hipStream_t s);
 
-// One BRKGA generation step (init=1: generation-0 keys + evaluation) for
-// every instance in d_descs; `blocks` = total population slots, one
-// workgroup each; `lds` = the largest decode layout among the instances.
-hipError_t launch_brkga_step(int kind, int block, const InstDesc* d_descs, int num_inst, int blocks, int lds,
-                             int init, hipStream_t s);
+// One BRKGA generation step of one population, descriptor passed by value.
+// cur = buffer parity of the parent generation; init=1 builds generation 0
+// (Philox keys + evaluation) in buffer 0.
+hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s);
 
-// Stable per-island sort of fitness (one workgroup per island) that closes
-// a generation step and advances each instance's step counter.
-hipError_t launch_partition(int block, const InstDesc* d_descs, int num_inst, int blocks, int lds, int init,
-                            hipStream_t s);
+// Same for many populations of one problem kind, descriptors in device
+// memory; `blocks` = total population slots, `lds` = the largest
+// step_lds_bytes among them.  cur < 0: each population's parity comes from
+// its step counter (populations at different generations).
+hipError_t launch_step_many(int kind, int block, const InstDesc* d_descs, int num_inst, int blocks, int lds,
+                            int cur, int init, hipStream_t s);
+
+// Stable per-island sort of the fitness in buffer `buf` into the rank order;
+// with advance=1 it closes a generation step and bumps the step counter.
+hipError_t launch_partition_one(const InstDesc& d, int buf, int advance, hipStream_t s);
+// buf < 0: buffer from each population's step counter.
+hipError_t launch_partition_many(const InstDesc* d_descs, int num_inst, int blocks, int max_Pi, int buf, int advance,
+                                 hipStream_t s);
+
+int step_lds_bytes(const InstDesc& d);
+int partition_lds_bytes(int Pi);
 
 // NSGA-II: offspring generation + evaluation (one workgroup per child).
 hipError_t launch_nsga2_offspring(int kind, int block, const InstDesc* d_descs, int num_inst, int blocks,
