@@ -214,6 +214,47 @@ brkga_status brkga_population_device_buffers(brkga_pop_t pop, float** d_keys, fl
 brkga_status brkga_population_profile(brkga_pop_t pop, int32_t generations, double* step_us, double* close_us,
                                       void* stream);
 
+/* ------------------------------------------------------------ batches --- */
+
+/* Many independent populations (any problem kinds, uneven sizes) evolved
+ * together.  The reference has no batched form: it runs one Solver per
+ * instance (core/solver.hpp:1517-1556); a batch member's result equals that
+ * of the same population evolved alone with brkga_evolve.
+ *   BRKGA_BATCH_PLAIN   one member at a time through brkga_population_* /
+ *                       brkga_evolve (one launch sequence per member and
+ *                       generation); the per-instance form.
+ *   BRKGA_BATCH_LAUNCH  all members per generation: one persistent
+ *                       queue-driven step launch per size bucket, one
+ *                       partition launch, NSGA-II launches per kind.
+ *   BRKGA_BATCH_GRAPH   the same launches replayed from a captured graph of
+ *                       the requested number of generations. */
+typedef struct brkga_batch_s* brkga_batch_t;
+typedef enum { BRKGA_BATCH_PLAIN = 0, BRKGA_BATCH_LAUNCH = 1, BRKGA_BATCH_GRAPH = 2 } brkga_batch_mode;
+
+/* count populations, member i over problems[i] with params[i]. */
+brkga_status brkga_batch_create(brkga_engine_t engine, int32_t count, const brkga_problem_t* problems,
+                                const brkga_params* params, brkga_batch_t* out);
+brkga_status brkga_batch_destroy(brkga_batch_t batch);
+
+/* Generation 0 of every member, then `generations` steps of every member. */
+brkga_status brkga_batch_initialize(brkga_batch_t batch, int32_t mode, void* stream);
+brkga_status brkga_batch_evolve(brkga_batch_t batch, int32_t generations, int32_t mode, void* stream);
+
+/* Picks the workgroup size and slots per work item of each size bucket by
+ * timing `reps` step launches per candidate on the current populations
+ * (call after initialize).  Does not advance or change any population. */
+brkga_status brkga_batch_autotune(brkga_batch_t batch, int32_t reps, void* stream);
+
+/* Member i as a population handle owned by the batch (read, get_best,
+ * generation; do not destroy or evolve it on its own while the batch runs). */
+brkga_status brkga_batch_population(brkga_batch_t batch, int32_t index, brkga_pop_t* out);
+
+/* Size buckets of the batched launch: count, then per bucket its problem
+ * kind, member n range and count, workgroup size and slots per item. */
+brkga_status brkga_batch_num_buckets(brkga_batch_t batch, int32_t* count);
+brkga_status brkga_batch_bucket_info(brkga_batch_t batch, int32_t bucket, int32_t* kind, int32_t* n_min,
+                                     int32_t* n_max, int32_t* members, int32_t* block, int32_t* chunk);
+
 #ifdef __cplusplus
 }
 #endif

@@ -119,6 +119,15 @@ _SIGNATURES = {
                                         ctypes.c_int),
     "brkga_population_profile": ([_P, _I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _P],
                                  ctypes.c_int),
+    "brkga_batch_create": ([_P, _I32, ctypes.POINTER(_P), ctypes.POINTER(CParams), ctypes.POINTER(_P)],
+                           ctypes.c_int),
+    "brkga_batch_destroy": ([_P], ctypes.c_int),
+    "brkga_batch_initialize": ([_P, _I32, _P], ctypes.c_int),
+    "brkga_batch_evolve": ([_P, _I32, _I32, _P], ctypes.c_int),
+    "brkga_batch_autotune": ([_P, _I32, _P], ctypes.c_int),
+    "brkga_batch_population": ([_P, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_batch_num_buckets": ([_P, _PI], ctypes.c_int),
+    "brkga_batch_bucket_info": ([_P, _I32, _PI, _PI, _PI, _PI, _PI, _PI], ctypes.c_int),
 }
 
 
@@ -269,6 +278,19 @@ class Population:
         h = _P()
         _check(lib().brkga_population_create(problem.handle, ctypes.byref(self._cp), ctypes.byref(h)))
         self.handle = h
+        self._owned = True
+
+    @classmethod
+    def _borrowed(cls, problem: Problem, params: Params, handle, owner):
+        """View of a population owned by a batch (not destroyed by this object)."""
+        self = cls.__new__(cls)
+        self.problem = problem
+        self.params = params
+        self._cp = params.to_c()
+        self.handle = handle
+        self._owned = False
+        self._owner = owner
+        return self
 
     @property
     def size(self):
@@ -324,8 +346,87 @@ class Population:
         return a.value, b.value
 
     def close(self):
-        if self.handle:
+        if self.handle and self._owned:
             lib().brkga_population_destroy(self.handle)
+        self.handle = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+BATCH_PLAIN = 0
+BATCH_LAUNCH = 1
+BATCH_GRAPH = 2
+_MODES = {"plain": BATCH_PLAIN, "launch": BATCH_LAUNCH, "graph": BATCH_GRAPH}
+
+
+class Batch:
+    """Many independent populations evolved together (brkga_batch_*).
+
+    mode "plain" runs the members one at a time through the single-population
+    launches, "launch" runs every member per generation through the
+    queue-driven batched kernels, "graph" replays those from a HIP graph.
+    Each member evolves exactly as it would alone."""
+
+    def __init__(self, engine: Engine, problems, params):
+        problems = list(problems)
+        params = list(params) if not isinstance(params, Params) else [params] * len(problems)
+        assert len(problems) == len(params) and problems
+        self.engine = engine
+        self.problems = problems
+        self.params = params
+        hs = (_P * len(problems))(*[p.handle for p in problems])
+        cps = (CParams * len(params))(*[q.to_c() for q in params])
+        h = _P()
+        _check(lib().brkga_batch_create(engine.handle, len(problems), hs, cps, ctypes.byref(h)))
+        self.handle = h
+        self._members = {}
+
+    def __len__(self):
+        return len(self.problems)
+
+    @staticmethod
+    def _mode(mode):
+        return _MODES[mode] if isinstance(mode, str) else int(mode)
+
+    def initialize(self, mode="launch", stream: int = 0):
+        _check(lib().brkga_batch_initialize(self.handle, self._mode(mode), stream or None))
+
+    def evolve(self, generations: int, mode="launch", stream: int = 0):
+        _check(lib().brkga_batch_evolve(self.handle, generations, self._mode(mode), stream or None))
+
+    def autotune(self, reps: int = 3, stream: int = 0):
+        _check(lib().brkga_batch_autotune(self.handle, reps, stream or None))
+
+    def member(self, i: int) -> Population:
+        if i not in self._members:
+            h = _P()
+            _check(lib().brkga_batch_population(self.handle, i, ctypes.byref(h)))
+            self._members[i] = Population._borrowed(self.problems[i], self.params[i], h, self)
+        return self._members[i]
+
+    def buckets(self):
+        """Per size bucket: kind, n range, member count, workgroup size, slots per item."""
+        c = ctypes.c_int32()
+        _check(lib().brkga_batch_num_buckets(self.handle, ctypes.byref(c)))
+        out = []
+        for b in range(c.value):
+            v = [ctypes.c_int32() for _ in range(6)]
+            _check(lib().brkga_batch_bucket_info(self.handle, b, *[ctypes.byref(x) for x in v]))
+            kind, n_min, n_max, count, block, chunk = (x.value for x in v)
+            out.append({"kind": kind, "n_min": n_min, "n_max": n_max, "members": count, "block": block,
+                        "chunk": chunk})
+        return out
+
+    def close(self):
+        if self.handle:
+            for m in self._members.values():
+                m.handle = _P()
+            self._members = {}
+            lib().brkga_batch_destroy(self.handle)
             self.handle = _P()
 
     def __del__(self):

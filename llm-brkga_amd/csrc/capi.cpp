@@ -15,19 +15,6 @@ namespace brkga {
 namespace {
 thread_local std::string g_last_error;
 
-struct DeviceGuard {
-    int prev = -1;
-    explicit DeviceGuard(int dev)
-    {
-        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
-        (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard()
-    {
-        if (prev >= 0) (void)hipSetDevice(prev);
-    }
-};
-
 constexpr int kMaxGenes = 16000;   // fixed-point sums hold 2^14 terms
 constexpr int kMaxIslandPop = 16384;
 constexpr int kMaxNsgaPop = 2048;  // parents + children index in 12 bits
@@ -221,47 +208,103 @@ brkga_status fill_inst_desc(const brkga_problem_s* pr, const brkga_params* q, In
     return BRKGA_OK;
 }
 
-brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, std::vector<void*>* owned)
+brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, DeviceAllocator* a, hipStream_t s,
+                                std::vector<void*>* owned)
 {
     const size_t rows = nsga2 ? (size_t)2 * d->pop : (size_t)d->pop;
     const int M = d->prob.num_obj;
+    auto get = [&](void** p, size_t bytes, const char* what) -> brkga_status {
+        brkga_status st = a ? a->alloc(p, bytes, what) : device_alloc(p, bytes, what);
+        if (st == BRKGA_OK) owned->push_back(*p);
+        return st;
+    };
+    brkga_status st;
     for (int b = 0; b < 2; ++b) {
         void* p;
-        brkga_status st;
-        if ((st = device_alloc(&p, rows * d->ld * sizeof(float), "population keys")) != BRKGA_OK) return st;
-        owned->push_back(p);
+        if ((st = get(&p, rows * d->ld * sizeof(float), "population keys")) != BRKGA_OK) return st;
         d->keys[b] = (float*)p;
-        if ((st = device_alloc(&p, rows * M * sizeof(float), "fitness")) != BRKGA_OK) return st;
-        owned->push_back(p);
+        if ((st = get(&p, rows * M * sizeof(float), "fitness")) != BRKGA_OK) return st;
         d->fit[b] = (float*)p;
-        if ((st = device_alloc(&p, rows * sizeof(int), "rank order")) != BRKGA_OK) return st;
-        owned->push_back(p);
+        if ((st = get(&p, rows * sizeof(int), "rank order")) != BRKGA_OK) return st;
         d->perm[b] = (int*)p;
         if (nsga2) {
-            if ((st = device_alloc(&p, rows * sizeof(int), "front ranks")) != BRKGA_OK) return st;
-            owned->push_back(p);
+            if ((st = get(&p, rows * sizeof(int), "front ranks")) != BRKGA_OK) return st;
             d->rank[b] = (int*)p;
-            if ((st = device_alloc(&p, rows * sizeof(float), "crowding")) != BRKGA_OK) return st;
-            owned->push_back(p);
+            if ((st = get(&p, rows * sizeof(float), "crowding")) != BRKGA_OK) return st;
             d->crowd[b] = (float*)p;
         }
     }
     void* c;
-    const size_t counters = 2;
-    brkga_status st = device_alloc(&c, counters * sizeof(int), "counters");
-    if (st != BRKGA_OK) return st;
-    owned->push_back(c);
+    if ((st = get(&c, 2 * sizeof(int), "counters")) != BRKGA_OK) return st;
     d->gen = (int*)c;
     d->done = (int*)c + 1;
     // the padding columns are never produced by the RNG paths of every
     // kernel; clear them once so copies stay deterministic
-    for (int b = 0; b < 2; ++b) {
-        hipError_t e = hipMemset(d->keys[b], 0, rows * d->ld * sizeof(float));
-        if (e != hipSuccess) return hip_error(e, "hipMemset(keys)");
-    }
-    hipError_t e = hipMemset(c, 0, counters * sizeof(int));
-    if (e != hipSuccess) return hip_error(e, "hipMemset(counters)");
+    for (int b = 0; b < 2; ++b) BRKGA_HIP_TRY(hipMemsetAsync(d->keys[b], 0, rows * d->ld * sizeof(float), s));
+    BRKGA_HIP_TRY(hipMemsetAsync(c, 0, 2 * sizeof(int), s));
     return BRKGA_OK;
+}
+
+brkga_status create_population(brkga_problem_s* problem, const brkga_params* params, DeviceAllocator* a,
+                               hipStream_t s, int gen0, brkga_pop_s** out)
+{
+    if (!problem || !params || !out) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    auto* pop = new brkga_pop_s();
+    pop->problem = problem;
+    pop->allocator = a;
+    pop->params = *params;
+    pop->d_desc = nullptr;
+    pop->capture_stream = nullptr;
+    brkga_status st = fill_inst_desc(problem, params, &pop->host, &pop->nsga2);
+    if (st == BRKGA_OK) st = alloc_inst_buffers(&pop->host, pop->nsga2, a, s, &pop->owned);
+    auto get = [&](void** p, size_t bytes, const char* what) -> brkga_status {
+        brkga_status r = a ? a->alloc(p, bytes, what) : device_alloc(p, bytes, what);
+        if (r == BRKGA_OK) pop->owned.push_back(*p);
+        return r;
+    };
+    void* p = nullptr;
+    if (st == BRKGA_OK && (st = get(&p, sizeof(InstDesc), "descriptor")) == BRKGA_OK) pop->d_desc = (InstDesc*)p;
+    if (st == BRKGA_OK && (st = get(&p, sizeof(int) * pop->host.L, "decode row")) == BRKGA_OK)
+        pop->d_decode_row = (int*)p;
+    if (st == BRKGA_OK && (st = get(&p, sizeof(float) * 2, "fitness row")) == BRKGA_OK) pop->d_fit_row = (float*)p;
+    if (st == BRKGA_OK) {
+        const int n = problem->desc.n;
+        pop->block = default_block(n);
+        pop->host_t = 0;
+        pop->step_lds = pop->nsga2 ? decode_layout(problem->desc.kind, n, pop->host.L, pop->host.nb).bytes
+                                   : step_lds_bytes(pop->host);
+        if ((size_t)partition_lds_bytes(pop->host.Pi) > problem->engine->lds_limit ||
+            (size_t)pop->step_lds > problem->engine->lds_limit)
+            st = set_error(BRKGA_ERR_INVALID_ARG, "population or chromosome too large for the LDS kernels");
+    }
+    if (st == BRKGA_OK) {
+        // pop->host outlives the copy, so the asynchronous upload is safe
+        hipError_t e = hipMemcpyAsync(pop->d_desc, &pop->host, sizeof(InstDesc), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess && gen0 < 0) e = hipMemsetAsync(pop->host.gen, 0xFF, sizeof(int), s);
+        if (e != hipSuccess) st = hip_error(e, "population upload");
+    }
+    if (st != BRKGA_OK) {
+        (void)hipStreamSynchronize(s);
+        destroy_population(pop);
+        return st;
+    }
+    *out = pop;
+    return BRKGA_OK;
+}
+
+void destroy_population(brkga_pop_s* pop)
+{
+    if (!pop) return;
+    for (auto& kv : pop->graphs) (void)hipGraphExecDestroy(kv.second);
+    if (pop->capture_stream) (void)hipStreamDestroy(pop->capture_stream);
+    if (pop->allocator) {
+        for (void* p : pop->owned) pop->allocator->release(p);
+        pop->owned.clear();
+    } else {
+        free_all(pop->owned);
+    }
+    delete pop;
 }
 
 }  // namespace brkga
@@ -539,53 +582,15 @@ brkga_status brkga_evaluate_host(brkga_problem_t problem, const float* keys, int
 brkga_status brkga_population_create(brkga_problem_t problem, const brkga_params* params, brkga_pop_t* out)
 {
     if (!problem || !params || !out) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
-    *out = nullptr;
-    auto* pop = new brkga_pop_s();
-    pop->problem = problem;
-    pop->params = *params;
-    pop->d_desc = nullptr;
-    pop->capture_stream = nullptr;
     DeviceGuard g(problem->engine->device);
-    brkga_status st = fill_inst_desc(problem, params, &pop->host, &pop->nsga2);
-    if (st == BRKGA_OK) st = alloc_inst_buffers(&pop->host, pop->nsga2, &pop->owned);
-    void* p = nullptr;
-    if (st == BRKGA_OK && (st = device_alloc(&p, sizeof(InstDesc), "descriptor")) == BRKGA_OK) {
-        pop->owned.push_back(p);
-        pop->d_desc = (InstDesc*)p;
+    brkga_status st = create_population(problem, params, nullptr, nullptr, 0, out);
+    if (st != BRKGA_OK) return st;
+    hipError_t e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+        destroy_population(*out);
+        *out = nullptr;
+        return hip_error(e, "hipStreamSynchronize");
     }
-    if (st == BRKGA_OK && (st = device_alloc(&p, sizeof(int) * pop->host.L, "decode row")) == BRKGA_OK) {
-        pop->owned.push_back(p);
-        pop->d_decode_row = (int*)p;
-    }
-    if (st == BRKGA_OK && (st = device_alloc(&p, sizeof(float) * 2, "fitness row")) == BRKGA_OK) {
-        pop->owned.push_back(p);
-        pop->d_fit_row = (float*)p;
-    }
-    if (st == BRKGA_OK) {
-        const int n = problem->desc.n;
-        pop->block = default_block(n);
-        pop->host_t = 0;
-        pop->step_lds = pop->nsga2 ? decode_layout(problem->desc.kind, n, pop->host.L, pop->host.nb).bytes
-                                   : step_lds_bytes(pop->host);
-        if ((size_t)partition_lds_bytes(pop->host.Pi) > problem->engine->lds_limit ||
-            (size_t)pop->step_lds > problem->engine->lds_limit)
-            st = set_error(BRKGA_ERR_INVALID_ARG, "population or chromosome too large for the LDS kernels");
-    }
-    if (st == BRKGA_OK) {
-        hipError_t e = hipMemcpy(pop->d_desc, &pop->host, sizeof(InstDesc), hipMemcpyHostToDevice);
-        if (e != hipSuccess) st = hip_error(e, "hipMemcpy(descriptor)");
-    }
-    if (st == BRKGA_OK) {
-        hipError_t e = hipStreamCreateWithFlags(&pop->capture_stream, hipStreamNonBlocking);
-        if (e != hipSuccess) st = hip_error(e, "hipStreamCreate");
-    }
-    if (st != BRKGA_OK) {
-        if (pop->capture_stream) (void)hipStreamDestroy(pop->capture_stream);
-        free_all(pop->owned);
-        delete pop;
-        return st;
-    }
-    *out = pop;
     return BRKGA_OK;
 }
 
@@ -594,10 +599,7 @@ brkga_status brkga_population_destroy(brkga_pop_t pop)
     if (!pop) return BRKGA_OK;
     DeviceGuard g(pop->problem->engine->device);
     (void)hipDeviceSynchronize();
-    for (auto& kv : pop->graphs) (void)hipGraphExecDestroy(kv.second);
-    if (pop->capture_stream) (void)hipStreamDestroy(pop->capture_stream);
-    free_all(pop->owned);
-    delete pop;
+    destroy_population(pop);
     return BRKGA_OK;
 }
 
@@ -635,6 +637,8 @@ brkga_status brkga_evolve_graph(brkga_pop_t pop, int32_t generations, void* stre
     const std::pair<int, int> key(generations, pop->host_t & 1);
     auto it = pop->graphs.find(key);
     if (it == pop->graphs.end()) {
+        if (!pop->capture_stream)
+            BRKGA_HIP_TRY(hipStreamCreateWithFlags(&pop->capture_stream, hipStreamNonBlocking));
         hipGraph_t graph = nullptr;
         BRKGA_HIP_TRY(hipStreamBeginCapture(pop->capture_stream, hipStreamCaptureModeThreadLocal));
         brkga_status st = BRKGA_OK;

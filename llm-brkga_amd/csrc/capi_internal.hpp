@@ -27,6 +27,20 @@ brkga_status device_alloc(void** p, size_t bytes, const char* what);
 
 int fixed_shift(double max_abs_term);
 
+// Makes `dev` current for the scope, restoring the previous device.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
 }  // namespace brkga
 
 struct brkga_engine_s {
@@ -41,8 +55,22 @@ struct brkga_problem_s {
     std::vector<void*> owned;
 };
 
+namespace brkga {
+
+// Source of a population's device buffers: plain hipMalloc / hipFree, or a
+// sub-range of a pre-reserved arena (served populations), whose release
+// does not wait for the device.
+struct DeviceAllocator {
+    virtual ~DeviceAllocator() = default;
+    virtual brkga_status alloc(void** p, size_t bytes, const char* what) = 0;
+    virtual void release(void* p) = 0;
+};
+
+}  // namespace brkga
+
 struct brkga_pop_s {
     brkga_problem_s* problem;
+    brkga::DeviceAllocator* allocator;  // nullptr: hipMalloc / hipFree
     brkga_params params;
     brkga::InstDesc host;     // descriptor mirror
     brkga::InstDesc* d_desc;  // descriptor in device memory
@@ -53,7 +81,7 @@ struct brkga_pop_s {
     int host_t;               // generation steps enqueued since initialize
     int* d_decode_row;        // scratch for get_best decode
     float* d_fit_row;
-    hipStream_t capture_stream;
+    hipStream_t capture_stream;  // created on the first graph capture
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (steps, start parity)
     std::mutex mu;
 };
@@ -62,7 +90,18 @@ namespace brkga {
 
 // Validates params against a problem and fills `d` (pointers excluded).
 brkga_status fill_inst_desc(const brkga_problem_s* pr, const brkga_params* q, InstDesc* d, int* nsga2);
-// Allocates the ping-pong buffers and counters described by d.
-brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, std::vector<void*>* owned);
+// Allocates the ping-pong buffers and counters described by d (from `a`,
+// or hipMalloc when null) and clears them on stream s.
+brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, DeviceAllocator* a, hipStream_t s,
+                                std::vector<void*>* owned);
+
+// Population over `pr` with buffers from `a` (nullptr: hipMalloc).  The
+// step counter starts at 0, or at -1 when gen0 < 0 (the first fleet tick
+// then builds generation 0).  Uploads run on stream s; the caller orders later work after them.
+brkga_status create_population(brkga_problem_s* pr, const brkga_params* q, DeviceAllocator* a, hipStream_t s,
+                               int gen0, brkga_pop_s** out);
+// Frees a population's buffers through its allocator (no device-wide wait
+// for arena-backed populations).
+void destroy_population(brkga_pop_s* pop);
 
 }  // namespace brkga

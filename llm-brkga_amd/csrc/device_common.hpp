@@ -267,9 +267,6 @@ __device__ __forceinline__ long long fx_value(const FxAcc& a)
 
 // ------------------------------------------------------------- argsort ----
 
-// Most keys one thread handles in the block argsort and decoders.
-constexpr int kSortItems = 16;
-
 // Bucket of a key for nb (power of two) value buckets over [0,1); any other
 // value is clamped to the first/last bucket, which keeps the map monotone.
 __device__ __forceinline__ int key_bucket(float k, int nb)

@@ -112,6 +112,9 @@ __host__ __device__ inline DecodeLayout decode_layout(int kind, int n, int L, in
     return d;
 }
 
+// Keys one decode thread holds: a workgroup of B threads decodes n <= 16 B.
+constexpr int kSortItems = 16;
+
 // Rank orders longer than this are read from global memory, not staged.
 constexpr int kMaxStagedOrder = 2048;
 

@@ -1,0 +1,307 @@
+// fleet.cpp - resident populations advanced one generation per tick.
+#include "fleet.hpp"
+
+#include <algorithm>
+#include <numeric>
+
+#include "launch.hpp"
+
+namespace brkga {
+
+namespace {
+
+constexpr int kQueueStride = 32;  // ints between two buckets' counters
+constexpr int kBlocks[] = {64, 128, 256, 512, 1024};
+constexpr int kChunks[] = {1, 2, 4, 8, 16};
+
+// Default slots per work item: enough decode work per item to hide the
+// queue pop and the per-item staging.
+int default_chunk(int n)
+{
+    if (n >= 512) return 1;
+    if (n >= 256) return 2;
+    if (n >= 128) return 4;
+    return 8;
+}
+
+}  // namespace
+
+int fleet_key(int n)
+{
+    int k = 64;
+    while (k < n) k <<= 1;
+    return k;
+}
+
+Fleet::~Fleet()
+{
+    DeviceGuard g(eng_->device);
+    drop_graphs();
+    if (capture_) (void)hipStreamDestroy(capture_);
+    if (d_descs_) (void)hipFree(d_descs_);
+    if (d_items_) (void)hipFree(d_items_);
+    if (d_queues_) (void)hipFree(d_queues_);
+}
+
+void Fleet::drop_graphs()
+{
+    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+    graphs_.clear();
+}
+
+brkga_status Fleet::reserve(void** p, size_t* cap, size_t bytes, const char* what)
+{
+    if (bytes <= *cap && *p) return BRKGA_OK;
+    size_t want = std::max<size_t>(bytes, 2 * *cap);
+    want = std::max<size_t>(want, 4096);
+    if (*p) {
+        // hipFree waits for the device; kernels of earlier ticks may still read it
+        (void)hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+    }
+    brkga_status st = device_alloc(p, want, what);
+    if (st != BRKGA_OK) return st;
+    *cap = want;
+    return BRKGA_OK;
+}
+
+brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream_t s)
+{
+    DeviceGuard g(eng_->device);
+    for (auto* p : pops)
+        if (!p) return set_error(BRKGA_ERR_INVALID_ARG, "null population in fleet");
+    // BRKGA members by (kind, size class, n descending), NSGA-II by (kind, n descending)
+    std::vector<brkga_pop_s*> order(pops);
+    std::stable_sort(order.begin(), order.end(), [](const brkga_pop_s* a, const brkga_pop_s* b) {
+        if (a->nsga2 != b->nsga2) return a->nsga2 < b->nsga2;
+        const int ka = a->host.prob.kind, kb = b->host.prob.kind;
+        if (ka != kb) return ka < kb;
+        const int na = a->host.prob.n, nb = b->host.prob.n;
+        if (!a->nsga2 && fleet_key(na) != fleet_key(nb)) return fleet_key(na) > fleet_key(nb);
+        return na > nb;
+    });
+    members_ = order;
+    descs_.assign(members_.size(), InstDesc());
+    buckets_.clear();
+    groups_.clear();
+    n_brkga_ = 0;
+    total_islands_ = 0;
+    max_Pi_ = 0;
+    for (size_t i = 0; i < members_.size(); ++i) {
+        const brkga_pop_s* p = members_[i];
+        InstDesc d = p->host;
+        d.slot_begin = 0;
+        d.seg_begin = 0;
+        const int n = d.prob.n;
+        if (!p->nsga2) {
+            d.seg_begin = total_islands_;
+            total_islands_ += d.islands;
+            max_Pi_ = std::max(max_Pi_, d.Pi);
+            ++n_brkga_;
+            const int key = fleet_key(n);
+            if (buckets_.empty() || buckets_.back().kind != d.prob.kind || buckets_.back().key != key) {
+                FleetBucket b{};
+                b.kind = d.prob.kind;
+                b.key = key;
+                b.first = (int)i;
+                b.n_min = n;
+                b.n_max = n;
+                auto it = tuned_.find({b.kind, key});
+                if (it != tuned_.end()) {
+                    b.block = it->second.first;
+                    b.chunk = it->second.second;
+                } else {
+                    b.block = default_block(key);
+                    b.chunk = default_chunk(key);
+                }
+                buckets_.push_back(b);
+            }
+            FleetBucket& b = buckets_.back();
+            ++b.count;
+            b.n_min = std::min(b.n_min, n);
+            b.n_max = std::max(b.n_max, n);
+            b.max_Pi = std::max(b.max_Pi, d.Pi);
+            b.lds = std::max(b.lds, step_lds_bytes(d));
+        } else {
+            if (groups_.empty() || groups_.back().kind != d.prob.kind) {
+                FleetGroup gr{};
+                gr.kind = d.prob.kind;
+                gr.first = (int)i;
+                groups_.push_back(gr);
+            }
+            FleetGroup& gr = groups_.back();
+            d.slot_begin = gr.rows;
+            gr.rows += d.pop;
+            ++gr.count;
+            gr.block = std::max(gr.block, p->block);
+            gr.lds = std::max(gr.lds, p->step_lds);
+            gr.max_pop = std::max(gr.max_pop, d.pop);
+        }
+        descs_[i] = d;
+    }
+    for (auto& b : buckets_) {
+        // the decoders hold up to 16 keys per thread
+        while (b.block * kSortItems < b.n_max) b.block <<= 1;
+        if ((size_t)b.lds > eng_->lds_limit)
+            return set_error(BRKGA_ERR_INVALID_ARG, "fleet member too large for the LDS kernels");
+    }
+    drop_graphs();
+    brkga_status st = reserve(reinterpret_cast<void**>(&d_descs_), &descs_cap_, sizeof(InstDesc) * descs_.size(),
+                              "fleet descriptors");
+    if (st != BRKGA_OK) return st;
+    st = reserve(reinterpret_cast<void**>(&d_queues_), &queues_cap_, sizeof(int) * kQueueStride * buckets_.size(),
+                 "fleet queues");
+    if (st != BRKGA_OK) return st;
+    if (!descs_.empty())
+        BRKGA_HIP_TRY(hipMemcpyAsync(d_descs_, descs_.data(), sizeof(InstDesc) * descs_.size(),
+                                     hipMemcpyHostToDevice, s));
+    BRKGA_HIP_TRY(hipMemsetAsync(d_queues_, 0, queues_cap_, s));
+    return build_items(s);
+}
+
+brkga_status Fleet::build_items(hipStream_t s)
+{
+    std::vector<int4> items;
+    for (auto& b : buckets_) {
+        b.item_begin = (int)items.size();
+        for (int m = b.first; m < b.first + b.count; ++m) {
+            const InstDesc& d = descs_[m];
+            for (int isl = 0; isl < d.islands; ++isl)
+                for (int s0 = 0; s0 < d.Pi; s0 += b.chunk)
+                    items.push_back(make_int4(m, isl * d.Pi + s0, isl * d.Pi + std::min(s0 + b.chunk, d.Pi), 0));
+        }
+        b.item_count = (int)items.size() - b.item_begin;
+        const int per_cu = step_queue_blocks_per_cu(b.kind, b.block, b.lds);
+        if (per_cu <= 0) return set_error(BRKGA_ERR_DEVICE, "queue step kernel does not fit on a CU");
+        b.grid = std::min(b.item_count, per_cu * eng_->cu_count);
+    }
+    brkga_status st =
+        reserve(reinterpret_cast<void**>(&d_items_), &items_cap_, sizeof(int4) * items.size(), "fleet work items");
+    if (st != BRKGA_OK) return st;
+    if (!items.empty())
+        BRKGA_HIP_TRY(hipMemcpyAsync(d_items_, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, s));
+    // the host vectors die here: wait for the copies
+    BRKGA_HIP_TRY(hipStreamSynchronize(s));
+    return BRKGA_OK;
+}
+
+brkga_status Fleet::set_counters(int gen, hipStream_t s)
+{
+    DeviceGuard g(eng_->device);
+    if (descs_.empty()) return BRKGA_OK;
+    hipError_t e = launch_set_counters(d_descs_, (int)descs_.size(), gen, s);
+    if (e != hipSuccess) return hip_error(e, "set counters");
+    return BRKGA_OK;
+}
+
+brkga_status Fleet::tick(hipStream_t s)
+{
+    DeviceGuard g(eng_->device);
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < buckets_.size() && e == hipSuccess; ++i) {
+        const FleetBucket& b = buckets_[i];
+        e = launch_step_queue(b.kind, b.block, d_descs_, d_items_ + b.item_begin, b.item_count,
+                              d_queues_ + kQueueStride * i, b.grid, b.lds, s);
+    }
+    if (e == hipSuccess && n_brkga_ > 0)
+        e = launch_partition_many(d_descs_, n_brkga_, total_islands_, max_Pi_, -1, 1, s);
+    for (size_t i = 0; i < groups_.size() && e == hipSuccess; ++i) {
+        const FleetGroup& gr = groups_[i];
+        const InstDesc* d = d_descs_ + gr.first;
+        e = launch_nsga2_offspring(gr.kind, gr.block, d, gr.count, gr.rows, gr.lds, 0, s);
+        if (e == hipSuccess) e = launch_nsga2_select(d, gr.count, 0, gr.max_pop, s);
+        if (e == hipSuccess) e = launch_nsga2_gather(d, gr.count, gr.rows, s);
+    }
+    if (e != hipSuccess) return hip_error(e, "fleet tick");
+    return BRKGA_OK;
+}
+
+brkga_status Fleet::tick_graph(int ticks, hipStream_t s)
+{
+    if (ticks <= 0) return BRKGA_OK;
+    DeviceGuard g(eng_->device);
+    auto it = graphs_.find(ticks);
+    if (it == graphs_.end()) {
+        if (!capture_) BRKGA_HIP_TRY(hipStreamCreateWithFlags(&capture_, hipStreamNonBlocking));
+        hipGraph_t graph = nullptr;
+        BRKGA_HIP_TRY(hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal));
+        brkga_status st = BRKGA_OK;
+        for (int i = 0; i < ticks && st == BRKGA_OK; ++i) st = tick(capture_);
+        hipError_t e = hipStreamEndCapture(capture_, &graph);
+        if (st != BRKGA_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return st;
+        }
+        if (e != hipSuccess) return hip_error(e, "hipStreamEndCapture");
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return hip_error(e, "hipGraphInstantiate");
+        it = graphs_.emplace(ticks, exec).first;
+    }
+    BRKGA_HIP_TRY(hipGraphLaunch(it->second, s));
+    return BRKGA_OK;
+}
+
+brkga_status Fleet::autotune(int reps, hipStream_t s)
+{
+    DeviceGuard g(eng_->device);
+    if (reps < 1) reps = 1;
+    hipEvent_t a = nullptr, b = nullptr;
+    BRKGA_HIP_TRY(hipEventCreate(&a));
+    hipError_t e = hipEventCreate(&b);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return hip_error(e, "hipEventCreate");
+    }
+    brkga_status st = BRKGA_OK;
+    for (size_t bi = 0; bi < buckets_.size() && st == BRKGA_OK; ++bi) {
+        float best_ms = 0.0f;
+        int best_block = buckets_[bi].block, best_chunk = buckets_[bi].chunk;
+        bool have = false;
+        for (int blk : kBlocks) {
+            if (blk * kSortItems < buckets_[bi].n_max) continue;
+            if (step_queue_blocks_per_cu(buckets_[bi].kind, blk, buckets_[bi].lds) <= 0) continue;
+            for (int ch : kChunks) {
+                if (ch > buckets_[bi].max_Pi) continue;
+                buckets_[bi].block = blk;
+                buckets_[bi].chunk = ch;
+                if ((st = build_items(s)) != BRKGA_OK) break;
+                const FleetBucket& bk = buckets_[bi];
+                int* q = d_queues_ + kQueueStride * bi;
+                e = launch_step_queue(bk.kind, bk.block, d_descs_, d_items_ + bk.item_begin, bk.item_count, q,
+                                      bk.grid, bk.lds, s);
+                if (e == hipSuccess) e = hipEventRecord(a, s);
+                for (int r = 0; r < reps && e == hipSuccess; ++r)
+                    e = launch_step_queue(bk.kind, bk.block, d_descs_, d_items_ + bk.item_begin, bk.item_count, q,
+                                          bk.grid, bk.lds, s);
+                if (e == hipSuccess) e = hipEventRecord(b, s);
+                if (e == hipSuccess) e = hipEventSynchronize(b);
+                float ms = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+                if (e != hipSuccess) {
+                    st = hip_error(e, "autotune");
+                    break;
+                }
+                if (!have || ms < best_ms) {
+                    have = true;
+                    best_ms = ms;
+                    best_block = blk;
+                    best_chunk = ch;
+                }
+            }
+            if (st != BRKGA_OK) break;
+        }
+        buckets_[bi].block = best_block;
+        buckets_[bi].chunk = best_chunk;
+        tuned_[{buckets_[bi].kind, buckets_[bi].key}] = {best_block, best_chunk};
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (st != BRKGA_OK) return st;
+    drop_graphs();
+    return build_items(s);
+}
+
+}  // namespace brkga

@@ -1,0 +1,92 @@
+// fleet.hpp - a set of resident populations advanced together, one
+// generation of every member per tick: the engine under the batched and the
+// served forms.
+//
+// BRKGA members are grouped into buckets by (problem kind, n rounded up to
+// a power of two); each bucket runs one persistent queue-step launch per
+// tick with its own workgroup size and work-item size (slots per item),
+// which the autotuner picks by timing.  One partition launch covers every
+// BRKGA member; NSGA-II members run the offspring / selection / gather
+// launches per problem kind.  Members may sit at different generations:
+// each kernel reads the member's step counter (-1 = build generation 0) and
+// leaves members at their generation limit untouched, so a member's result
+// does not depend on which other members share its ticks.
+#pragma once
+
+#include <map>
+#include <utility>
+#include <vector>
+
+#include "capi_internal.hpp"
+
+namespace brkga {
+
+struct FleetBucket {
+    int kind;
+    int key;           // smallest power of two >= n (at least 64) shared by the members
+    int first, count;  // members [first, first + count) in the descriptor array
+    int n_min, n_max, max_Pi;
+    int lds;           // largest step LDS footprint among the members
+    int block, chunk;  // launch shape; results do not depend on it
+    int item_begin, item_count, grid;
+};
+
+struct FleetGroup {  // NSGA-II members of one problem kind
+    int kind, first, count, rows, block, lds, max_pop;
+};
+
+class Fleet {
+public:
+    explicit Fleet(brkga_engine_s* engine) : eng_(engine) {}
+    ~Fleet();
+    Fleet(const Fleet&) = delete;
+    Fleet& operator=(const Fleet&) = delete;
+
+    // Replaces the member list (populations are borrowed).  Uploads the
+    // descriptors and work items on stream s and waits for the upload.
+    brkga_status set_members(const std::vector<brkga_pop_s*>& pops, hipStream_t s);
+    // Sets every member's step counter to `gen` (-1: the next tick builds
+    // generation 0).
+    brkga_status set_counters(int gen, hipStream_t s);
+    // One generation of every member below its limit.
+    brkga_status tick(hipStream_t s);
+    // `ticks` ticks replayed from a captured graph (cached per count).
+    brkga_status tick_graph(int ticks, hipStream_t s);
+    // Times every admissible (workgroup size, slots per item) pair of each
+    // bucket on the members' current state and keeps the fastest.  Runs
+    // step launches only, which rewrite the same next-generation buffers
+    // from unchanged inputs, so the populations are not advanced.
+    brkga_status autotune(int reps, hipStream_t s);
+
+    const std::vector<FleetBucket>& buckets() const { return buckets_; }
+    const std::vector<brkga_pop_s*>& members() const { return members_; }
+    size_t size() const { return members_.size(); }
+
+private:
+    brkga_status build_items(hipStream_t s);
+    brkga_status reserve(void** p, size_t* cap, size_t bytes, const char* what);
+    void drop_graphs();
+
+    brkga_engine_s* eng_;
+    std::vector<brkga_pop_s*> members_;  // BRKGA members (bucket order), then NSGA-II groups
+    std::vector<InstDesc> descs_;
+    std::vector<FleetBucket> buckets_;
+    std::vector<FleetGroup> groups_;
+    int n_brkga_ = 0, total_islands_ = 0, max_Pi_ = 0;
+    // launch shape per (kind, key), kept across member changes
+    std::map<std::pair<int, int>, std::pair<int, int>> tuned_;
+
+    InstDesc* d_descs_ = nullptr;
+    size_t descs_cap_ = 0;
+    int4* d_items_ = nullptr;
+    size_t items_cap_ = 0;
+    int* d_queues_ = nullptr;  // two counters per bucket, 128 bytes apart
+    size_t queues_cap_ = 0;
+    hipStream_t capture_ = nullptr;
+    std::map<int, hipGraphExec_t> graphs_;
+};
+
+// Power-of-two size class of n used to bucket instances (at least 64).
+int fleet_key(int n);
+
+}  // namespace brkga

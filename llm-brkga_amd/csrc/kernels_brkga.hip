@@ -2,7 +2,7 @@
 //
 //   k_evaluate       decode + fitness of caller-owned key rows: the
 //                    evaluate_population_gpu hook (core/config.hpp:50-53).
-//   k_step_*         one workgroup per child slot: elite / migrant row copy,
+//   k_step_one       one workgroup per child slot: elite / migrant row copy,
 //                    mutant keys from Philox, or biased uniform crossover,
 //                    then decode + fitness of the new row straight from LDS.
 //                    Replaces brkga_generation_kernel (core/cuda_kernels.cuh:88-138)
@@ -12,7 +12,7 @@
 //                    replaces thrust::sequence + sort_by_key (core/solver.hpp:416-419).
 //
 // The *_one kernels take the population descriptor by value (kernel
-// arguments), the *_many kernels read one descriptor per instance from
+// arguments); k_partition_many reads one descriptor per instance from
 // device memory (batched and served populations).  `cur` is the buffer
 // parity of the parent generation, known to the launcher; the generation
 // counter in device memory only feeds the Philox counter and the migration
@@ -21,46 +21,9 @@
 // Key rows are [pop][ld] FP32 with ld a multiple of 32, so every row starts
 // 128-byte aligned and is streamed as float4.
 #include "launch.hpp"
-#include "decoders.hpp"
+#include "step_device.hpp"
 
 namespace brkga {
-
-__device__ __forceinline__ int find_instance_by_slot(const InstDesc* __restrict__ d, int num, int b)
-{
-    int lo = 0, hi = num - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (d[mid].slot_begin <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ int find_instance_by_seg(const InstDesc* __restrict__ d, int num, int b)
-{
-    int lo = 0, hi = num - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (d[mid].seg_begin <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-// Order-preserving FP32 -> u32 (smaller = better): -0 == +0, NaN last.
-__device__ __forceinline__ uint32_t fitness_order_key(float f, int maximize)
-{
-    if (f != f) return 0xFFFFFFFFu;
-    if (f == 0.0f) f = 0.0f;
-    uint32_t u = __float_as_uint(f);
-    u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-    return maximize ? ~u : u;
-}
-
-__device__ __forceinline__ float4 unit4(const U4& r)
-{
-    return make_float4(unit_from_u32(r.x), unit_from_u32(r.y), unit_from_u32(r.z), unit_from_u32(r.w));
-}
 
 template <int KIND, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_evaluate(ProblemDesc p, const float* __restrict__ keys, long long ld,
@@ -85,7 +48,7 @@ template <int KIND, int BLOCK>
 __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, int init, char* lds)
 {
     constexpr bool kXY = KIND == KIND_TSP_COORDS;
-    const int Pi = d.Pi, Ei = d.Ei, Mi = d.Mi, I = d.islands;
+    const int Pi = d.Pi, Ei = d.Ei, I = d.islands;
     const int island = gs / Pi;
     const int s = gs - island * Pi;
     const int nxt = init ? 0 : cur ^ 1;
@@ -134,56 +97,8 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
     argsort_clear<BLOCK>(ss, nb);
     __syncthreads();
 
-    if (init) {
-        for (int c = threadIdx.x; c < nq; c += BLOCK) {
-            const float4 x = unit4(rng_block(d.seed_lo, d.seed_hi, island, 0, s, TAG_INIT, c));
-            out[c] = x;
-            kl[c] = x;
-        }
-    } else {
-        const uint32_t g = (uint32_t)t + 1u;
-        const bool migrate = maybe_migrant && t > 0 && (t % d.mig_interval) == 0;
-        int src = -1;
-        if (migrate) {
-            const int si = (island + I - 1) % I;
-            src = si * Pi + d.perm[cur][si * Pi + (s - (Pi - k))];
-        } else if (s < Ei) {
-            src = island * Pi + (staged ? order[s] : perm_cur[s]);
-        }
-        if (src >= 0) {
-            const float4* in = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)src * ld);
-            for (int c = threadIdx.x; c < nq; c += BLOCK) out[c] = in[c];
-            if (threadIdx.x < M) fit_out[threadIdx.x] = d.fit[cur][(size_t)src * M + threadIdx.x];
-            return;
-        }
-        if (s >= Pi - Mi) {
-            for (int c = threadIdx.x; c < nq; c += BLOCK) {
-                const float4 x = unit4(rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_MUTANT, c));
-                out[c] = x;
-                kl[c] = x;
-            }
-        } else {
-            const U4 pr = rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_PARENTS, 0);
-            const int ei = (int)(pr.x % (uint32_t)Ei);
-            const int ni = Ei + (int)(pr.y % (uint32_t)(Pi - Ei));
-            const int ep = island * Pi + (staged ? order[ei] : perm_cur[ei]);
-            const int np = island * Pi + (staged ? order[ni] : perm_cur[ni]);
-            const float4* pe = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)ep * ld);
-            const float4* pn = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)np * ld);
-            const float rho = d.rho;
-            for (int c = threadIdx.x; c < nq; c += BLOCK) {
-                const U4 r = rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_COINS, c);
-                const float4 a = pe[c], b = pn[c];
-                float4 x;
-                x.x = unit_from_u32(r.x) < rho ? a.x : b.x;
-                x.y = unit_from_u32(r.y) < rho ? a.y : b.y;
-                x.z = unit_from_u32(r.z) < rho ? a.z : b.z;
-                x.w = unit_from_u32(r.w) < rho ? a.w : b.w;
-                out[c] = x;
-                kl[c] = x;
-            }
-        }
-    }
+    const bool child = build_child<BLOCK>(d, island, s, cur, init, t, staged ? order : perm_cur, out, kl, fit_out);
+    if (!child) return;
     if constexpr (kXY) {
 #pragma unroll
         for (int k = 0; k < kSortItems; ++k) {
@@ -200,23 +115,6 @@ __global__ __launch_bounds__(BLOCK) void k_step_one(InstDesc d, int cur, int ini
 {
     extern __shared__ __align__(16) char lds[];
     step_body<KIND, BLOCK>(d, blockIdx.x, cur, init, lds);
-}
-
-template <int KIND, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_step_many(const InstDesc* __restrict__ descs, int num_inst, int cur,
-                                                     int init)
-{
-    extern __shared__ __align__(16) char lds[];
-    const InstDesc& d = descs[find_instance_by_slot(descs, num_inst, blockIdx.x)];
-    const int gs = blockIdx.x - d.slot_begin;
-    if (gs >= d.pop) return;
-    int c = cur;
-    if (!init) {
-        const int t = *d.gen;
-        if (t >= d.gen_limit) return;
-        if (c < 0) c = t & 1;
-    }
-    step_body<KIND, BLOCK>(d, gs, c, init, lds);
 }
 
 constexpr int kPartBlock = 1024;
@@ -312,6 +210,8 @@ __global__ __launch_bounds__(kPartBlock) void k_partition_many(const InstDesc* _
     if (island >= d.islands) return;
     int t = 0, b = buf;
     if (advance) {
+        // t = -1: generation 0 was just built into buffer 0; the counter
+        // moves to 0 like after any other step
         t = *d.gen;
         if (t >= d.gen_limit) return;
         if (b < 0) b = (t + 1) & 1;
@@ -371,46 +271,15 @@ hipError_t step_single(const InstDesc& d, int lds, int cur, int init, hipStream_
     return hipGetLastError();
 }
 
-template <int KIND, int BLOCK>
-hipError_t step_multi(const InstDesc* d, int num, int blocks, int lds, int cur, int init, hipStream_t s)
-{
-    ensure_lds(k_step_many<KIND, BLOCK>, lds);
-    hipLaunchKernelGGL((k_step_many<KIND, BLOCK>), dim3(blocks), dim3(BLOCK), lds, s, d, num, cur, init);
-    return hipGetLastError();
-}
-
 template <int KIND>
-hipError_t step_kind(int block, const InstDesc* one, const InstDesc* many, int num, int blocks, int lds, int cur,
-                     int init, hipStream_t s)
+hipError_t step_kind(int block, const InstDesc& d, int lds, int cur, int init, hipStream_t s)
 {
-#define BRKGA_STEP_CASE(B)                                                       \
-    case B:                                                                      \
-        return one ? step_single<KIND, B>(*one, lds, cur, init, s)               \
-                   : step_multi<KIND, B>(many, num, blocks, lds, cur, init, s);
     switch (block) {
-        BRKGA_STEP_CASE(64)
-        BRKGA_STEP_CASE(128)
-        BRKGA_STEP_CASE(256)
-        BRKGA_STEP_CASE(512)
-        BRKGA_STEP_CASE(1024)
-    default: return hipErrorInvalidValue;
-    }
-#undef BRKGA_STEP_CASE
-}
-
-hipError_t step_dispatch(int kind, int block, const InstDesc* one, const InstDesc* many, int num, int blocks,
-                         int lds, int cur, int init, hipStream_t s)
-{
-    switch (kind) {
-    case KIND_TSP_COORDS:
-        return step_kind<KIND_TSP_COORDS>(block, one, many, num, blocks, lds, cur, init, s);
-    case KIND_TSP_MATRIX:
-        return step_kind<KIND_TSP_MATRIX>(block, one, many, num, blocks, lds, cur, init, s);
-    case KIND_KNAP_THRESHOLD:
-        return step_kind<KIND_KNAP_THRESHOLD>(block, one, many, num, blocks, lds, cur, init, s);
-    case KIND_KNAP_GREEDY:
-        return step_kind<KIND_KNAP_GREEDY>(block, one, many, num, blocks, lds, cur, init, s);
-    case KIND_TSPJ: return step_kind<KIND_TSPJ>(block, one, many, num, blocks, lds, cur, init, s);
+    case 64: return step_single<KIND, 64>(d, lds, cur, init, s);
+    case 128: return step_single<KIND, 128>(d, lds, cur, init, s);
+    case 256: return step_single<KIND, 256>(d, lds, cur, init, s);
+    case 512: return step_single<KIND, 512>(d, lds, cur, init, s);
+    case 1024: return step_single<KIND, 1024>(d, lds, cur, init, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -442,14 +311,15 @@ int step_lds_bytes(const InstDesc& d)
 
 hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s)
 {
-    return step_dispatch(d.prob.kind, block, &d, nullptr, 1, d.pop, step_lds_bytes(d), cur, init, s);
-}
-
-hipError_t launch_step_many(int kind, int block, const InstDesc* d_descs, int num_inst, int blocks, int lds, int cur,
-                            int init, hipStream_t s)
-{
-    if (blocks <= 0) return hipSuccess;
-    return step_dispatch(kind, block, nullptr, d_descs, num_inst, blocks, lds, cur, init, s);
+    const int lds = step_lds_bytes(d);
+    switch (d.prob.kind) {
+    case KIND_TSP_COORDS: return step_kind<KIND_TSP_COORDS>(block, d, lds, cur, init, s);
+    case KIND_TSP_MATRIX: return step_kind<KIND_TSP_MATRIX>(block, d, lds, cur, init, s);
+    case KIND_KNAP_THRESHOLD: return step_kind<KIND_KNAP_THRESHOLD>(block, d, lds, cur, init, s);
+    case KIND_KNAP_GREEDY: return step_kind<KIND_KNAP_GREEDY>(block, d, lds, cur, init, s);
+    case KIND_TSPJ: return step_kind<KIND_TSPJ>(block, d, lds, cur, init, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_partition_one(const InstDesc& d, int buf, int advance, hipStream_t s)

@@ -77,8 +77,10 @@ __global__ __launch_bounds__(BLOCK) void k_nsga2_offspring(const InstDesc* __res
     const int c = b - d.slot_begin;  // child (or, at init, parent) index
     const int N = d.pop;
     if (c >= N) return;
-    const int t = init ? 0 : *d.gen;
-    if (!init && t >= d.gen_limit) return;
+    // step counter -1 (or the init flag): build the first parents
+    const int t = init ? -1 : *d.gen;
+    if (t >= d.gen_limit) return;
+    init = t < 0;
     const int cur = t & 1;
     const int row = init ? c : N + c;
     const int ld = d.ld, L = d.L;
@@ -168,8 +170,9 @@ __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* _
     extern __shared__ __align__(16) char lds[];
     const InstDesc& d = descs[blockIdx.x];
     const int N = d.pop;
-    const int t = init ? 0 : *d.gen;
-    if (!init && t >= d.gen_limit) return;
+    const int t = init ? -1 : *d.gen;
+    if (t >= d.gen_limit) return;
+    init = t < 0;
     const int cur = init ? 0 : (t & 1);
     const int C = init ? N : 2 * N;
     const int Cp = pow2_at_least(C < 2 ? 2 : C);
@@ -347,13 +350,17 @@ __global__ __launch_bounds__(256) void k_nsga2_gather(const InstDesc* __restrict
     if (i >= N) return;
     const int t = *d.gen;
     if (t >= d.gen_limit) return;
-    const int cur = t & 1, nxt = cur ^ 1;
-    const int src = d.perm[cur][i];
-    const int nq = (d.L + 3) >> 2;
-    const float4* in = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)src * d.ld);
-    float4* out = reinterpret_cast<float4*>(d.keys[nxt] + (size_t)i * d.ld);
-    for (int q = threadIdx.x; q < nq; q += 256) out[q] = in[q];
-    if (threadIdx.x < 2) d.fit[nxt][2 * i + threadIdx.x] = d.fit[cur][2 * src + threadIdx.x];
+    if (t >= 0) {
+        // at t = -1 the first parents are already in place; only the
+        // counter moves to 0
+        const int cur = t & 1, nxt = cur ^ 1;
+        const int src = d.perm[cur][i];
+        const int nq = (d.L + 3) >> 2;
+        const float4* in = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)src * d.ld);
+        float4* out = reinterpret_cast<float4*>(d.keys[nxt] + (size_t)i * d.ld);
+        for (int q = threadIdx.x; q < nq; q += 256) out[q] = in[q];
+        if (threadIdx.x < 2) d.fit[nxt][2 * i + threadIdx.x] = d.fit[cur][2 * src + threadIdx.x];
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();

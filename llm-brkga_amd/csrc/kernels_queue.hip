@@ -1,0 +1,157 @@
+// kernels_queue.hip - persistent, queue-driven generation step of many
+// resident populations (batched and served instances): the batched
+// counterpart of k_step_one, built on the same child builder and decoders,
+// so every instance evolves exactly as it would alone.
+#include "launch.hpp"
+#include "step_device.hpp"
+
+namespace brkga {
+
+// Persistent generation step over a work queue (batched and served
+// populations).  A work item {instance, first slot, end slot, -} covers
+// consecutive child slots of one island; items are handed out largest
+// instance first.  The workgroup stages the island's rank order and, for
+// Euclidean instances, the coordinates once per item, then builds and
+// decodes the item's children one after another.  Each instance's step
+// counter selects its buffers: -1 builds generation 0 into buffer 0,
+// t >= gen_limit leaves the instance untouched.  queue[0] hands out items
+// and queue[1] counts retired workgroups; the last workgroup re-arms both,
+// so the kernel can be relaunched (or replayed from a graph) as is.
+template <int KIND, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_step_queue(const InstDesc* __restrict__ descs,
+                                                      const int4* __restrict__ items, int num_items,
+                                                      int* __restrict__ queue)
+{
+    extern __shared__ __align__(16) char lds[];
+    __shared__ int s_item;
+    constexpr bool kXY = KIND == KIND_TSP_COORDS;
+    for (;;) {
+        __syncthreads();  // the previous item is done with LDS and s_item
+        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+        __syncthreads();
+        const int it = s_item;
+        if (it >= num_items) break;
+        const int4 w = items[it];
+        const InstDesc& d = descs[w.x];
+        const int t = *d.gen;
+        if (t >= d.gen_limit) continue;
+        const int init = t < 0;
+        const int cur = init ? 0 : (t & 1);
+        const int nxt = init ? 0 : cur ^ 1;
+        const int Pi = d.Pi, n = d.prob.n, M = d.prob.num_obj, ld = d.ld, nb = d.nb;
+        const int island = w.y / Pi;
+        const StepLayout sl = step_layout(KIND, n, d.L, nb, Pi);
+        float4* kl = reinterpret_cast<float4*>(lds + sl.dec.keys_off);
+        float2* xy = reinterpret_cast<float2*>(lds + sl.xy_off);
+        int* order = reinterpret_cast<int*>(lds + sl.isl_off);
+        const bool staged = Pi <= kMaxStagedOrder;
+        const int* perm_cur = d.perm[cur] + (size_t)island * Pi;
+        SortScratch ss;
+        ss.cnt = reinterpret_cast<int*>(lds + sl.dec.cnt_off);
+        ss.red = reinterpret_cast<long long*>(lds + sl.dec.red_off);
+        if constexpr (kXY) {
+            const float2* gxy = reinterpret_cast<const float2*>(d.prob.a);
+            for (int i = threadIdx.x; i < n; i += BLOCK) xy[i] = gxy[i];
+        }
+        if (!init && staged)
+            for (int i = threadIdx.x; i < Pi; i += BLOCK) order[i] = perm_cur[i];
+        for (int gs = w.y; gs < w.z; ++gs) {
+            const int s = gs - island * Pi;
+            float4* out = reinterpret_cast<float4*>(d.keys[nxt] + (size_t)gs * ld);
+            float* fit_out = d.fit[nxt] + (size_t)gs * M;
+            __syncthreads();  // staging done; the previous child's decode is done with LDS
+            argsort_clear<BLOCK>(ss, nb);
+            const bool child =
+                build_child<BLOCK>(d, island, s, cur, init, t, staged ? order : perm_cur, out, kl, fit_out);
+            if (!child) continue;
+            __syncthreads();
+            decode_block<KIND, BLOCK, true, kXY>(d.prob, lds, sl.dec, nb, fit_out, nullptr, xy, nullptr);
+        }
+    }
+    if (threadIdx.x == 0) {
+        const int retired = atomicAdd(queue + 1, 1);
+        if (retired == (int)gridDim.x - 1) {
+            atomicExch(queue, 0);
+            atomicExch(queue + 1, 0);
+        }
+    }
+}
+
+// Sets the step counter of every listed instance to `gen` and clears its
+// partition arrival counter.
+__global__ __launch_bounds__(256) void k_set_counters(const InstDesc* __restrict__ descs, int num, int gen)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= num) return;
+    *descs[i].gen = gen;
+    *descs[i].done = 0;
+}
+
+namespace {
+
+template <typename F>
+void ensure_lds(F func, int bytes)
+{
+    if (bytes > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(func), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  bytes);
+}
+
+// Kernel symbol of the queue step for (kind, block); nullptr when the pair
+// has no instantiation.
+template <int KIND>
+const void* queue_kernel_kind(int block)
+{
+    switch (block) {
+    case 64: return reinterpret_cast<const void*>(k_step_queue<KIND, 64>);
+    case 128: return reinterpret_cast<const void*>(k_step_queue<KIND, 128>);
+    case 256: return reinterpret_cast<const void*>(k_step_queue<KIND, 256>);
+    case 512: return reinterpret_cast<const void*>(k_step_queue<KIND, 512>);
+    case 1024: return reinterpret_cast<const void*>(k_step_queue<KIND, 1024>);
+    default: return nullptr;
+    }
+}
+
+const void* queue_kernel(int kind, int block)
+{
+    switch (kind) {
+    case KIND_TSP_COORDS: return queue_kernel_kind<KIND_TSP_COORDS>(block);
+    case KIND_TSP_MATRIX: return queue_kernel_kind<KIND_TSP_MATRIX>(block);
+    case KIND_KNAP_THRESHOLD: return queue_kernel_kind<KIND_KNAP_THRESHOLD>(block);
+    case KIND_KNAP_GREEDY: return queue_kernel_kind<KIND_KNAP_GREEDY>(block);
+    case KIND_TSPJ: return queue_kernel_kind<KIND_TSPJ>(block);
+    default: return nullptr;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_step_queue(int kind, int block, const InstDesc* d_descs, const int4* d_items, int num_items,
+                             int* d_queue, int grid, int lds, hipStream_t s)
+{
+    const void* f = queue_kernel(kind, block);
+    if (!f) return hipErrorInvalidValue;
+    if (num_items <= 0 || grid <= 0) return hipSuccess;
+    ensure_lds(f, lds);
+    void* args[] = {(void*)&d_descs, (void*)&d_items, (void*)&num_items, (void*)&d_queue};
+    return hipLaunchKernel(f, dim3(grid), dim3(block), args, (size_t)lds, s);
+}
+
+int step_queue_blocks_per_cu(int kind, int block, int lds)
+{
+    const void* f = queue_kernel(kind, block);
+    if (!f) return 0;
+    ensure_lds(f, lds);
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, block, (size_t)lds) != hipSuccess) return 0;
+    return nb;
+}
+
+hipError_t launch_set_counters(const InstDesc* d_descs, int num, int gen, hipStream_t s)
+{
+    if (num <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_set_counters, dim3((num + 255) / 256), dim3(256), 0, s, d_descs, num, gen);
+    return hipGetLastError();
+}
+
+}  // namespace brkga

@@ -17,19 +17,22 @@ hipError_t launch_evaluate(const ProblemDesc& p, const float* keys, long long ld
 // (Philox keys + evaluation) in buffer 0.
 hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s);
 
-// Same for many populations of one problem kind, descriptors in device
-// memory; `blocks` = total population slots, `lds` = the largest
-// step_lds_bytes among them.  cur < 0: each population's parity comes from
-// its step counter (populations at different generations).
-hipError_t launch_step_many(int kind, int block, const InstDesc* d_descs, int num_inst, int blocks, int lds,
-                            int cur, int init, hipStream_t s);
-
 // Stable per-island sort of the fitness in buffer `buf` into the rank order;
 // with advance=1 it closes a generation step and bumps the step counter.
 hipError_t launch_partition_one(const InstDesc& d, int buf, int advance, hipStream_t s);
 // buf < 0: buffer from each population's step counter.
 hipError_t launch_partition_many(const InstDesc* d_descs, int num_inst, int blocks, int max_Pi, int buf, int advance,
                                  hipStream_t s);
+
+// Persistent queue step over work items {instance, first slot, end slot, -}
+// of one problem kind (batched / served populations).  d_queue holds two
+// counters that must be zero at the first launch; the kernel re-arms them.
+hipError_t launch_step_queue(int kind, int block, const InstDesc* d_descs, const int4* d_items, int num_items,
+                             int* d_queue, int grid, int lds, hipStream_t s);
+// Resident workgroups of the queue step per CU (0: no such kernel).
+int step_queue_blocks_per_cu(int kind, int block, int lds);
+// Step counter := gen and arrival counter := 0 for `num` instances.
+hipError_t launch_set_counters(const InstDesc* d_descs, int num, int gen, hipStream_t s);
 
 int step_lds_bytes(const InstDesc& d);
 int partition_lds_bytes(int Pi);
