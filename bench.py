@@ -13,6 +13,15 @@ kernel's roofline fraction (HIP-event timing of each launch, algorithmic
 bytes per generation B = 4 L (3P - M) + 12 P), and the CPU oracle timed on a
 bounded sample of the same workload.
 
+Batched leg ("batched"): 512 independent Euclidean TSP instances, n
+log-uniform in [50, 2000], coordinates uniform in [0, 1000)^2, population
+256, elite 38, mutants 38, rho 0.7, 300 generations each.  One solve =
+generation 0 + 300 steps.  Timed three ways on the same instances: the
+plain per-instance form (one instance after the other, one launch sequence
+per generation), the batched launches, and the batched launches replayed
+from a captured graph (after the autotuner picked each size bucket's launch
+shape).  Every member's best fitness must be identical across the forms.
+
 Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 50]
 """
 import argparse
@@ -67,6 +76,74 @@ def cpu_baseline(budget_s):
                       f"one thread), {dt:.1f} s"}
 
 
+BATCH_COUNT, BATCH_POP, BATCH_ELITE, BATCH_MUTANTS, BATCH_RHO, BATCH_GENS = 512, 256, 38, 38, 0.7, 300
+BATCH_N_LO, BATCH_N_HI = 50, 2000
+
+
+def batched_leg(brkga_amd, workloads, eng, gens):
+    sizes = [int(n) for n in workloads.log_uniform_sizes(BATCH_COUNT, BATCH_N_LO, BATCH_N_HI)]
+    probs = [brkga_amd.Problem.tsp_coords(eng, workloads.tsp_coords(1000 + i, n, 1000.0))
+             for i, n in enumerate(sizes)]
+    params = [brkga_amd.Params(BATCH_POP, BATCH_ELITE, BATCH_MUTANTS, BATCH_RHO, 1, 0, 0, SEED + i)
+              for i in range(BATCH_COUNT)]
+    batch = brkga_amd.Batch(eng, probs, params)
+
+    def sync():
+        return batch.member(0).generation
+
+    def bests():
+        return [batch.member(i).best()[0] for i in range(BATCH_COUNT)]
+
+    # warm-up: one batched solve, tuning, and the graph capture
+    batch.initialize("launch")
+    batch.autotune(reps=2)
+    batch.evolve(gens, "graph")
+    sync()
+
+    t0 = time.perf_counter()
+    batch.initialize("launch")
+    batch.evolve(gens, "graph")
+    sync()
+    t_graph = time.perf_counter() - t0
+    ref = bests()
+
+    t0 = time.perf_counter()
+    batch.initialize("launch")
+    batch.evolve(gens, "launch")
+    sync()
+    t_launch = time.perf_counter() - t0
+    same = bests() == ref
+
+    t0 = time.perf_counter()
+    batch.initialize("plain")
+    batch.evolve(gens, "plain")
+    sync()
+    t_plain = time.perf_counter() - t0
+    same = same and bests() == ref
+
+    B = sum(algorithmic_bytes_per_generation(n, BATCH_POP, BATCH_MUTANTS) for n in sizes) * gens
+    achieved = B / t_graph / 1e9
+    out = {
+        "workload": f"{BATCH_COUNT} Euclidean TSP instances, n log-uniform in [{BATCH_N_LO},{BATCH_N_HI}] "
+                    f"(sum n = {sum(sizes)}), coordinates uniform FP32 in [0,1000)^2, P={BATCH_POP}, "
+                    f"elite {BATCH_ELITE}, mutants {BATCH_MUTANTS}, rho {BATCH_RHO}, {gens} generations",
+        "solves_per_s": round(BATCH_COUNT / t_graph, 2),
+        "solves_per_s_launch": round(BATCH_COUNT / t_launch, 2),
+        "solves_per_s_plain": round(BATCH_COUNT / t_plain, 2),
+        "speedup_vs_plain": round(t_plain / t_graph, 2),
+        "ms_per_batch_generation": round(1000.0 * t_graph / gens, 4),
+        "plain_seconds": round(t_plain, 3),
+        "results_identical": bool(same),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": B},
+        "buckets": batch.buckets(),
+    }
+    batch.close()
+    for p in probs:
+        p.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +151,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-gens", type=int, default=BATCH_GENS)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the island model runs all islands on one device); "
@@ -143,6 +222,8 @@ def main():
         "best_tour": round(float(best), 3),
         "valid_tour": sorted(tour.tolist()) == list(range(L)),
     }
+    if not args.no_batch:
+        out["batched"] = batched_leg(brkga_amd, workloads, eng, args.batch_gens)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))

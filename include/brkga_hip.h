@@ -255,6 +255,66 @@ brkga_status brkga_batch_num_buckets(brkga_batch_t batch, int32_t* count);
 brkga_status brkga_batch_bucket_info(brkga_batch_t batch, int32_t bucket, int32_t* kind, int32_t* n_min,
                                      int32_t* n_max, int32_t* members, int32_t* block, int32_t* chunk);
 
+/* ------------------------------------------------------------ serving --- */
+
+/* A request loop over one device (no reference counterpart: the reference
+ * solves one instance per process run, main.cu:46-305).  Requests of any
+ * problem kind and size are admitted first-come first-served while their
+ * populations fit the HBM budget (and the resident cap); resident requests
+ * advance together, one generation each per tick, through the batched
+ * kernels, and each retires at its own generation deadline with its best
+ * individual.  Requests that arrive while others run join at the next
+ * admission round.  A request's result does not depend on the others it
+ * shared the device with. */
+typedef struct brkga_server_s* brkga_server_t;
+
+typedef struct {
+    uint64_t hbm_budget_bytes; /* device arena for resident populations (0: 8 GiB) */
+    int32_t max_resident;      /* most requests on the device at once (0: no cap) */
+    int32_t ticks_per_round;   /* generations per resident request between admission rounds (0: 4) */
+} brkga_server_config;
+
+typedef struct {
+    double submit_us; /* server clock: microseconds since brkga_server_start */
+    double admit_us;
+    double done_us;
+    int32_t generations; /* generation steps completed */
+    int32_t status;      /* brkga_status of the request */
+    uint64_t device_bytes; /* arena bytes the request held while resident */
+} brkga_request_info;
+
+typedef struct {
+    uint64_t submitted, admitted, completed, rejected;
+    int32_t resident, pending;
+    uint64_t bytes_in_use, peak_bytes, budget_bytes;
+    uint64_t ticks;   /* batched ticks run */
+    uint64_t rounds;  /* admission rounds */
+} brkga_server_stats;
+
+/* Starts the worker thread (config may be NULL for defaults). */
+brkga_status brkga_server_start(brkga_engine_t engine, const brkga_server_config* config, brkga_server_t* out);
+
+/* Queues a solve of `problem` with `params` for `generations` steps.
+ * own_problem = 1 hands the problem to the server, which destroys it once
+ * the request is done; otherwise the caller keeps it alive until then.
+ * A request whose population can never fit the budget is rejected with
+ * BRKGA_ERR_OOM.  Thread-safe. */
+brkga_status brkga_server_submit(brkga_server_t server, brkga_problem_t problem, const brkga_params* params,
+                                 int32_t generations, int32_t own_problem, uint64_t* request_id);
+
+/* Collects a finished request: best fitness[num_obj] and decoded
+ * solution[chrom_len] (either may be NULL) and its timings.  Waits up to
+ * timeout_ms (0: poll, < 0: no limit); BRKGA_ERR_BUSY if still running,
+ * BRKGA_ERR_NOT_FOUND for an unknown or already collected id.  Thread-safe. */
+brkga_status brkga_server_wait(brkga_server_t server, uint64_t request_id, int32_t timeout_ms, float* fitness,
+                               int32_t* decoded, brkga_request_info* info);
+
+brkga_status brkga_server_stats_get(brkga_server_t server, brkga_server_stats* stats);
+
+/* Finishes every submitted request, stops the worker and frees the server;
+ * results not collected by then are dropped. */
+brkga_status brkga_server_stop(brkga_server_t server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,23 @@ class Params:
                        self.seed & 0xFFFFFFFFFFFFFFFF)
 
 
+class CServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_resident", ctypes.c_int32),
+                ("ticks_per_round", ctypes.c_int32)]
+
+
+class CRequestInfo(ctypes.Structure):
+    _fields_ = [("submit_us", ctypes.c_double), ("admit_us", ctypes.c_double), ("done_us", ctypes.c_double),
+                ("generations", ctypes.c_int32), ("status", ctypes.c_int32), ("device_bytes", ctypes.c_uint64)]
+
+
+class CServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("admitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint64), ("resident", ctypes.c_int32), ("pending", ctypes.c_int32),
+                ("bytes_in_use", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64),
+                ("budget_bytes", ctypes.c_uint64), ("ticks", ctypes.c_uint64), ("rounds", ctypes.c_uint64)]
+
+
 _lib = None
 
 _P = ctypes.c_void_p
@@ -128,6 +145,12 @@ _SIGNATURES = {
     "brkga_batch_population": ([_P, _I32, ctypes.POINTER(_P)], ctypes.c_int),
     "brkga_batch_num_buckets": ([_P, _PI], ctypes.c_int),
     "brkga_batch_bucket_info": ([_P, _I32, _PI, _PI, _PI, _PI, _PI, _PI], ctypes.c_int),
+    "brkga_server_start": ([_P, ctypes.POINTER(CServerConfig), ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_server_submit": ([_P, _P, ctypes.POINTER(CParams), _I32, _I32, ctypes.POINTER(ctypes.c_uint64)],
+                            ctypes.c_int),
+    "brkga_server_wait": ([_P, ctypes.c_uint64, _I32, _PF, _PI, ctypes.POINTER(CRequestInfo)], ctypes.c_int),
+    "brkga_server_stats_get": ([_P, ctypes.POINTER(CServerStats)], ctypes.c_int),
+    "brkga_server_stop": ([_P], ctypes.c_int),
 }
 
 
@@ -432,5 +455,62 @@ class Batch:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop with continuous batching (brkga_server_*).
+
+    submit() queues a solve and returns its id; wait() returns
+    (fitness, decoded, info) once the request reached its generation
+    deadline.  Populations are admitted while they fit `hbm_budget_bytes`."""
+
+    def __init__(self, engine: Engine, hbm_budget_bytes: int = 0, max_resident: int = 0,
+                 ticks_per_round: int = 0):
+        self.engine = engine
+        cfg = CServerConfig(hbm_budget_bytes, max_resident, ticks_per_round)
+        h = _P()
+        _check(lib().brkga_server_start(engine.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self._shape = {}  # request id -> (num_obj, chrom_len)
+
+    def submit(self, problem: Problem, params: Params, generations: int, own_problem: bool = False) -> int:
+        rid = ctypes.c_uint64()
+        cp = params.to_c()
+        _check(lib().brkga_server_submit(self.handle, problem.handle, ctypes.byref(cp), generations,
+                                         1 if own_problem else 0, ctypes.byref(rid)))
+        if own_problem:
+            problem.handle = _P()  # the server destroys it
+        self._shape[rid.value] = (problem.num_obj, problem.chrom_len)
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: int = -1):
+        """(fitness, decoded, info dict); raises BrkgaError(ERR_BUSY) on timeout."""
+        M, L = self._shape.get(rid, (2, 1))
+        fit = np.empty(M, np.float32)
+        dec = np.empty(L, np.int32)
+        info = CRequestInfo()
+        st = lib().brkga_server_wait(self.handle, rid, timeout_ms, _fptr(fit), _iptr(dec), ctypes.byref(info))
+        if st == ERR_BUSY:
+            raise BrkgaError(st, "request still running")
+        _check(st)
+        self._shape.pop(rid, None)
+        d = {k: getattr(info, k) for k, _ in CRequestInfo._fields_}
+        return (fit if M > 1 else float(fit[0])), dec, d
+
+    def stats(self):
+        st = CServerStats()
+        _check(lib().brkga_server_stats_get(self.handle, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in CServerStats._fields_}
+
+    def stop(self):
+        if self.handle:
+            lib().brkga_server_stop(self.handle)
+            self.handle = _P()
+
+    def __del__(self):
+        try:
+            self.stop()
         except Exception:
             pass

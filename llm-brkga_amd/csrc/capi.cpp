@@ -208,8 +208,7 @@ brkga_status fill_inst_desc(const brkga_problem_s* pr, const brkga_params* q, In
     return BRKGA_OK;
 }
 
-brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, DeviceAllocator* a, hipStream_t s,
-                                std::vector<void*>* owned)
+brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, DeviceAllocator* a, std::vector<void*>* owned)
 {
     const size_t rows = nsga2 ? (size_t)2 * d->pop : (size_t)d->pop;
     const int M = d->prob.num_obj;
@@ -238,10 +237,62 @@ brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, DeviceAllocator* a, hipS
     if ((st = get(&c, 2 * sizeof(int), "counters")) != BRKGA_OK) return st;
     d->gen = (int*)c;
     d->done = (int*)c + 1;
-    // the padding columns are never produced by the RNG paths of every
-    // kernel; clear them once so copies stay deterministic
-    for (int b = 0; b < 2; ++b) BRKGA_HIP_TRY(hipMemsetAsync(d->keys[b], 0, rows * d->ld * sizeof(float), s));
-    BRKGA_HIP_TRY(hipMemsetAsync(c, 0, 2 * sizeof(int), s));
+    return BRKGA_OK;
+}
+
+namespace {
+
+// Allocations of one population besides the ping-pong buffers.
+brkga_status alloc_pop_extras(brkga_pop_s* pop, DeviceAllocator* a)
+{
+    auto get = [&](void** p, size_t bytes, const char* what) -> brkga_status {
+        brkga_status r = a ? a->alloc(p, bytes, what) : device_alloc(p, bytes, what);
+        if (r == BRKGA_OK) pop->owned.push_back(*p);
+        return r;
+    };
+    brkga_status st;
+    void* p = nullptr;
+    if ((st = get(&p, sizeof(InstDesc), "descriptor")) != BRKGA_OK) return st;
+    pop->d_desc = (InstDesc*)p;
+    if ((st = get(&p, sizeof(int) * pop->host.L, "decode row")) != BRKGA_OK) return st;
+    pop->d_decode_row = (int*)p;
+    if ((st = get(&p, sizeof(float) * 2, "fitness row")) != BRKGA_OK) return st;
+    pop->d_fit_row = (float*)p;
+    return BRKGA_OK;
+}
+
+// Counts bytes (at the arena's 256-byte granularity) instead of allocating.
+struct CountingAllocator : DeviceAllocator {
+    size_t bytes = 0;
+    uintptr_t next = 256;
+    brkga_status alloc(void** p, size_t n, const char*) override
+    {
+        const size_t r = (n + 255) & ~(size_t)255;
+        bytes += r;
+        *p = reinterpret_cast<void*>(next);
+        next += r;
+        return BRKGA_OK;
+    }
+    void release(void*) override {}
+};
+
+}  // namespace
+
+brkga_status population_bytes(const brkga_problem_s* pr, const brkga_params* q, size_t* bytes)
+{
+    InstDesc d;
+    int nsga2 = 0;
+    brkga_status st = fill_inst_desc(pr, q, &d, &nsga2);
+    if (st != BRKGA_OK) return st;
+    CountingAllocator c;
+    std::vector<void*> owned;
+    st = alloc_inst_buffers(&d, nsga2, &c, &owned);
+    if (st != BRKGA_OK) return st;
+    brkga_pop_s tmp;
+    tmp.host = d;
+    st = alloc_pop_extras(&tmp, &c);
+    if (st != BRKGA_OK) return st;
+    *bytes = c.bytes;
     return BRKGA_OK;
 }
 
@@ -257,17 +308,8 @@ brkga_status create_population(brkga_problem_s* problem, const brkga_params* par
     pop->d_desc = nullptr;
     pop->capture_stream = nullptr;
     brkga_status st = fill_inst_desc(problem, params, &pop->host, &pop->nsga2);
-    if (st == BRKGA_OK) st = alloc_inst_buffers(&pop->host, pop->nsga2, a, s, &pop->owned);
-    auto get = [&](void** p, size_t bytes, const char* what) -> brkga_status {
-        brkga_status r = a ? a->alloc(p, bytes, what) : device_alloc(p, bytes, what);
-        if (r == BRKGA_OK) pop->owned.push_back(*p);
-        return r;
-    };
-    void* p = nullptr;
-    if (st == BRKGA_OK && (st = get(&p, sizeof(InstDesc), "descriptor")) == BRKGA_OK) pop->d_desc = (InstDesc*)p;
-    if (st == BRKGA_OK && (st = get(&p, sizeof(int) * pop->host.L, "decode row")) == BRKGA_OK)
-        pop->d_decode_row = (int*)p;
-    if (st == BRKGA_OK && (st = get(&p, sizeof(float) * 2, "fitness row")) == BRKGA_OK) pop->d_fit_row = (float*)p;
+    if (st == BRKGA_OK) st = alloc_inst_buffers(&pop->host, pop->nsga2, a, &pop->owned);
+    if (st == BRKGA_OK) st = alloc_pop_extras(pop, a);
     if (st == BRKGA_OK) {
         const int n = problem->desc.n;
         pop->block = default_block(n);
@@ -279,9 +321,17 @@ brkga_status create_population(brkga_problem_s* problem, const brkga_params* par
             st = set_error(BRKGA_ERR_INVALID_ARG, "population or chromosome too large for the LDS kernels");
     }
     if (st == BRKGA_OK) {
-        // pop->host outlives the copy, so the asynchronous upload is safe
-        hipError_t e = hipMemcpyAsync(pop->d_desc, &pop->host, sizeof(InstDesc), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess && gen0 < 0) e = hipMemsetAsync(pop->host.gen, 0xFF, sizeof(int), s);
+        // clear the key buffers (their padding columns are never written by
+        // the RNG paths, and copies must stay deterministic) and the
+        // counters; pop->host outlives the descriptor copy
+        const InstDesc& d = pop->host;
+        const size_t rows = pop->nsga2 ? (size_t)2 * d.pop : (size_t)d.pop;
+        hipError_t e = hipSuccess;
+        for (int b = 0; b < 2 && e == hipSuccess; ++b)
+            e = hipMemsetAsync(d.keys[b], 0, rows * d.ld * sizeof(float), s);
+        if (e == hipSuccess) e = hipMemsetAsync(d.gen, 0, 2 * sizeof(int), s);
+        if (e == hipSuccess && gen0 < 0) e = hipMemsetAsync(d.gen, 0xFF, sizeof(int), s);
+        if (e == hipSuccess) e = hipMemcpyAsync(pop->d_desc, &pop->host, sizeof(InstDesc), hipMemcpyHostToDevice, s);
         if (e != hipSuccess) st = hip_error(e, "population upload");
     }
     if (st != BRKGA_OK) {

@@ -91,9 +91,12 @@ namespace brkga {
 // Validates params against a problem and fills `d` (pointers excluded).
 brkga_status fill_inst_desc(const brkga_problem_s* pr, const brkga_params* q, InstDesc* d, int* nsga2);
 // Allocates the ping-pong buffers and counters described by d (from `a`,
-// or hipMalloc when null) and clears them on stream s.
-brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, DeviceAllocator* a, hipStream_t s,
-                                std::vector<void*>* owned);
+// or hipMalloc when null); create_population clears them.
+brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, DeviceAllocator* a, std::vector<void*>* owned);
+
+// Device bytes create_population takes from an allocator for (pr, q), each
+// buffer rounded up to 256 bytes.
+brkga_status population_bytes(const brkga_problem_s* pr, const brkga_params* q, size_t* bytes);
 
 // Population over `pr` with buffers from `a` (nullptr: hipMalloc).  The
 // step counter starts at 0, or at -1 when gen0 < 0 (the first fleet tick

@@ -1,0 +1,418 @@
+// server.cpp - request loop with continuous batching (brkga_server_*).
+//
+// One worker thread owns a Fleet of resident requests.  Each round it
+//   1. admits pending requests first-come first-served while each one's
+//      population (one contiguous slab of the HBM arena) fits the budget and
+//      the resident cap; admitted populations start at step -1, so the next
+//      tick builds their generation 0,
+//   2. runs `ticks_per_round` ticks of every resident request on its own
+//      stream (members at their deadline are skipped by the kernels),
+//   3. retires the requests that reached their deadline: best individual to
+//      host, slab back to the arena.
+// Arena slabs are never returned to the driver while the server runs, so no
+// round waits on a device-wide free.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "capi_internal.hpp"
+#include "fleet.hpp"
+
+using namespace brkga;
+
+namespace {
+
+constexpr size_t kSlabAlign = 256;
+
+// First-fit allocator over one device allocation; free ranges coalesce.
+class Arena {
+public:
+    brkga_status reserve(size_t bytes)
+    {
+        void* p = nullptr;
+        brkga_status st = device_alloc(&p, bytes, "server arena");
+        if (st != BRKGA_OK) return st;
+        base_ = static_cast<char*>(p);
+        size_ = bytes;
+        free_[0] = bytes;
+        return BRKGA_OK;
+    }
+    ~Arena()
+    {
+        if (base_) (void)hipFree(base_);
+    }
+    char* take(size_t bytes)
+    {
+        bytes = (bytes + kSlabAlign - 1) & ~(kSlabAlign - 1);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            const size_t off = it->first, len = it->second;
+            free_.erase(it);
+            if (len > bytes) free_[off + bytes] = len - bytes;
+            used_ += bytes;
+            peak_ = std::max(peak_, used_);
+            live_[off] = bytes;
+            return base_ + off;
+        }
+        return nullptr;
+    }
+    void give(char* p)
+    {
+        const size_t off = (size_t)(p - base_);
+        auto lv = live_.find(off);
+        if (lv == live_.end()) return;
+        size_t len = lv->second;
+        live_.erase(lv);
+        used_ -= len;
+        size_t start = off;
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && next->first == off + len) {
+            len += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                start = prev->first;
+                len += prev->second;
+                free_.erase(prev);
+            }
+        }
+        free_[start] = len;
+    }
+    size_t size() const { return size_; }
+    size_t used() const { return used_; }
+    size_t peak() const { return peak_; }
+
+private:
+    char* base_ = nullptr;
+    size_t size_ = 0, used_ = 0, peak_ = 0;
+    std::map<size_t, size_t> free_;  // offset -> length
+    std::unordered_map<size_t, size_t> live_;
+};
+
+// Carves one population's buffers out of a single arena slab.
+struct SlabAllocator : DeviceAllocator {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+    brkga_status alloc(void** p, size_t n, const char* what) override
+    {
+        const size_t r = (n + kSlabAlign - 1) & ~(kSlabAlign - 1);
+        if (used + r > size)
+            return set_error(BRKGA_ERR_OOM, std::string("server slab exhausted (") + what + ")");
+        *p = base + used;
+        used += r;
+        return BRKGA_OK;
+    }
+    void release(void*) override {}
+};
+
+struct Request {
+    uint64_t id = 0;
+    brkga_problem_s* problem = nullptr;
+    bool own_problem = false;
+    brkga_params params{};
+    int generations = 0;
+    size_t bytes = 0;
+    char* slab = nullptr;
+    SlabAllocator alloc;
+    brkga_pop_s* pop = nullptr;
+    int ticks = 0;
+    brkga_request_info info{};
+    std::vector<float> fitness;
+    std::vector<int32_t> decoded;
+};
+
+double now_us(std::chrono::steady_clock::time_point t0)
+{
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct brkga_server_s {
+    brkga_engine_s* engine = nullptr;
+    brkga_server_config cfg{};
+    Arena arena;
+    std::unique_ptr<Fleet> fleet;
+    hipStream_t stream = nullptr;
+    std::chrono::steady_clock::time_point t0;
+    std::thread worker;
+
+    std::mutex mu;
+    std::condition_variable cv_work;  // worker: new requests or stop
+    std::condition_variable cv_done;  // waiters: a request finished
+    bool stopping = false;
+    uint64_t next_id = 1;
+    std::deque<std::unique_ptr<Request>> pending;
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> finished;
+    std::unordered_map<uint64_t, int> in_flight;  // ids admitted and not finished
+    brkga_server_stats stats{};
+    std::string worker_error;
+
+    // worker-only state
+    std::vector<std::unique_ptr<Request>> resident;
+
+    void run();
+    bool admit();
+    brkga_status retire_done();
+    void finish(std::unique_ptr<Request> r, brkga_status st);
+};
+
+void brkga_server_s::finish(std::unique_ptr<Request> r, brkga_status st)
+{
+    r->info.status = st;
+    r->info.done_us = now_us(t0);
+    if (r->pop) {
+        destroy_population(r->pop);
+        r->pop = nullptr;
+    }
+    if (r->slab) {
+        arena.give(r->slab);
+        r->slab = nullptr;
+    }
+    if (r->own_problem && r->problem) {
+        (void)brkga_problem_destroy(r->problem);
+        r->problem = nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    in_flight.erase(r->id);
+    ++stats.completed;
+    stats.bytes_in_use = arena.used();
+    finished[r->id] = std::move(r);
+    cv_done.notify_all();
+}
+
+// Admits pending requests in arrival order while they fit; true when the
+// resident set changed.
+bool brkga_server_s::admit()
+{
+    bool changed = false;
+    for (;;) {
+        std::unique_ptr<Request> r;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            if (pending.empty()) break;
+            if (cfg.max_resident > 0 && (int)resident.size() >= cfg.max_resident) break;
+            Request* head = pending.front().get();
+            char* slab = arena.take(head->bytes);
+            if (!slab) break;  // first-come first-served: the head waits for memory
+            r = std::move(pending.front());
+            pending.pop_front();
+            r->slab = slab;
+            in_flight[r->id] = 1;
+            ++stats.admitted;
+            stats.bytes_in_use = arena.used();
+            stats.peak_bytes = arena.peak();
+        }
+        r->alloc.base = r->slab;
+        r->alloc.size = r->bytes;
+        r->info.admit_us = now_us(t0);
+        brkga_status st = create_population(r->problem, &r->params, &r->alloc, stream, -1, &r->pop);
+        if (st != BRKGA_OK) {
+            r->pop = nullptr;
+            finish(std::move(r), st);
+            continue;
+        }
+        r->pop->host.gen_limit = r->generations;
+        resident.push_back(std::move(r));
+        changed = true;
+    }
+    return changed;
+}
+
+brkga_status brkga_server_s::retire_done()
+{
+    std::vector<std::unique_ptr<Request>> keep;
+    bool changed = false;
+    for (auto& r : resident) {
+        // tick 1 builds generation 0, tick g + 1 completes step g
+        if (r->ticks < r->generations + 1) {
+            keep.push_back(std::move(r));
+            continue;
+        }
+        const int M = r->problem->desc.num_obj, L = r->problem->desc.chrom_len;
+        r->fitness.assign(M, 0.0f);
+        r->decoded.assign(L, 0);
+        brkga_status st = brkga_get_best(r->pop, r->fitness.data(), nullptr, r->decoded.data(), stream);
+        r->info.generations = r->generations;
+        r->info.device_bytes = r->bytes;
+        finish(std::move(r), st);
+        changed = true;
+    }
+    resident.swap(keep);
+    if (!changed) return BRKGA_OK;
+    std::vector<brkga_pop_s*> pops;
+    for (auto& r : resident) pops.push_back(r->pop);
+    return fleet->set_members(pops, stream);
+}
+
+void brkga_server_s::run()
+{
+    DeviceGuard g(engine->device);
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lock(mu);
+            cv_work.wait(lock, [&] { return stopping || !pending.empty() || !resident.empty(); });
+            if (stopping && pending.empty() && resident.empty()) break;
+            ++stats.rounds;
+        }
+        brkga_status st = BRKGA_OK;
+        if (admit()) {
+            std::vector<brkga_pop_s*> pops;
+            for (auto& r : resident) pops.push_back(r->pop);
+            st = fleet->set_members(pops, stream);
+        }
+        if (st == BRKGA_OK && !resident.empty()) {
+            for (int k = 0; k < cfg.ticks_per_round && st == BRKGA_OK; ++k) st = fleet->tick(stream);
+            if (st == BRKGA_OK) {
+                hipError_t e = hipStreamSynchronize(stream);
+                if (e != hipSuccess) st = hip_error(e, "server tick");
+            }
+            if (st == BRKGA_OK) {
+                for (auto& r : resident) r->ticks += cfg.ticks_per_round;
+                std::lock_guard<std::mutex> lock(mu);
+                stats.ticks += cfg.ticks_per_round;
+            }
+        }
+        if (st == BRKGA_OK) st = retire_done();
+        if (st != BRKGA_OK) {
+            // a device failure ends every resident request with that status
+            {
+                std::lock_guard<std::mutex> lock(mu);
+                worker_error = brkga_last_error();
+            }
+            std::vector<std::unique_ptr<Request>> rs;
+            rs.swap(resident);
+            for (auto& r : rs) finish(std::move(r), st);
+            (void)fleet->set_members({}, stream);
+        }
+    }
+}
+
+extern "C" {
+
+brkga_status brkga_server_start(brkga_engine_t engine, const brkga_server_config* config, brkga_server_t* out)
+{
+    if (!engine || !out) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    DeviceGuard g(engine->device);
+    std::unique_ptr<brkga_server_s> s(new brkga_server_s());
+    s->engine = engine;
+    if (config) s->cfg = *config;
+    if (s->cfg.hbm_budget_bytes == 0) s->cfg.hbm_budget_bytes = (uint64_t)8 << 30;
+    if (s->cfg.ticks_per_round <= 0) s->cfg.ticks_per_round = 4;
+    if (s->cfg.max_resident < 0) return set_error(BRKGA_ERR_INVALID_ARG, "max_resident must be >= 0");
+    brkga_status st = s->arena.reserve(s->cfg.hbm_budget_bytes);
+    if (st != BRKGA_OK) return st;
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_error(e, "hipStreamCreate");
+    s->fleet.reset(new Fleet(engine));
+    s->stats.budget_bytes = s->cfg.hbm_budget_bytes;
+    s->t0 = std::chrono::steady_clock::now();
+    brkga_server_s* raw = s.get();
+    s->worker = std::thread([raw] { raw->run(); });
+    *out = s.release();
+    return BRKGA_OK;
+}
+
+brkga_status brkga_server_submit(brkga_server_t server, brkga_problem_t problem, const brkga_params* params,
+                                 int32_t generations, int32_t own_problem, uint64_t* request_id)
+{
+    if (!server || !problem || !params || !request_id)
+        return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
+    if (problem->engine != server->engine) return set_error(BRKGA_ERR_INVALID_ARG, "problem of another engine");
+    if (generations < 0) return set_error(BRKGA_ERR_INVALID_ARG, "negative generation count");
+    std::unique_ptr<Request> r(new Request());
+    r->problem = problem;
+    r->own_problem = own_problem != 0;
+    r->params = *params;
+    r->generations = generations;
+    brkga_status st = population_bytes(problem, params, &r->bytes);
+    std::lock_guard<std::mutex> lock(server->mu);
+    if (server->stopping) return set_error(BRKGA_ERR_INVALID_ARG, "server is stopping");
+    if (st != BRKGA_OK) {
+        ++server->stats.rejected;
+        return st;
+    }
+    if (r->bytes > server->arena.size()) {
+        ++server->stats.rejected;
+        return set_error(BRKGA_ERR_OOM, "request needs " + std::to_string(r->bytes) +
+                                            " bytes, more than the server budget");
+    }
+    r->id = server->next_id++;
+    r->info.submit_us = now_us(server->t0);
+    *request_id = r->id;
+    ++server->stats.submitted;
+    server->pending.push_back(std::move(r));
+    server->cv_work.notify_one();
+    return BRKGA_OK;
+}
+
+brkga_status brkga_server_wait(brkga_server_t server, uint64_t request_id, int32_t timeout_ms, float* fitness,
+                               int32_t* decoded, brkga_request_info* info)
+{
+    if (!server) return set_error(BRKGA_ERR_INVALID_ARG, "null server");
+    std::unique_lock<std::mutex> lock(server->mu);
+    auto known = [&] {
+        if (server->finished.count(request_id) || server->in_flight.count(request_id)) return true;
+        for (auto& p : server->pending)
+            if (p->id == request_id) return true;
+        return false;
+    };
+    if (!known()) return set_error(BRKGA_ERR_NOT_FOUND, "unknown or already collected request");
+    auto ready = [&] { return server->finished.count(request_id) > 0; };
+    if (!ready()) {
+        if (timeout_ms == 0) return BRKGA_ERR_BUSY;
+        if (timeout_ms < 0) {
+            server->cv_done.wait(lock, ready);
+        } else if (!server->cv_done.wait_for(lock, std::chrono::milliseconds(timeout_ms), ready)) {
+            return BRKGA_ERR_BUSY;
+        }
+    }
+    std::unique_ptr<Request> r = std::move(server->finished[request_id]);
+    server->finished.erase(request_id);
+    if (info) *info = r->info;
+    if (r->info.status != BRKGA_OK)
+        return set_error((brkga_status)r->info.status, "request failed: " + server->worker_error);
+    if (fitness) std::copy(r->fitness.begin(), r->fitness.end(), fitness);
+    if (decoded) std::copy(r->decoded.begin(), r->decoded.end(), decoded);
+    return BRKGA_OK;
+}
+
+brkga_status brkga_server_stats_get(brkga_server_t server, brkga_server_stats* stats)
+{
+    if (!server || !stats) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lock(server->mu);
+    *stats = server->stats;
+    stats->pending = (int32_t)server->pending.size();
+    stats->resident = (int32_t)server->in_flight.size();
+    return BRKGA_OK;
+}
+
+brkga_status brkga_server_stop(brkga_server_t server)
+{
+    if (!server) return BRKGA_OK;
+    {
+        std::lock_guard<std::mutex> lock(server->mu);
+        server->stopping = true;
+        server->cv_work.notify_all();
+    }
+    if (server->worker.joinable()) server->worker.join();
+    {
+        DeviceGuard g(server->engine->device);
+        (void)hipStreamSynchronize(server->stream);
+        server->fleet.reset();
+        (void)hipStreamDestroy(server->stream);
+    }
+    delete server;
+    return BRKGA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+"""Serving form: requests admitted by HBM budget, batched while resident,
+retired at their generation deadline.
+
+The bar: every request's best fitness and decoded solution are identical to
+those of the same population evolved alone (brkga_evolve) for the same
+number of generations, whatever else shared the device with it.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import brkga_amd
+from cases import make_problem
+
+pytestmark = pytest.mark.gpu
+
+REQUESTS = [
+    # kind, n, P, E, M, rho, islands, interval, migrants, generations
+    ("tsp", 25, 64, 12, 6, 0.7, 1, 0, 0, 9),
+    ("tsp", 300, 128, 20, 20, 0.7, 2, 3, 2, 6),
+    ("knap_greedy", 150, 64, 12, 6, 0.7, 1, 0, 0, 11),
+    ("bitsp", 40, 64, 0, 0, 0.7, 1, 0, 0, 5),
+    ("tsp", 1200, 64, 10, 10, 0.7, 1, 0, 0, 4),
+    ("tsp_matrix", 50, 48, 9, 5, 0.65, 1, 0, 0, 7),
+    ("knap_threshold", 90, 64, 12, 6, 0.7, 1, 0, 0, 0),
+    ("tspj", 40, 64, 12, 6, 0.75, 1, 0, 0, 1),
+    ("tsp", 64, 64, 10, 10, 0.7, 1, 0, 0, 13),
+    ("bitsp", 120, 128, 0, 0, 0.7, 1, 0, 0, 3),
+    ("knap_greedy", 700, 128, 20, 12, 0.7, 1, 0, 0, 5),
+    ("tsp", 5000, 32, 5, 5, 0.7, 1, 0, 0, 2),
+]
+
+
+def solo_best(gp, prm, G):
+    pop = brkga_amd.Population(gp, prm)
+    pop.initialize()
+    pop.evolve(G)
+    fit, _, dec = pop.best()
+    pop.close()
+    return fit, dec
+
+
+def build_requests(engine):
+    out = []
+    for i, (kind, n, P, E, M, rho, I, iv, k, G) in enumerate(REQUESTS):
+        gp, _ = make_problem(engine, kind, n, inst=200 + i)
+        prm = brkga_amd.Params(P, E, M, rho, I, iv, k, 0x5E4E + 11 * i)
+        out.append((gp, prm, G))
+    return out
+
+
+def check(result, ref, ctx):
+    fit, dec, info = result
+    rfit, rdec = ref
+    assert info["status"] == 0, ctx
+    assert np.array_equal(np.asarray(fit), np.asarray(rfit)), f"best fitness differs ({ctx})"
+    assert np.array_equal(dec, rdec), f"decoded solution differs ({ctx})"
+
+
+@pytest.mark.parametrize("budget_mib,max_resident,ticks", [(512, 0, 3), (24, 3, 1), (512, 2, 5)])
+def test_server_results_match_solo(engine, budget_mib, max_resident, ticks):
+    print(f"server mix budget={budget_mib} MiB max_resident={max_resident} ticks={ticks}", flush=True)
+    reqs = build_requests(engine)
+    refs = [solo_best(gp, prm, G) for gp, prm, G in reqs]
+    srv = brkga_amd.Server(engine, hbm_budget_bytes=budget_mib << 20, max_resident=max_resident,
+                           ticks_per_round=ticks)
+    ids = []
+    for i, (gp, prm, G) in enumerate(reqs):
+        ids.append(srv.submit(gp, prm, G))
+        if i % 4 == 3:
+            time.sleep(0.002)  # later requests join while earlier ones run
+    results = [srv.wait(r, timeout_ms=60000) for r in ids]
+    st = srv.stats()
+    srv.stop()
+    for i, (res, ref) in enumerate(zip(results, refs)):
+        check(res, ref, f"request {i} {REQUESTS[i][:2]}")
+        assert res[2]["generations"] == REQUESTS[i][-1]
+        assert res[2]["submit_us"] <= res[2]["admit_us"] <= res[2]["done_us"]
+    assert st["completed"] == len(reqs) and st["submitted"] == len(reqs)
+    assert st["peak_bytes"] <= st["budget_bytes"]
+
+
+def test_server_alone_equals_in_mix(engine):
+    """One request served alone and served among others: same result."""
+    print("server alone vs mix", flush=True)
+    reqs = build_requests(engine)
+    gp, prm, G = reqs[1]
+    srv = brkga_amd.Server(engine, hbm_budget_bytes=256 << 20)
+    alone = srv.wait(srv.submit(gp, prm, G), timeout_ms=60000)
+    ids = [srv.submit(q, p, g) for q, p, g in reqs]
+    mixed = [srv.wait(r, timeout_ms=60000) for r in ids]
+    srv.stop()
+    check(mixed[1], (alone[0], alone[1]), "request 1 alone vs in the mix")
+
+
+def test_server_rejects_and_reports(engine):
+    print("server rejection / wait semantics", flush=True)
+    gp, _ = make_problem(engine, "tsp", 2000, inst=7)
+    prm = brkga_amd.Params(1024, 150, 150, 0.7, 1, 0, 0, 3)
+    srv = brkga_amd.Server(engine, hbm_budget_bytes=1 << 20)
+    with pytest.raises(brkga_amd.BrkgaError) as e:
+        srv.submit(gp, prm, 10)
+    assert e.value.status == brkga_amd.ERR_OOM
+    with pytest.raises(brkga_amd.BrkgaError) as e:
+        srv.wait(12345, timeout_ms=0)
+    assert e.value.status == brkga_amd.ERR_NOT_FOUND
+    small, _ = make_problem(engine, "tsp", 30, inst=8)
+    rid = srv.submit(small, brkga_amd.Params(32, 6, 4, 0.7, 1, 0, 0, 4), 3)
+    fit, dec, info = srv.wait(rid, timeout_ms=60000)
+    assert sorted(dec.tolist()) == list(range(30))
+    with pytest.raises(brkga_amd.BrkgaError) as e:
+        srv.wait(rid, timeout_ms=0)
+    assert e.value.status == brkga_amd.ERR_NOT_FOUND
+    st = srv.stats()
+    assert st["rejected"] == 1 and st["completed"] == 1
+    srv.stop()
