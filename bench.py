@@ -22,10 +22,23 @@ per generation), the batched launches, and the batched launches replayed
 from a captured graph (after the autotuner picked each size bucket's launch
 shape).  Every member's best fitness must be identical across the forms.
 
+Served leg ("served"): Poisson arrival streams (rates 50 ... 2000 req/s, and
+all requests at once) of mixed requests: 60% Euclidean TSP with n
+log-uniform in [20, 5000], 25% greedy knapsack with n uniform in
+{100..2000}, 15% bi-objective TSP (NSGA-II, population 512) with n uniform
+in {50..500}; TSP and knapsack populations 256 with elite/mutants 38; every
+request has a 200-generation deadline.  The same trace is served by the
+request loop (HBM-budget admission, continuous batching) and by the plain
+per-instance form (one request after the other, plain launches); both report
+solves/s and p50/p99 latency (arrival to result on the host, problem upload
+included), and every request's best fitness must agree between the two.
+
 Usage: python bench.py [--gpus 1] [--steps 1000] [--warmup 50]
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -144,6 +157,108 @@ def batched_leg(brkga_amd, workloads, eng, gens):
     return out
 
 
+SERVE_RATES = (50, 100, 200, 500, 1000, 2000, 0)  # 0: every request arrives at t = 0
+SERVE_GENS, SERVE_POP, SERVE_ELITE, SERVE_MUTANTS, SERVE_NSGA_POP = 200, 256, 38, 38, 512
+
+
+def serve_requests(brkga_amd, workloads, count):
+    reqs = []
+    for i, (kind, n) in enumerate(workloads.request_mix(count)):
+        inst, seed = 5000 + i, SEED + 0x10000 + i
+        if kind == "tsp":
+            data = workloads.tsp_coords(inst, n, 1000.0)
+        elif kind == "knapsack":
+            data = workloads.knapsack(inst, n)
+        else:
+            data = workloads.bitsp_coords(inst, n)
+        if kind == "bitsp":
+            prm = brkga_amd.Params(SERVE_NSGA_POP, 0, 0, 0.7, 1, 0, 0, seed)
+        else:
+            prm = brkga_amd.Params(SERVE_POP, SERVE_ELITE, SERVE_MUTANTS, 0.7, 1, 0, 0, seed)
+        reqs.append((kind, n, data, prm))
+    return reqs
+
+
+def serve_problem(brkga_amd, eng, kind, data):
+    if kind == "tsp":
+        return brkga_amd.Problem.tsp_coords(eng, data)
+    if kind == "knapsack":
+        w, v, cap = data
+        return brkga_amd.Problem.knapsack(eng, w, v, cap, greedy=True)
+    a, b = data
+    return brkga_amd.Problem.bitsp_coords(eng, a, b)
+
+
+def percentiles(lat):
+    a = sorted(lat)
+    pick = lambda q: a[min(len(a) - 1, int(round(q * (len(a) - 1))))]
+    return round(1000.0 * pick(0.5), 3), round(1000.0 * pick(0.99), 3)
+
+
+def served_run(brkga_amd, eng, reqs, arrivals, budget):
+    srv = brkga_amd.Server(eng, hbm_budget_bytes=budget, ticks_per_round=4)
+    ids, t_sub = [], []
+    t0 = time.perf_counter()
+    for (kind, _, data, prm), ta in zip(reqs, arrivals):
+        wait = ta - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        p = serve_problem(brkga_amd, eng, kind, data)
+        ids.append(srv.submit(p, prm, SERVE_GENS, own_problem=True))
+        t_sub.append(time.perf_counter() - t0)
+    res = [srv.wait(r) for r in ids]
+    stats = srv.stats()
+    srv.stop()
+    done = [ts + (info["done_us"] - info["submit_us"]) * 1e-6 for ts, (_, _, info) in zip(t_sub, res)]
+    lat = [d - ta for d, ta in zip(done, arrivals)]
+    span = max(done) - float(arrivals[0])
+    return [r[0] for r in res], lat, span, stats
+
+
+def plain_run(brkga_amd, eng, reqs, arrivals):
+    fits, lat, done = [], [], []
+    t0 = time.perf_counter()
+    for (kind, _, data, prm), ta in zip(reqs, arrivals):
+        wait = ta - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        p = serve_problem(brkga_amd, eng, kind, data)
+        pop = brkga_amd.Population(p, prm)
+        pop.initialize()
+        pop.evolve(SERVE_GENS)
+        fits.append(pop.best()[0])
+        pop.close()
+        p.close()
+        t = time.perf_counter() - t0
+        done.append(t)
+        lat.append(t - ta)
+    return fits, lat, max(done) - float(arrivals[0])
+
+
+def served_leg(brkga_amd, workloads, eng, count, rates, budget):
+    reqs = serve_requests(brkga_amd, workloads, count)
+    # warm-up of every kernel shape the mix uses
+    served_run(brkga_amd, eng, reqs[:16], [0.0] * 16, budget)
+    rows, all_same = [], True
+    for rate in rates:
+        arrivals = workloads.poisson_arrivals(count, rate) if rate > 0 else [0.0] * count
+        s_fit, s_lat, s_span, st = served_run(brkga_amd, eng, reqs, arrivals, budget)
+        p_fit, p_lat, p_span = plain_run(brkga_amd, eng, reqs, arrivals)
+        same = all(np.array_equal(np.asarray(a), np.asarray(b)) for a, b in zip(s_fit, p_fit))
+        all_same = all_same and same
+        s50, s99 = percentiles(s_lat)
+        p50, p99 = percentiles(p_lat)
+        rows.append({"rate_req_per_s": rate if rate > 0 else "all at t=0", "requests": count,
+                     "served": {"solves_per_s": round(count / s_span, 2), "p50_ms": s50, "p99_ms": s99,
+                                "peak_resident_bytes": st["peak_bytes"], "ticks": st["ticks"]},
+                     "plain": {"solves_per_s": round(count / p_span, 2), "p50_ms": p50, "p99_ms": p99},
+                     "speedup_solves_per_s": round(p_span / s_span, 2), "results_identical": bool(same)})
+    return {"workload": f"Poisson arrivals of {count} mixed requests per rate: 60% TSP n log-uniform [20,5000], "
+                        f"25% greedy knapsack n uniform [100,2000], 15% bi-objective TSP (NSGA-II, N=512) "
+                        f"n uniform [50,500]; P=256 elite/mutants 38 otherwise; {SERVE_GENS}-generation deadline",
+            "hbm_budget_bytes": budget, "rates": rows, "results_identical": bool(all_same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +268,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-gens", type=int, default=BATCH_GENS)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rates", type=str, default=",".join(str(r) for r in SERVE_RATES))
+    ap.add_argument("--serve-budget-gib", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the island model runs all islands on one device); "
@@ -224,6 +343,10 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = batched_leg(brkga_amd, workloads, eng, args.batch_gens)
+    if not args.no_serve:
+        rates = [float(r) for r in args.serve_rates.split(",") if r]
+        out["served"] = served_leg(brkga_amd, workloads, eng, args.serve_requests, rates,
+                                   int(args.serve_budget_gib * (1 << 30)))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))

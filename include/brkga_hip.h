@@ -112,7 +112,8 @@ brkga_status brkga_problem_create_tsp_matrix(brkga_engine_t engine, const float*
 /* 0/1 knapsack with integer weights/values, maximised.  greedy = 0: the
  * reference's threshold decoder (key > 0.5, penalty 1000 per unit over
  * capacity; configs/knapsack_config.hpp:43-75).  greedy = 1: pack items in
- * ascending key order when they fit; exact integer value. */
+ * ascending key order when they fit; exact integer value; weights and
+ * capacity must be >= 0. */
 brkga_status brkga_problem_create_knapsack(brkga_engine_t engine, const int32_t* weights, const int32_t* values,
                                            int32_t n, int64_t capacity, int32_t greedy, brkga_problem_t* out);
 

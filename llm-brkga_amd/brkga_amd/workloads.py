@@ -114,3 +114,34 @@ def log_uniform_sizes(count: int, lo: int, hi: int, tag: int = TAG_SIZES, seed: 
     u = unit(w).astype(np.float64)
     n = np.rint(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64)
     return np.clip(n, lo, hi).astype(np.int32)
+
+
+TAG_ARRIVALS = 0x109
+
+
+def request_mix(count: int, seed: int = MASTER_SEED):
+    """Mixed serving requests: (kind, n) per request, 60% Euclidean TSP with
+    n log-uniform in [20, 5000], 25% knapsack with n uniform in {100..2000},
+    15% bi-objective TSP with n uniform in {50..500}."""
+    w = words(1, TAG_MIX, count, seed)
+    pick = unit(w[:, 0]).astype(np.float64)
+    u = unit(w[:, 1]).astype(np.float64)
+    out = []
+    for i in range(count):
+        if pick[i] < 0.60:
+            n = int(np.clip(np.rint(np.exp(np.log(20) + u[i] * (np.log(5000) - np.log(20)))), 20, 5000))
+            out.append(("tsp", n))
+        elif pick[i] < 0.85:
+            out.append(("knapsack", 100 + int(w[i, 2] % np.uint32(1901))))
+        else:
+            out.append(("bitsp", 50 + int(w[i, 2] % np.uint32(451))))
+    return out
+
+
+def poisson_arrivals(count: int, rate: float, seed: int = MASTER_SEED):
+    """Arrival times (seconds from 0) of a Poisson stream of `rate` requests
+    per second: exponential gaps -ln(1 - u) / rate from one Philox stream."""
+    w = words(2, TAG_ARRIVALS, count, seed)[:, 0]
+    u = unit(w).astype(np.float64)
+    return np.cumsum(-np.log1p(-u) / float(rate))
+

@@ -486,6 +486,11 @@ brkga_status brkga_problem_create_knapsack(brkga_engine_t engine, const int32_t*
                                            int32_t n, int64_t capacity, int32_t greedy, brkga_problem_t* out)
 {
     if (!weights || !values) return set_error(BRKGA_ERR_INVALID_ARG, "null weights or values");
+    if (greedy) {
+        if (capacity < 0) return set_error(BRKGA_ERR_INVALID_ARG, "greedy knapsack needs capacity >= 0");
+        for (int32_t i = 0; i < n; ++i)
+            if (weights[i] < 0) return set_error(BRKGA_ERR_INVALID_ARG, "greedy knapsack needs weights >= 0");
+    }
     brkga_problem_s* pr;
     brkga_status st = new_problem(engine, greedy ? KIND_KNAP_GREEDY : KIND_KNAP_THRESHOLD, n, &pr);
     if (st != BRKGA_OK) return st;

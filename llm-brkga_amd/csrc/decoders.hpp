@@ -87,12 +87,41 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
         }
     } else if constexpr (KIND == KIND_KNAP_GREEDY) {
         block_argsort<BLOCK, CLEARED>(keys, n, nb, ss, perm);
-        // Wave 0 walks the key order 64 items at a time: the first open lane
-        // whose weight fits is packed, open lanes before it are rejected.
+        // With weights >= 0 the greedy fill takes every item up to the first
+        // position f where the running weight in key order exceeds the
+        // capacity (a block prefix scan finds f), rejects item f, and from
+        // there on the remaining capacity is below w[f]: wave 0 walks the
+        // rest 64 items at a time, packing the first open item that fits.
+        const int chunk = (n + BLOCK - 1) / BLOCK;
+        const int lo = tid * chunk < n ? tid * chunk : n;
+        const int hi = lo + chunk < n ? lo + chunk : n;
+        long long wsum = 0;
+        for (int q = lo; q < hi; ++q) wsum += p.w[perm[q]];
+        long long run = block_inclusive_scan_i64<BLOCK>(wsum, red) - wsum;
+        int first = n;
+        long long vsum = 0, rem_at = 0;
+        for (int q = lo; q < hi; ++q) {
+            const int item = perm[q];
+            const long long w = p.w[item];
+            if (run + w > p.cap) {
+                first = q;
+                rem_at = p.cap - run;
+                break;
+            }
+            run += w;
+            vsum += p.v[item];
+        }
+        const int f = (int)-block_max_i64<BLOCK>(-(long long)first, red);
+        long long val = block_sum_i64<BLOCK>(vsum, red);
+        // the thread that owns position f publishes the capacity left there
+        if (first == f && f < n) red[16] = rem_at;
+        __syncthreads();
+        if (decoded)
+            for (int q = tid; q < n && q <= f; q += BLOCK) decoded[perm[q]] = q < f ? 1 : 0;
         if (tid < kWave) {
             const int lane = tid;
-            long long rem = p.cap, val = 0;
-            for (int base = 0; base < n; base += kWave) {
+            long long rem = f < n ? red[16] : 0;
+            for (int base = f + 1; base < n; base += kWave) {
                 const int q = base + lane;
                 const bool valid = q < n;
                 const int item = valid ? perm[q] : 0;
@@ -103,11 +132,11 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
                 while (true) {
                     const unsigned long long fits = __ballot(open && w <= rem);
                     if (fits == 0ull) break;
-                    const int f = __builtin_ctzll(fits);
-                    rem -= __shfl(w, f, kWave);
-                    val += __shfl(v, f, kWave);
-                    taken = taken || lane == f;
-                    open = open && lane > f;
+                    const int fl = __builtin_ctzll(fits);
+                    rem -= __shfl(w, fl, kWave);
+                    val += __shfl(v, fl, kWave);
+                    taken = taken || lane == fl;
+                    open = open && lane > fl;
                 }
                 if (decoded && valid) decoded[item] = taken ? 1 : 0;
             }

@@ -8,8 +8,10 @@
 //   k_nsga2_select     one workgroup per population over parents + children:
 //                      domination counts and front peeling, crowding
 //                      distance per front, survivor order;
-//   k_nsga2_gather     one workgroup per survivor row: copy into the next
-//                      buffer; the last block advances the step counter.
+//   k_nsga2_gather     survivor rows copied into the next buffer, four rows
+//                      per workgroup.
+// The selection advances the step counter (one workgroup per population, no
+// atomics) and leaves d.done = 1 when the gather has rows to copy.
 // Buffers: keys[b] holds 2N rows (parents [0,N), children [N,2N)).
 #include "launch.hpp"
 #include "decoders.hpp"
@@ -18,16 +20,16 @@ namespace brkga {
 
 namespace {
 
-__device__ __forceinline__ int find_inst(const InstDesc* __restrict__ d, int num, int b, int per_block_pop)
+// Instance owning row b: instances are laid out with slot_begin = prefix
+// of their populations.
+__device__ __forceinline__ int find_inst(const InstDesc* __restrict__ d, int num, int b)
 {
-    // instances are laid out with slot_begin = prefix of their populations
     int lo = 0, hi = num - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (d[mid].slot_begin <= b) lo = mid;
         else hi = mid - 1;
     }
-    (void)per_block_pop;
     return lo;
 }
 
@@ -73,7 +75,7 @@ __global__ __launch_bounds__(BLOCK) void k_nsga2_offspring(const InstDesc* __res
 {
     extern __shared__ __align__(16) char lds[];
     const int b = blockIdx.x;
-    const InstDesc& d = descs[find_inst(descs, num_inst, b, 0)];
+    const InstDesc& d = descs[find_inst(descs, num_inst, b)];
     const int c = b - d.slot_begin;  // child (or, at init, parent) index
     const int N = d.pop;
     if (c >= N) return;
@@ -170,8 +172,12 @@ __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* _
     extern __shared__ __align__(16) char lds[];
     const InstDesc& d = descs[blockIdx.x];
     const int N = d.pop;
+    const int forced = init;  // generation 0 built by the host path: the counter stays
     const int t = init ? -1 : *d.gen;
-    if (t >= d.gen_limit) return;
+    if (t >= d.gen_limit) {
+        if (threadIdx.x == 0) *d.done = 0;  // nothing for the gather
+        return;
+    }
     init = t < 0;
     const int cur = init ? 0 : (t & 1);
     const int C = init ? N : 2 * N;
@@ -285,6 +291,11 @@ __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* _
             d.rank[0][i] = rk[i];
             d.crowd[0][i] = cr[i];
         }
+        if (!forced && threadIdx.x == 0) {
+            // counter -1: generation 0 is complete
+            *d.done = 0;
+            *d.gen = 0;
+        }
         return;
     }
     // survivor order (population.hpp:491-517)
@@ -339,37 +350,34 @@ __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* _
             d.crowd[nxt][cum + q] = cr[e];
         }
     }
+    // every thread read *d.gen before the first barrier above
+    if (threadIdx.x == 0) {
+        *d.done = 1;
+        *d.gen = t + 1;
+    }
 }
 
-__global__ __launch_bounds__(256) void k_nsga2_gather(const InstDesc* __restrict__ descs, int num_inst)
+constexpr int kGatherRows = 4;  // rows per workgroup, one wave each
+
+// Copies survivor i (rank order from the selection) of buffer t-1 into row
+// i of buffer t, t being the counter the selection just advanced.
+__global__ __launch_bounds__(kGatherRows * kWave) void k_nsga2_gather(const InstDesc* __restrict__ descs,
+                                                                      int num_inst, int total_rows)
 {
-    const int b = blockIdx.x;
-    const InstDesc& d = descs[find_inst(descs, num_inst, b, 0)];
+    const int b = blockIdx.x * kGatherRows + threadIdx.x / kWave;
+    if (b >= total_rows) return;
+    const InstDesc& d = descs[find_inst(descs, num_inst, b)];
     const int i = b - d.slot_begin;
-    const int N = d.pop;
-    if (i >= N) return;
+    if (i >= d.pop || *d.done == 0) return;
     const int t = *d.gen;
-    if (t >= d.gen_limit) return;
-    if (t >= 0) {
-        // at t = -1 the first parents are already in place; only the
-        // counter moves to 0
-        const int cur = t & 1, nxt = cur ^ 1;
-        const int src = d.perm[cur][i];
-        const int nq = (d.L + 3) >> 2;
-        const float4* in = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)src * d.ld);
-        float4* out = reinterpret_cast<float4*>(d.keys[nxt] + (size_t)i * d.ld);
-        for (int q = threadIdx.x; q < nq; q += 256) out[q] = in[q];
-        if (threadIdx.x < 2) d.fit[nxt][2 * i + threadIdx.x] = d.fit[cur][2 * src + threadIdx.x];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const int old = atomicAdd(d.done, 1);
-        if (old == N - 1) {
-            *d.done = 0;
-            *d.gen = t + 1;
-        }
-    }
+    const int cur = (t - 1) & 1, nxt = t & 1;
+    const int src = d.perm[cur][i];
+    const int nq = (d.L + 3) >> 2;
+    const int lane = threadIdx.x & (kWave - 1);
+    const float4* in = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)src * d.ld);
+    float4* out = reinterpret_cast<float4*>(d.keys[nxt] + (size_t)i * d.ld);
+    for (int q = lane; q < nq; q += kWave) out[q] = in[q];
+    if (lane < 2) d.fit[nxt][2 * i + lane] = d.fit[cur][2 * src + lane];
 }
 
 // ------------------------------------------------------------- launchers --
@@ -426,7 +434,8 @@ hipError_t launch_nsga2_select(const InstDesc* d_descs, int num_inst, int init, 
 hipError_t launch_nsga2_gather(const InstDesc* d_descs, int num_inst, int total_rows, hipStream_t s)
 {
     if (total_rows <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_nsga2_gather, dim3(total_rows), dim3(256), 0, s, d_descs, num_inst);
+    hipLaunchKernelGGL(k_nsga2_gather, dim3((total_rows + kGatherRows - 1) / kGatherRows), dim3(kGatherRows * kWave),
+                       0, s, d_descs, num_inst, total_rows);
     return hipGetLastError();
 }
 

@@ -10,7 +10,11 @@ Runs, in order:
      island, rho 0.7, migration of the top 2 every 50 generations;
   3. with --batch G, G batched generations (one launch per size bucket per
      generation, no graph) of the mixed-size workload: 512 Euclidean TSP
-     instances, n log-uniform in [50, 2000], P=256, elite/mutants 38.
+     instances, n log-uniform in [50, 2000], P=256, elite/mutants 38;
+  4. with --serve R, R mixed serving requests (60% TSP n log-uniform in
+     [20, 5000], 25% greedy knapsack n in [100, 2000], 15% bi-objective TSP
+     with NSGA-II population 512; 200-generation deadline) submitted at once
+     to the request loop.
 
 Example:
   rocprofv3 --kernel-trace --stats --output-format csv -d out -o run -- \
@@ -30,6 +34,7 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--n", type=int, default=1000)
     ap.add_argument("--batch", type=int, default=0)
+    ap.add_argument("--serve", type=int, default=0)
     args = ap.parse_args()
 
     import torch
@@ -61,6 +66,26 @@ def main():
         batch.initialize("launch")
         batch.evolve(args.batch, "launch")
         print(f"batched generations: {batch.member(0).generation} of 512 TSP instances", flush=True)
+    if args.serve > 0:
+        srv = brkga_amd.Server(eng, hbm_budget_bytes=8 << 30, ticks_per_round=4)
+        ids = []
+        for i, (kind, n) in enumerate(workloads.request_mix(args.serve)):
+            inst, seed = 5000 + i, 0xB5EED + 0x10000 + i
+            if kind == "tsp":
+                p = brkga_amd.Problem.tsp_coords(eng, workloads.tsp_coords(inst, n, 1000.0))
+            elif kind == "knapsack":
+                w, v, cap = workloads.knapsack(inst, n)
+                p = brkga_amd.Problem.knapsack(eng, w, v, cap, greedy=True)
+            else:
+                a, b = workloads.bitsp_coords(inst, n)
+                p = brkga_amd.Problem.bitsp_coords(eng, a, b)
+            prm = (brkga_amd.Params(512, 0, 0, 0.7, 1, 0, 0, seed) if kind == "bitsp"
+                   else brkga_amd.Params(256, 38, 38, 0.7, 1, 0, 0, seed))
+            ids.append(srv.submit(p, prm, 200, own_problem=True))
+        for r in ids:
+            srv.wait(r)
+        print(f"served requests: {len(ids)}, ticks {srv.stats()['ticks']}", flush=True)
+        srv.stop()
 
 
 if __name__ == "__main__":

@@ -59,6 +59,45 @@ def test_evaluate_rejects_wrong_chrom_len(engine):
     assert st == brkga_amd.ERR_INVALID_ARG
 
 
+@pytest.mark.parametrize("case", ["cap0", "zero_weights", "all_fit", "exact_fit", "one_heavy"])
+def test_greedy_knapsack_edges(engine, case):
+    """Greedy fill boundaries: nothing fits, zero weights, everything fits,
+    capacity equal to the total weight, one item heavier than the capacity."""
+    print(f"greedy knapsack edge {case}", flush=True)
+    g = np.random.default_rng(len(case))
+    n = 300
+    w = g.integers(1, 101, n).astype(np.int32)
+    v = (w + g.integers(0, 11, n)).astype(np.int32)
+    cap = int(w.sum()) // 2
+    if case == "cap0":
+        cap = 0
+    elif case == "zero_weights":
+        w[::3] = 0
+    elif case == "all_fit":
+        cap = int(w.sum()) * 2
+    elif case == "exact_fit":
+        cap = int(w.sum())
+    elif case == "one_heavy":
+        w[7] = 10 ** 6
+        cap = int(w.sum()) - 10 ** 6 + 5
+    gp = brkga_amd.Problem.knapsack(engine, w, v, cap, True)
+    op = pyoracle.Problem.knapsack(w, v, cap, True)
+    for ties in (False, True):
+        keys = random_keys(77 + ties, 40, n, ties=ties)
+        gf, gd = gpu_evaluate(gp, keys)
+        of, od = op.evaluate(keys)
+        assert np.array_equal(gd, od) and np.array_equal(gf, of), f"greedy knapsack {case} ties={ties}"
+
+
+def test_greedy_knapsack_rejects_negative(engine):
+    w = np.array([3, -1, 4], np.int32)
+    v = np.array([1, 1, 1], np.int32)
+    for ww, cap in ((w, 5), (np.abs(w), -1)):
+        with pytest.raises(brkga_amd.BrkgaError) as e:
+            brkga_amd.Problem.knapsack(engine, ww, v, cap, True)
+        assert e.value.status == brkga_amd.ERR_INVALID_ARG
+
+
 def compare_brkga(pop, ost, ctx):
     r = pop.read()
     assert np.array_equal(r["keys"], ost.keys), f"keys differ {ctx}"
