@@ -38,6 +38,11 @@ Fleet::~Fleet()
     DeviceGuard g(eng_->device);
     drop_graphs();
     if (capture_) (void)hipStreamDestroy(capture_);
+    for (int k = 1; k < kLanes; ++k) {
+        if (lanes_[k]) (void)hipStreamDestroy(lanes_[k]);
+        if (join_[k]) (void)hipEventDestroy(join_[k]);
+    }
+    if (fork_) (void)hipEventDestroy(fork_);
     if (d_descs_) (void)hipFree(d_descs_);
     if (d_items_) (void)hipFree(d_items_);
     if (d_queues_) (void)hipFree(d_queues_);
@@ -119,6 +124,7 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
             }
             FleetBucket& b = buckets_.back();
             ++b.count;
+            b.work += (double)d.pop * n;
             b.n_min = std::min(b.n_min, n);
             b.n_max = std::max(b.n_max, n);
             b.max_Pi = std::max(b.max_Pi, d.Pi);
@@ -146,6 +152,7 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
         if ((size_t)b.lds > eng_->lds_limit)
             return set_error(BRKGA_ERR_INVALID_ARG, "fleet member too large for the LDS kernels");
     }
+    assign_lanes();
     drop_graphs();
     brkga_status st = reserve(reinterpret_cast<void**>(&d_descs_), &descs_cap_, sizeof(InstDesc) * descs_.size(),
                               "fleet descriptors");
@@ -186,6 +193,36 @@ brkga_status Fleet::build_items(hipStream_t s)
     return BRKGA_OK;
 }
 
+// Largest-work-first onto the least-loaded lane; with NSGA-II members the
+// last lane is theirs.
+void Fleet::assign_lanes()
+{
+    const int brkga_lanes = groups_.empty() ? kLanes : kLanes - 1;
+    nsga_lane_ = groups_.empty() ? 0 : (buckets_.empty() ? 0 : kLanes - 1);
+    std::vector<int> idx(buckets_.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return buckets_[a].work > buckets_[b].work; });
+    double load[kLanes] = {};
+    for (int i : idx) {
+        int best = 0;
+        for (int k = 1; k < brkga_lanes; ++k)
+            if (load[k] < load[best]) best = k;
+        buckets_[i].lane = best;
+        load[best] += buckets_[i].work;
+    }
+}
+
+brkga_status Fleet::ensure_lanes()
+{
+    if (fork_) return BRKGA_OK;
+    for (int k = 1; k < kLanes; ++k) {
+        BRKGA_HIP_TRY(hipStreamCreateWithFlags(&lanes_[k], hipStreamNonBlocking));
+        BRKGA_HIP_TRY(hipEventCreateWithFlags(&join_[k], hipEventDisableTiming));
+    }
+    BRKGA_HIP_TRY(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+    return BRKGA_OK;
+}
+
 brkga_status Fleet::set_counters(int gen, hipStream_t s)
 {
     DeviceGuard g(eng_->device);
@@ -198,20 +235,44 @@ brkga_status Fleet::set_counters(int gen, hipStream_t s)
 brkga_status Fleet::tick(hipStream_t s)
 {
     DeviceGuard g(eng_->device);
+    brkga_status st = ensure_lanes();
+    if (st != BRKGA_OK) return st;
+    bool used[kLanes] = {};
+    for (const auto& b : buckets_) used[b.lane] = true;
+    if (!groups_.empty()) used[nsga_lane_] = true;
+    auto lane = [&](int k) { return k == 0 ? s : lanes_[k]; };
     hipError_t e = hipSuccess;
+    bool forked = false;
+    for (int k = 1; k < kLanes && e == hipSuccess; ++k)
+        if (used[k]) {
+            if (!forked) e = hipEventRecord(fork_, s);
+            forked = true;
+            if (e == hipSuccess) e = hipStreamWaitEvent(lanes_[k], fork_, 0);
+        }
     for (size_t i = 0; i < buckets_.size() && e == hipSuccess; ++i) {
         const FleetBucket& b = buckets_[i];
         e = launch_step_queue(b.kind, b.block, d_descs_, d_items_ + b.item_begin, b.item_count,
-                              d_queues_ + kQueueStride * i, b.grid, b.lds, s);
+                              d_queues_ + kQueueStride * i, b.grid, b.lds, lane(b.lane));
     }
-    if (e == hipSuccess && n_brkga_ > 0)
-        e = launch_partition_many(d_descs_, n_brkga_, total_islands_, max_Pi_, -1, 1, s);
     for (size_t i = 0; i < groups_.size() && e == hipSuccess; ++i) {
         const FleetGroup& gr = groups_[i];
         const InstDesc* d = d_descs_ + gr.first;
-        e = launch_nsga2_offspring(gr.kind, gr.block, d, gr.count, gr.rows, gr.lds, 0, s);
-        if (e == hipSuccess) e = launch_nsga2_select(d, gr.count, 0, gr.max_pop, s);
-        if (e == hipSuccess) e = launch_nsga2_gather(d, gr.count, gr.rows, s);
+        hipStream_t ls = lane(nsga_lane_);
+        e = launch_nsga2_offspring(gr.kind, gr.block, d, gr.count, gr.rows, gr.lds, 0, ls);
+        if (e == hipSuccess) e = launch_nsga2_select(d, gr.count, 0, gr.max_pop, ls);
+        if (e == hipSuccess) e = launch_nsga2_gather(d, gr.count, gr.rows, ls);
+    }
+    // join the BRKGA lanes before the partition, the NSGA-II lane at the end
+    for (int k = 1; k < kLanes && e == hipSuccess; ++k)
+        if (used[k] && !(k == nsga_lane_ && !groups_.empty())) {
+            e = hipEventRecord(join_[k], lanes_[k]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, join_[k], 0);
+        }
+    if (e == hipSuccess && n_brkga_ > 0)
+        e = launch_partition_many(d_descs_, n_brkga_, total_islands_, max_Pi_, -1, 1, s);
+    if (e == hipSuccess && !groups_.empty() && nsga_lane_ != 0) {
+        e = hipEventRecord(join_[nsga_lane_], lanes_[nsga_lane_]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s, join_[nsga_lane_], 0);
     }
     if (e != hipSuccess) return hip_error(e, "fleet tick");
     return BRKGA_OK;
@@ -224,6 +285,8 @@ brkga_status Fleet::tick_graph(int ticks, hipStream_t s)
     auto it = graphs_.find(ticks);
     if (it == graphs_.end()) {
         if (!capture_) BRKGA_HIP_TRY(hipStreamCreateWithFlags(&capture_, hipStreamNonBlocking));
+        brkga_status ls = ensure_lanes();  // no stream / event creation while capturing
+        if (ls != BRKGA_OK) return ls;
         hipGraph_t graph = nullptr;
         BRKGA_HIP_TRY(hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal));
         brkga_status st = BRKGA_OK;

@@ -29,11 +29,17 @@ struct FleetBucket {
     int lds;           // largest step LDS footprint among the members
     int block, chunk;  // launch shape; results do not depend on it
     int item_begin, item_count, grid;
+    double work;       // sum of population x n over the members
+    int lane;          // stream lane of the bucket's launch (0 = caller's stream)
 };
 
 struct FleetGroup {  // NSGA-II members of one problem kind
     int kind, first, count, rows, block, lds, max_pop;
 };
+
+// Streams a tick spreads its independent launches over (the caller's stream
+// and kLanes - 1 forked ones).
+constexpr int kLanes = 4;
 
 class Fleet {
 public:
@@ -65,6 +71,8 @@ public:
 private:
     brkga_status build_items(hipStream_t s);
     brkga_status reserve(void** p, size_t* cap, size_t bytes, const char* what);
+    brkga_status ensure_lanes();
+    void assign_lanes();
     void drop_graphs();
 
     brkga_engine_s* eng_;
@@ -84,6 +92,10 @@ private:
     size_t queues_cap_ = 0;
     hipStream_t capture_ = nullptr;
     std::map<int, hipGraphExec_t> graphs_;
+    hipStream_t lanes_[kLanes] = {};  // [0] unused: lane 0 is the caller's stream
+    hipEvent_t fork_ = nullptr;
+    hipEvent_t join_[kLanes] = {};
+    int nsga_lane_ = 0;
 };
 
 // Power-of-two size class of n used to bucket instances (at least 64).
