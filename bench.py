@@ -60,11 +60,16 @@ def algorithmic_bytes_per_generation(L, P, M):
 
 
 def traffic_from_profiles():
-    path = os.path.join(ROOT, "profiles", "traffic_k_brkga_step_tsp_n1000_p1024.json")
+    """HBM bytes per k_step_one launch (FETCH_SIZE + WRITE_SIZE, calibrated
+    against a copy kernel) from the committed profile summary."""
+    path = os.path.join(ROOT, "profiles", "k_step_one_tsp_n1000_p1024.md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
 
 
 def cpu_baseline(budget_s):
@@ -307,8 +312,7 @@ def main():
     sync()
     t_plain = time.perf_counter() - t0
 
-    prof_gens = min(K, 200)
-    step_us, close_us = pop.profile(prof_gens)
+    step_us, close_us = pop.profile(200)
     best, _, tour = pop.best()
     L, P, M = N_CITIES, POP, MUTANTS
     B = algorithmic_bytes_per_generation(L, P, M)
@@ -333,7 +337,7 @@ def main():
                    "launch": "hipGraph replay of the K-generation loop"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                     "kernel": "k_brkga_step (generation + decode + fitness)",
+                     "kernel": "brkga::k_step_one<0, 256> (crossover / mutants / elite copy + decode + fitness)",
                      "kernel_us": round(step_us, 3), "partition_us": round(close_us, 3),
                      "bytes_per_generation": B,
                      "generation_frac": round(B * gens_per_s / (HBM_PEAK_GBS * 1e9), 4)},

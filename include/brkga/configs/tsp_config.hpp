@@ -1,0 +1,141 @@
+// brkga/configs/tsp_config.hpp - TSPConfig<T> of the C++ facade.
+//
+// Constructor, decoder / fitness callbacks, size presets and accessors of
+// the reference's TSPConfig (configs/tsp_config.hpp:25-300).  The device
+// problem is either the N x N matrix the reference takes (engine kind
+// BRKGA_TSP_MATRIX) or, through from_coordinates(), the Euclidean
+// coordinates themselves (BRKGA_TSP_COORDS: edges computed on the fly with
+// the reference's formula, configs/tsp_config.hpp:366-368, no matrix).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../core/config.hpp"
+
+template <typename T>
+class TSPConfig : public BRKGAConfig<T> {
+public:
+    // configs/tsp_config.hpp:38-61: visit all N cities of the matrix
+    TSPConfig(const std::vector<std::vector<T>>& distances, const std::string& name = "TSP")
+        : BRKGAConfig<T>(std::vector<int>{static_cast<int>(distances.size())}), distance_matrix_(distances),
+          num_cities_(static_cast<int>(distances.size())), instance_name_(name)
+    {
+        setup();
+    }
+
+    // Euclidean instance from (x, y) coordinates; no matrix is formed.
+    static std::unique_ptr<TSPConfig<T>> from_coordinates(const std::vector<std::pair<T, T>>& xy,
+                                                          const std::string& name = "TSP")
+    {
+        std::unique_ptr<TSPConfig<T>> c(new TSPConfig<T>(static_cast<int>(xy.size()), name));
+        c->coords_ = xy;
+        return c;
+    }
+
+    // configs/tsp_config.hpp:180-193, evaluated by the engine's decoder
+    T calculate_tsp_fitness(const Individual<T>& individual)
+    {
+        std::vector<float> f;
+        this->host_evaluate(individual, &f, nullptr);
+        return static_cast<T>(f[0]);
+    }
+
+    // configs/tsp_config.hpp:195-204: the tour as one component
+    std::vector<std::vector<T>> decode_to_solution(const Individual<T>& individual)
+    {
+        const std::vector<int> tour = decode_tour_internal(individual);
+        std::vector<std::vector<T>> out(1);
+        for (int c : tour) out[0].push_back(static_cast<T>(c));
+        return out;
+    }
+
+    // configs/tsp_config.hpp:206-225: stable argsort of the keys
+    std::vector<int> decode_tour_internal(const Individual<T>& individual)
+    {
+        std::vector<int32_t> d;
+        this->host_evaluate(individual, nullptr, &d);
+        return std::vector<int>(d.begin(), d.end());
+    }
+
+    // configs/tsp_config.hpp:267-287
+    static void configure_for_size(TSPConfig<T>* config, int num_cities)
+    {
+        if (num_cities <= 20) {
+            config->population_size = 200;
+            config->elite_size = 40;
+            config->mutant_size = 20;
+            config->max_generations = 100;
+        } else if (num_cities <= 50) {
+            config->population_size = 1000;
+            config->elite_size = 800;
+            config->mutant_size = 400;
+            config->max_generations = 2000;
+        } else {
+            config->population_size = 8000;
+            config->elite_size = 1600;
+            config->mutant_size = 800;
+            config->max_generations = 5000;
+        }
+        config->elite_prob = 0.7;
+        config->update_cuda_grid_size();
+    }
+
+    int get_num_cities() const { return num_cities_; }
+    const std::string& get_instance_name() const { return instance_name_; }
+    bool is_gpu_available() const { return true; }
+
+    void print_instance_info() const
+    {
+        std::cout << "=== TSP Instance Info ===\nName: " << instance_name_ << "\nCities: " << num_cities_
+                  << "\nGPU Support: Yes\n=========================" << std::endl;
+    }
+
+    int native_kind() const override { return coords_.empty() ? BRKGA_TSP_MATRIX : BRKGA_TSP_COORDS; }
+
+protected:
+    brkga_problem_t create_native_problem(brkga_engine_t engine) override
+    {
+        brkga_problem_t p = nullptr;
+        if (!coords_.empty()) {
+            std::vector<float> xy;
+            for (const auto& c : coords_) {
+                xy.push_back(static_cast<float>(c.first));
+                xy.push_back(static_cast<float>(c.second));
+            }
+            brkga_facade::check(brkga_problem_create_tsp_coords(engine, xy.data(), num_cities_, &p),
+                                "brkga_problem_create_tsp_coords");
+        } else {
+            std::vector<float> m;
+            m.reserve((size_t)num_cities_ * num_cities_);
+            for (const auto& row : distance_matrix_)
+                for (T v : row) m.push_back(static_cast<float>(v));
+            brkga_facade::check(brkga_problem_create_tsp_matrix(engine, m.data(), num_cities_, &p),
+                                "brkga_problem_create_tsp_matrix");
+        }
+        return p;
+    }
+
+private:
+    TSPConfig(int n, const std::string& name)
+        : BRKGAConfig<T>(std::vector<int>{n}), num_cities_(n), instance_name_(name)
+    {
+        setup();
+    }
+
+    void setup()
+    {
+        this->fitness_function = [this](const Individual<T>& ind) { return calculate_tsp_fitness(ind); };
+        this->decoder = [this](const Individual<T>& ind) { return decode_to_solution(ind); };
+        this->comparator = [](T a, T b) { return a < b; };
+        this->threads_per_block = 256;
+        this->update_cuda_grid_size();
+    }
+
+    std::vector<std::vector<T>> distance_matrix_;
+    std::vector<std::pair<T, T>> coords_;
+    int num_cities_;
+    std::string instance_name_;
+};

@@ -1,0 +1,121 @@
+// brkga/configs/tspj_config.hpp - TSPJConfig<T> of the C++ facade.
+//
+// Constructor, callbacks, size presets and accessors of the reference's
+// TSPJConfig (configs/tspj_config.hpp:27-870): travel[dim][dim] and
+// jobs[dim][dim] with row / column 0 the depot, chromosome = n city keys |
+// n job keys, makespan minimised (configs/tspj_config.hpp:555-642).  The
+// TSPJ local search (configs/tspj_config.hpp:77-135) is not part of the
+// engine: has_local_search() is false.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../core/config.hpp"
+
+template <typename T>
+class TSPJConfig : public BRKGAConfig<T> {
+public:
+    TSPJConfig(const std::vector<std::vector<T>>& travel, const std::vector<std::vector<T>>& jobs,
+               const std::string& name = "TSPJ")
+        : BRKGAConfig<T>(std::vector<int>{static_cast<int>(travel.size() - 1), static_cast<int>(travel.size() - 1)}),
+          travel_(travel), jobs_(jobs), num_cities_(static_cast<int>(travel.size() - 1)),
+          num_jobs_(static_cast<int>(travel.size() - 1)), instance_name_(name)
+    {
+        this->fitness_function = [this](const Individual<T>& ind) { return calculate_tspj_fitness(ind); };
+        this->decoder = [this](const Individual<T>& ind) { return decode_to_solution(ind); };
+        this->comparator = [](T a, T b) { return a < b; };
+        this->threads_per_block = 256;
+        this->update_cuda_grid_size();
+    }
+
+    // configs/tspj_config.hpp:555-585
+    T calculate_tspj_fitness(const Individual<T>& individual)
+    {
+        std::vector<float> f;
+        this->host_evaluate(individual, &f, nullptr);
+        return static_cast<T>(f[0]);
+    }
+
+    // configs/tspj_config.hpp:587-604: [1-based cities], [1-based jobs]
+    std::vector<std::vector<T>> decode_to_solution(const Individual<T>& individual)
+    {
+        const auto s = decode_solution_internal(individual);
+        std::vector<std::vector<T>> out(2);
+        for (int c : s.first) out[0].push_back(static_cast<T>(c));
+        for (int j : s.second) out[1].push_back(static_cast<T>(j));
+        return out;
+    }
+
+    std::pair<std::vector<int>, std::vector<int>> decode_solution_internal(const Individual<T>& individual)
+    {
+        std::vector<int32_t> d;
+        this->host_evaluate(individual, nullptr, &d);
+        return {std::vector<int>(d.begin(), d.begin() + num_cities_),
+                std::vector<int>(d.begin() + num_cities_, d.end())};
+    }
+
+    // configs/tspj_config.hpp:805-830
+    static void configure_for_size(TSPJConfig<T>* config, int num_cities)
+    {
+        if (num_cities <= 10) {
+            config->population_size = 300;
+            config->elite_size = 60;
+            config->mutant_size = 30;
+            config->max_generations = 500;
+        } else if (num_cities <= 20) {
+            config->population_size = 600;
+            config->elite_size = 120;
+            config->mutant_size = 60;
+            config->max_generations = 1000;
+        } else if (num_cities <= 50) {
+            config->population_size = 1000;
+            config->elite_size = 200;
+            config->mutant_size = 100;
+            config->max_generations = 2000;
+        } else {
+            config->population_size = 2000;
+            config->elite_size = 350;
+            config->mutant_size = 200;
+            config->max_generations = 30000;
+        }
+        config->elite_prob = 0.75;
+        config->update_cuda_grid_size();
+    }
+
+    int get_num_cities() const { return num_cities_; }
+    int get_num_jobs() const { return num_jobs_; }
+    const std::string& get_instance_name() const { return instance_name_; }
+
+    void print_instance_info() const
+    {
+        std::cout << "=== TSPJ Instance Info ===\nName: " << instance_name_ << "\nCities: " << num_cities_
+                  << "\nJobs: " << num_jobs_ << "\n==========================" << std::endl;
+    }
+
+    int native_kind() const override { return BRKGA_TSPJ; }
+
+protected:
+    brkga_problem_t create_native_problem(brkga_engine_t engine) override
+    {
+        const int dim = num_cities_ + 1;
+        std::vector<float> t, j;
+        for (const auto& row : travel_)
+            for (T v : row) t.push_back(static_cast<float>(v));
+        for (const auto& row : jobs_)
+            for (T v : row) j.push_back(static_cast<float>(v));
+        if ((int)t.size() != dim * dim || (int)j.size() != dim * dim)
+            throw std::invalid_argument("TSPJ matrices must both be (n + 1) x (n + 1)");
+        brkga_problem_t p = nullptr;
+        brkga_facade::check(brkga_problem_create_tspj(engine, t.data(), j.data(), dim, &p),
+                            "brkga_problem_create_tspj");
+        return p;
+    }
+
+private:
+    std::vector<std::vector<T>> travel_, jobs_;
+    int num_cities_, num_jobs_;
+    std::string instance_name_;
+};

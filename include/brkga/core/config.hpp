@@ -201,6 +201,23 @@ public:
                   << "\n  Max generations: " << max_generations << std::endl;
     }
 
+    // Decode + fitness of one host individual on the device (backs the
+    // fitness_function / decoder callbacks of the native configs; the
+    // reference runs them on the host).  fitness gets num_objectives values,
+    // decoded the decoder's integer output (chromosome length).
+    void host_evaluate(const Individual<T>& ind, std::vector<float>* fitness, std::vector<int32_t>* decoded)
+    {
+        brkga_problem_t p = native_problem(device_);
+        if (!p) throw std::runtime_error("config has no engine decoder");
+        const std::vector<T> row = ind.flatten();
+        std::vector<float> keys(row.begin(), row.end());
+        std::vector<float> fit(num_objectives);
+        std::vector<int32_t> dec(keys.size());
+        brkga_facade::check(brkga_evaluate_host(p, keys.data(), 1, fit.data(), dec.data()), "brkga_evaluate_host");
+        if (fitness) *fitness = fit;
+        if (decoded) *decoded = dec;
+    }
+
 protected:
     // Native configs upload their data here (called once per device).
     virtual brkga_problem_t create_native_problem(brkga_engine_t) { return nullptr; }

@@ -206,13 +206,17 @@ brkga_status brkga_get_best(brkga_pop_t pop, float* fitness, float* keys, int32_
 brkga_status brkga_population_device_buffers(brkga_pop_t pop, float** d_keys, float** d_fitness,
                                              int32_t** d_order, int32_t* ld);
 
-/* Runs `generations` plain generation steps with HIP events recorded on
- * `stream` around every kernel launch and returns the mean duration of the
- * generation kernel (BRKGA step / NSGA-II offspring) and of the closing
- * kernels (partition / NSGA-II select + gather), in microseconds.
- * Replaces the reference's PerformanceStats timers (core/solver.hpp:143-157,
- * 406-438), which time synchronous launches with std::chrono. */
-brkga_status brkga_population_profile(brkga_pop_t pop, int32_t generations, double* step_us, double* close_us,
+/* Mean device time of the generation kernel (BRKGA step / NSGA-II
+ * offspring) and of the closing kernels (partition / NSGA-II selection +
+ * gather), in microseconds, from HIP events on `stream`.  BRKGA: `reps`
+ * back-to-back step launches from the current generation (each rewrites
+ * the same next-generation rows, so the population does not advance), then
+ * `reps` partition launches of the current generation.  NSGA-II: `reps`
+ * real generations with an event pair around each kernel group (the
+ * population advances).  Replaces the reference's PerformanceStats timers
+ * (core/solver.hpp:143-157, 406-438), which time synchronous launches with
+ * std::chrono. */
+brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* step_us, double* close_us,
                                       void* stream);
 
 /* ------------------------------------------------------------ batches --- */

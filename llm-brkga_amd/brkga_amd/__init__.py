@@ -360,11 +360,12 @@ class Population:
         _check(lib().brkga_get_best(self.handle, _fptr(fit), _fptr(keys), _iptr(dec), stream or None))
         return (fit if M > 1 else float(fit[0])), keys, dec
 
-    def profile(self, generations: int, stream: int = 0):
-        """Mean (generation kernel, closing kernels) durations in microseconds,
-        from HIP events around each launch of `generations` plain steps."""
+    def profile(self, reps: int, stream: int = 0):
+        """Mean (generation kernel, closing kernels) device time in
+        microseconds over `reps` back-to-back launches (BRKGA: the population
+        does not advance; NSGA-II: it advances `reps` generations)."""
         a, b = ctypes.c_double(), ctypes.c_double()
-        _check(lib().brkga_population_profile(self.handle, generations, ctypes.byref(a), ctypes.byref(b),
+        _check(lib().brkga_population_profile(self.handle, reps, ctypes.byref(a), ctypes.byref(b),
                                               stream or None))
         return a.value, b.value
 

@@ -834,59 +834,59 @@ brkga_status brkga_population_device_buffers(brkga_pop_t pop, float** d_keys, fl
     return BRKGA_OK;
 }
 
-brkga_status brkga_population_profile(brkga_pop_t pop, int32_t generations, double* step_us, double* close_us,
+brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* step_us, double* close_us,
                                       void* stream)
 {
-    if (!pop || generations <= 0) return set_error(BRKGA_ERR_INVALID_ARG, "null population or no generations");
+    if (!pop || reps <= 0) return set_error(BRKGA_ERR_INVALID_ARG, "null population or no repetitions");
     DeviceGuard g(pop->problem->engine->device);
     hipStream_t s = as_stream(stream);
     const InstDesc& d = pop->host;
-    constexpr int kChunk = 64;
-    hipEvent_t ev[3 * kChunk];
-    for (int i = 0; i < 3 * kChunk; ++i) BRKGA_HIP_TRY(hipEventCreate(&ev[i]));
-    double sum_step = 0.0, sum_close = 0.0;
+    hipEvent_t ev[3];
+    for (int i = 0; i < 3; ++i) BRKGA_HIP_TRY(hipEventCreate(&ev[i]));
     brkga_status st = BRKGA_OK;
-    for (int done = 0; done < generations && st == BRKGA_OK;) {
-        const int m = std::min(kChunk, generations - done);
-        for (int i = 0; i < m && st == BRKGA_OK; ++i) {
-            const int cur = pop->host_t & 1;
-            hipError_t e = hipEventRecord(ev[3 * i], s);
-            if (e == hipSuccess) {
-                e = pop->nsga2 ? launch_nsga2_offspring(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds,
-                                                        0, s)
-                               : launch_step_one(d, pop->block, cur, 0, s);
-            }
-            if (e == hipSuccess) e = hipEventRecord(ev[3 * i + 1], s);
-            if (e == hipSuccess) {
-                if (pop->nsga2) {
-                    e = launch_nsga2_select(pop->d_desc, 1, 0, d.pop, s);
-                    if (e == hipSuccess) e = launch_nsga2_gather(pop->d_desc, 1, d.pop, s);
-                } else {
-                    e = launch_partition_one(d, cur ^ 1, 1, s);
-                }
-            }
+    hipError_t e = hipSuccess;
+    float a = 0.f, b = 0.f;
+    if (!pop->nsga2) {
+        // the step reads generation t and rewrites the same rows of t + 1 on
+        // every launch; the partition re-sorts generation t in place
+        const int cur = pop->host_t & 1;
+        e = launch_step_one(d, pop->block, cur, 0, s);  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(ev[0], s);
+        for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch_step_one(d, pop->block, cur, 0, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+        for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch_partition_one(d, cur, 0, s);
+        if (e == hipSuccess) e = hipEventRecord(ev[2], s);
+    } else {
+        // selection advances the counter: one event pair per real generation
+        double sa = 0.0, sb = 0.0;
+        for (int i = 0; i < reps && e == hipSuccess; ++i) {
+            e = hipEventRecord(ev[0], s);
+            if (e == hipSuccess)
+                e = launch_nsga2_offspring(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, 0, s);
+            if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+            if (e == hipSuccess) e = launch_nsga2_select(pop->d_desc, 1, 0, d.pop, s);
+            if (e == hipSuccess) e = launch_nsga2_gather(pop->d_desc, 1, d.pop, s);
+            if (e == hipSuccess) e = hipEventRecord(ev[2], s);
+            if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&a, ev[0], ev[1]);
+            if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
+            sa += a;
+            sb += b;
             ++pop->host_t;
-            if (e == hipSuccess) e = hipEventRecord(ev[3 * i + 2], s);
-            if (e != hipSuccess) st = hip_error(e, "profiled launch");
         }
-        if (st == BRKGA_OK) {
-            hipError_t e = hipEventSynchronize(ev[3 * (m - 1) + 2]);
-            if (e != hipSuccess) st = hip_error(e, "hipEventSynchronize");
-        }
-        for (int i = 0; i < m && st == BRKGA_OK; ++i) {
-            float a = 0.f, b = 0.f;
-            hipError_t e = hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]);
-            if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]);
-            if (e != hipSuccess) st = hip_error(e, "hipEventElapsedTime");
-            sum_step += a;
-            sum_close += b;
-        }
-        done += m;
+        a = (float)sa;
+        b = (float)sb;
     }
-    for (int i = 0; i < 3 * kChunk; ++i) (void)hipEventDestroy(ev[i]);
+    if (e == hipSuccess && !pop->nsga2) {
+        e = hipEventSynchronize(ev[2]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&a, ev[0], ev[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
+    }
+    if (e != hipSuccess) st = hip_error(e, "profiled launches");
+    for (int i = 0; i < 3; ++i) (void)hipEventDestroy(ev[i]);
     if (st != BRKGA_OK) return st;
-    if (step_us) *step_us = 1000.0 * sum_step / generations;
-    if (close_us) *close_us = 1000.0 * sum_close / generations;
+    if (step_us) *step_us = 1000.0 * a / reps;
+    if (close_us) *close_us = 1000.0 * b / reps;
     return BRKGA_OK;
 }
 
