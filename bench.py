@@ -297,8 +297,10 @@ def main():
     def sync():
         return pop.generation  # synchronises the stream the steps run on
 
-    # warm-up: plain steps, then one run of the K-step graph (captures it)
+    # warm-up: plain steps, workgroup-size tuning, then one run of the
+    # K-step graph (captures it)
     pop.evolve(W)
+    block = pop.autotune(20)
     pop.evolve(K, graph=True)
     sync()
 
@@ -334,10 +336,10 @@ def main():
         "config": {"workload": "TSP n=1000, P=1024 = 4 islands x 256, elite 38 + mutants 38 per island, "
                                "rho 0.7, ring migration top-2 every 50 generations",
                    "n": L, "population": P, "islands": ISLANDS, "elite": ELITE, "mutants": M, "rho": RHO,
-                   "launch": "hipGraph replay of the K-generation loop"},
+                   "launch": "hipGraph replay of the K-generation loop", "step_block": block},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                     "kernel": "brkga::k_step_one<0, 256> (crossover / mutants / elite copy + decode + fitness)",
+                     "kernel": f"brkga::k_step_one<0, {block}> (crossover / mutants / elite copy + decode + fitness)",
                      "kernel_us": round(step_us, 3), "partition_us": round(close_us, 3),
                      "bytes_per_generation": B,
                      "generation_frac": round(B * gens_per_s / (HBM_PEAK_GBS * 1e9), 4)},

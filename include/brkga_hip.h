@@ -206,6 +206,13 @@ brkga_status brkga_get_best(brkga_pop_t pop, float* fitness, float* keys, int32_
 brkga_status brkga_population_device_buffers(brkga_pop_t pop, float** d_keys, float** d_fitness,
                                              int32_t** d_order, int32_t* ld);
 
+/* Picks the workgroup size of the population's generation kernel by timing
+ * `reps` back-to-back launches of each admissible size (64 ... 1024, at
+ * least n / 16) on the current generation; the launches rewrite the same
+ * next-generation rows, so the population does not change.  Results never
+ * depend on the size.  *block (may be NULL) receives the choice. */
+brkga_status brkga_population_autotune(brkga_pop_t pop, int32_t reps, int32_t* block, void* stream);
+
 /* Mean device time of the generation kernel (BRKGA step / NSGA-II
  * offspring) and of the closing kernels (partition / NSGA-II selection +
  * gather), in microseconds, from HIP events on `stream`.  BRKGA: `reps`

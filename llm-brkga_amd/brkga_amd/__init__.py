@@ -136,6 +136,7 @@ _SIGNATURES = {
                                         ctypes.c_int),
     "brkga_population_profile": ([_P, _I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _P],
                                  ctypes.c_int),
+    "brkga_population_autotune": ([_P, _I32, _PI, _P], ctypes.c_int),
     "brkga_batch_create": ([_P, _I32, ctypes.POINTER(_P), ctypes.POINTER(CParams), ctypes.POINTER(_P)],
                            ctypes.c_int),
     "brkga_batch_destroy": ([_P], ctypes.c_int),
@@ -359,6 +360,12 @@ class Population:
         dec = np.empty(L, np.int32)
         _check(lib().brkga_get_best(self.handle, _fptr(fit), _fptr(keys), _iptr(dec), stream or None))
         return (fit if M > 1 else float(fit[0])), keys, dec
+
+    def autotune(self, reps: int = 20, stream: int = 0) -> int:
+        """Fastest workgroup size of the generation kernel (results unchanged)."""
+        b = ctypes.c_int32()
+        _check(lib().brkga_population_autotune(self.handle, reps, ctypes.byref(b), stream or None))
+        return b.value
 
     def profile(self, reps: int, stream: int = 0):
         """Mean (generation kernel, closing kernels) device time in

@@ -834,6 +834,59 @@ brkga_status brkga_population_device_buffers(brkga_pop_t pop, float** d_keys, fl
     return BRKGA_OK;
 }
 
+brkga_status brkga_population_autotune(brkga_pop_t pop, int32_t reps, int32_t* block, void* stream)
+{
+    if (!pop || reps <= 0) return set_error(BRKGA_ERR_INVALID_ARG, "null population or no repetitions");
+    std::lock_guard<std::mutex> lock(pop->mu);
+    DeviceGuard g(pop->problem->engine->device);
+    hipStream_t s = as_stream(stream);
+    const InstDesc& d = pop->host;
+    const int n = d.prob.n;
+    hipEvent_t a, b;
+    BRKGA_HIP_TRY(hipEventCreate(&a));
+    hipError_t e = hipEventCreate(&b);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return hip_error(e, "hipEventCreate");
+    }
+    const int cur = pop->host_t & 1;
+    const int keep = pop->block;
+    int best = keep;
+    float best_ms = 0.0f;
+    bool have = false;
+    // the step (BRKGA) / offspring (NSGA-II) kernel rewrites the same rows
+    // from unchanged inputs on every launch: timing it changes nothing
+    auto launch = [&](int blk) {
+        return pop->nsga2 ? launch_nsga2_offspring(d.prob.kind, blk, pop->d_desc, 1, d.pop, pop->step_lds, 0, s)
+                          : launch_step_one(d, blk, cur, 0, s);
+    };
+    for (int blk = 64; blk <= 1024 && e == hipSuccess; blk <<= 1) {
+        if (blk * kSortItems < n) continue;
+        e = launch(blk);
+        if (e == hipSuccess) e = hipEventRecord(a, s);
+        for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch(blk);
+        if (e == hipSuccess) e = hipEventRecord(b, s);
+        if (e == hipSuccess) e = hipEventSynchronize(b);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+        if (e == hipSuccess && (!have || ms < best_ms)) {
+            have = true;
+            best_ms = ms;
+            best = blk;
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (e != hipSuccess) return hip_error(e, "autotune launches");
+    if (best != keep) {
+        pop->block = best;
+        for (auto& kv : pop->graphs) (void)hipGraphExecDestroy(kv.second);
+        pop->graphs.clear();
+    }
+    if (block) *block = best;
+    return BRKGA_OK;
+}
+
 brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* step_us, double* close_us,
                                       void* stream)
 {

@@ -341,22 +341,19 @@ __device__ __forceinline__ void scan_bucket_counts(int* cnt, int nb, int n, int*
     __syncthreads();
 }
 
-__device__ __forceinline__ void cswap(unsigned long long& a, unsigned long long& b)
-{
-    const unsigned long long lo = a < b ? a : b;
-    b = a < b ? b : a;
-    a = lo;
-}
-
 // perm[p] = index of the element at position p of the stable ascending order
 // of keys[0..n) (equal keys keep ascending index order, i.e. std::sort on
 // pair<float,int>(key, index) as configs/tsp_config.hpp:209-216 does).
 //
 // Bucket sort: keys are counted into nb value buckets (LDS atomics), the
-// counts are scanned into bucket ranges, every key drops its
-// (order key, index) item into its range, and the first claimant of each
-// bucket sorts that bucket in registers and writes its ranks.  keys and
-// perm live in LDS; the result is ready after the final barrier.
+// counts are scanned into bucket ranges and every key drops its
+// (order key, index) item into its range.  Items are unique, so each key's
+// final position is its bucket start plus the number of smaller items in
+// its bucket (buckets hold about two items on average); every key ranks
+// itself, with no per-bucket leader and no divergent sorting networks.
+// keys and perm live in LDS and may overlay each other: keys are last read
+// before the barrier that precedes the first write of perm.  The result is
+// ready after the final barrier.
 // CLEARED: the caller already ran argsort_clear and a barrier.
 template <int BLOCK, bool CLEARED = false>
 __device__ void block_argsort(const float* keys, int n, int nb, const SortScratch& s, int* perm)
@@ -387,47 +384,13 @@ __device__ void block_argsort(const float* keys, int n, int nb, const SortScratc
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
         const int e = tid + i * BLOCK;
-        if (e < n && slot[i] == 0) {
+        if (e < n) {
             const int st = s.cnt[bkt[i]];
-            const int sz = s.cnt[bkt[i] + 1] - st;
-            if (sz == 1) {
-                perm[st] = e;
-            } else if (sz <= 4) {
-                unsigned long long v0 = s.items[st], v1 = s.items[st + 1];
-                unsigned long long v2 = sz > 2 ? s.items[st + 2] : ~0ull;
-                unsigned long long v3 = sz > 3 ? s.items[st + 3] : ~0ull;
-                cswap(v0, v1);
-                cswap(v2, v3);
-                cswap(v0, v2);
-                cswap(v1, v3);
-                cswap(v1, v2);
-                perm[st] = (int)(uint32_t)v0;
-                perm[st + 1] = (int)(uint32_t)v1;
-                if (sz > 2) perm[st + 2] = (int)(uint32_t)v2;
-                if (sz > 3) perm[st + 3] = (int)(uint32_t)v3;
-            } else if (sz <= 8) {
-                unsigned long long v[8];
-#pragma unroll
-                for (int k = 0; k < 8; ++k) v[k] = k < sz ? s.items[st + k] : ~0ull;
-                // Batcher's odd-even merge network for 8 (19 exchanges)
-                cswap(v[0], v[1]); cswap(v[2], v[3]); cswap(v[4], v[5]); cswap(v[6], v[7]);
-                cswap(v[0], v[2]); cswap(v[1], v[3]); cswap(v[4], v[6]); cswap(v[5], v[7]);
-                cswap(v[1], v[2]); cswap(v[5], v[6]);
-                cswap(v[0], v[4]); cswap(v[1], v[5]); cswap(v[2], v[6]); cswap(v[3], v[7]);
-                cswap(v[2], v[4]); cswap(v[3], v[5]);
-                cswap(v[1], v[2]); cswap(v[3], v[4]); cswap(v[5], v[6]);
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    if (k < sz) perm[st + k] = (int)(uint32_t)v[k];
-            } else {
-                // rare large bucket: rank every member by counting
-                for (int q = st; q < st + sz; ++q) {
-                    const unsigned long long x = s.items[q];
-                    int r = 0;
-                    for (int p = st; p < st + sz; ++p) r += s.items[p] < x;
-                    perm[st + r] = (int)(uint32_t)x;
-                }
-            }
+            const int en = s.cnt[bkt[i] + 1];
+            const unsigned long long mine = s.items[st + slot[i]];
+            int r = 0;
+            for (int q = st; q < en; ++q) r += s.items[q] < mine ? 1 : 0;
+            perm[st + r] = e;
         }
     }
     __syncthreads();
@@ -453,10 +416,10 @@ __host__ __device__ inline int pow2_at_least(int v)
     return p;
 }
 
-// Buckets used to argsort n keys: the power of two >= n, at least 64.
+// Buckets used to argsort n keys: the power of two >= n / 2, at least 64.
 __host__ __device__ inline int sort_buckets(int n)
 {
-    const int p = pow2_at_least(n);
+    const int p = pow2_at_least((n + 1) / 2);
     return p < 64 ? 64 : p;
 }
 

@@ -78,7 +78,8 @@ struct DecodeLayout {
 };
 
 // LDS of a generation-step block: the decode layout, then the staged
-// coordinates, then the staged rank order of the block's island.
+// coordinates (when staged), then the staged rank order of the block's
+// island.
 struct StepLayout {
     DecodeLayout dec;
     int xy_off, isl_off, bytes;
@@ -102,10 +103,13 @@ __host__ __device__ inline DecodeLayout decode_layout(int kind, int n, int L, in
     d.items_off = off;
     off += n * 8;
     off = round_up_int(off, 16);
+    // the decoded order overlays the keys: the argsort reads each key for
+    // the last time before it writes the first position
     d.keys_off = off;
-    off += round_up_int(L, 32) * 4;
     d.perm_off = off;
-    off += round_up_int(kind == KIND_TSPJ ? 2 * n : n, 4) * 4 + 16;
+    const int key_bytes = round_up_int(L, 32) * 4;
+    const int perm_bytes = round_up_int(kind == KIND_TSPJ ? 2 * n : n, 4) * 4 + 16;
+    off += key_bytes > perm_bytes ? key_bytes : perm_bytes;
     d.cnt_off = off;
     off += (nb + 4) * 4;
     d.bytes = round_up_int(off, 16);
@@ -118,12 +122,12 @@ constexpr int kSortItems = 16;
 // Rank orders longer than this are read from global memory, not staged.
 constexpr int kMaxStagedOrder = 2048;
 
-__host__ __device__ inline StepLayout step_layout(int kind, int n, int L, int nb, int Pi)
+__host__ __device__ inline StepLayout step_layout(int kind, int n, int L, int nb, int Pi, bool stage_xy = true)
 {
     StepLayout s;
     s.dec = decode_layout(kind, n, L, nb);
     s.xy_off = s.dec.bytes;
-    s.isl_off = s.xy_off + round_up_int(coords_bytes(kind, n), 16);
+    s.isl_off = s.xy_off + (stage_xy ? round_up_int(coords_bytes(kind, n), 16) : 0);
     s.bytes = s.isl_off + round_up_int(4 * (Pi <= kMaxStagedOrder ? Pi : 0), 16);
     return s;
 }

@@ -114,11 +114,13 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
                 b.n_max = n;
                 auto it = tuned_.find({b.kind, key});
                 if (it != tuned_.end()) {
-                    b.block = it->second.first;
-                    b.chunk = it->second.second;
+                    b.block = it->second.block;
+                    b.chunk = it->second.chunk;
+                    b.xys = it->second.xys;
                 } else {
                     b.block = default_block(key);
                     b.chunk = default_chunk(key);
+                    b.xys = b.kind == KIND_TSP_COORDS && key <= 2048;
                 }
                 buckets_.push_back(b);
             }
@@ -128,7 +130,7 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
             b.n_min = std::min(b.n_min, n);
             b.n_max = std::max(b.n_max, n);
             b.max_Pi = std::max(b.max_Pi, d.Pi);
-            b.lds = std::max(b.lds, step_lds_bytes(d));
+            b.lds = std::max(b.lds, step_layout(d.prob.kind, n, d.L, d.nb, d.Pi, b.xys).bytes);
         } else {
             if (groups_.empty() || groups_.back().kind != d.prob.kind) {
                 FleetGroup gr{};
@@ -179,7 +181,7 @@ brkga_status Fleet::build_items(hipStream_t s)
                     items.push_back(make_int4(m, isl * d.Pi + s0, isl * d.Pi + std::min(s0 + b.chunk, d.Pi), 0));
         }
         b.item_count = (int)items.size() - b.item_begin;
-        const int per_cu = step_queue_blocks_per_cu(b.kind, b.block, b.lds);
+        const int per_cu = step_queue_blocks_per_cu(b.kind, b.block, b.xys, b.lds);
         if (per_cu <= 0) return set_error(BRKGA_ERR_DEVICE, "queue step kernel does not fit on a CU");
         b.grid = std::min(b.item_count, per_cu * eng_->cu_count);
     }
@@ -251,7 +253,7 @@ brkga_status Fleet::tick(hipStream_t s)
         }
     for (size_t i = 0; i < buckets_.size() && e == hipSuccess; ++i) {
         const FleetBucket& b = buckets_[i];
-        e = launch_step_queue(b.kind, b.block, d_descs_, d_items_ + b.item_begin, b.item_count,
+        e = launch_step_queue(b.kind, b.block, b.xys, d_descs_, d_items_ + b.item_begin, b.item_count,
                               d_queues_ + kQueueStride * i, b.grid, b.lds, lane(b.lane));
     }
     for (size_t i = 0; i < groups_.size() && e == hipSuccess; ++i) {
@@ -307,6 +309,17 @@ brkga_status Fleet::tick_graph(int ticks, hipStream_t s)
     return BRKGA_OK;
 }
 
+// LDS of bucket b's launch with coordinates staged or not.
+static int bucket_lds(const std::vector<InstDesc>& descs, const FleetBucket& b, bool xys)
+{
+    int lds = 0;
+    for (int m = b.first; m < b.first + b.count; ++m) {
+        const InstDesc& d = descs[m];
+        lds = std::max(lds, step_layout(d.prob.kind, d.prob.n, d.L, d.nb, d.Pi, xys).bytes);
+    }
+    return lds;
+}
+
 brkga_status Fleet::autotune(int reps, hipStream_t s)
 {
     DeviceGuard g(eng_->device);
@@ -321,44 +334,53 @@ brkga_status Fleet::autotune(int reps, hipStream_t s)
     brkga_status st = BRKGA_OK;
     for (size_t bi = 0; bi < buckets_.size() && st == BRKGA_OK; ++bi) {
         float best_ms = 0.0f;
-        int best_block = buckets_[bi].block, best_chunk = buckets_[bi].chunk;
+        Shape best{buckets_[bi].block, buckets_[bi].chunk, buckets_[bi].xys};
         bool have = false;
-        for (int blk : kBlocks) {
-            if (blk * kSortItems < buckets_[bi].n_max) continue;
-            if (step_queue_blocks_per_cu(buckets_[bi].kind, blk, buckets_[bi].lds) <= 0) continue;
-            for (int ch : kChunks) {
-                if (ch > buckets_[bi].max_Pi) continue;
-                buckets_[bi].block = blk;
-                buckets_[bi].chunk = ch;
-                if ((st = build_items(s)) != BRKGA_OK) break;
-                const FleetBucket& bk = buckets_[bi];
-                int* q = d_queues_ + kQueueStride * bi;
-                e = launch_step_queue(bk.kind, bk.block, d_descs_, d_items_ + bk.item_begin, bk.item_count, q,
-                                      bk.grid, bk.lds, s);
-                if (e == hipSuccess) e = hipEventRecord(a, s);
-                for (int r = 0; r < reps && e == hipSuccess; ++r)
-                    e = launch_step_queue(bk.kind, bk.block, d_descs_, d_items_ + bk.item_begin, bk.item_count, q,
-                                          bk.grid, bk.lds, s);
-                if (e == hipSuccess) e = hipEventRecord(b, s);
-                if (e == hipSuccess) e = hipEventSynchronize(b);
-                float ms = 0.0f;
-                if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
-                if (e != hipSuccess) {
-                    st = hip_error(e, "autotune");
-                    break;
+        const int stagings = buckets_[bi].kind == KIND_TSP_COORDS ? 2 : 1;
+        for (int xi = 0; xi < stagings && st == BRKGA_OK; ++xi) {
+            const bool xys = buckets_[bi].kind == KIND_TSP_COORDS && xi == 0;
+            const int lds = bucket_lds(descs_, buckets_[bi], xys);
+            if ((size_t)lds > eng_->lds_limit) continue;
+            for (int blk : kBlocks) {
+                if (blk * kSortItems < buckets_[bi].n_max) continue;
+                if (step_queue_blocks_per_cu(buckets_[bi].kind, blk, xys, lds) <= 0) continue;
+                for (int ch : kChunks) {
+                    if (ch > buckets_[bi].max_Pi) continue;
+                    buckets_[bi].block = blk;
+                    buckets_[bi].chunk = ch;
+                    buckets_[bi].xys = xys;
+                    buckets_[bi].lds = lds;
+                    if ((st = build_items(s)) != BRKGA_OK) break;
+                    const FleetBucket& bk = buckets_[bi];
+                    int* q = d_queues_ + kQueueStride * bi;
+                    e = launch_step_queue(bk.kind, bk.block, bk.xys, d_descs_, d_items_ + bk.item_begin,
+                                          bk.item_count, q, bk.grid, bk.lds, s);
+                    if (e == hipSuccess) e = hipEventRecord(a, s);
+                    for (int r = 0; r < reps && e == hipSuccess; ++r)
+                        e = launch_step_queue(bk.kind, bk.block, bk.xys, d_descs_, d_items_ + bk.item_begin,
+                                              bk.item_count, q, bk.grid, bk.lds, s);
+                    if (e == hipSuccess) e = hipEventRecord(b, s);
+                    if (e == hipSuccess) e = hipEventSynchronize(b);
+                    float ms = 0.0f;
+                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+                    if (e != hipSuccess) {
+                        st = hip_error(e, "autotune");
+                        break;
+                    }
+                    if (!have || ms < best_ms) {
+                        have = true;
+                        best_ms = ms;
+                        best = Shape{blk, ch, xys};
+                    }
                 }
-                if (!have || ms < best_ms) {
-                    have = true;
-                    best_ms = ms;
-                    best_block = blk;
-                    best_chunk = ch;
-                }
+                if (st != BRKGA_OK) break;
             }
-            if (st != BRKGA_OK) break;
         }
-        buckets_[bi].block = best_block;
-        buckets_[bi].chunk = best_chunk;
-        tuned_[{buckets_[bi].kind, buckets_[bi].key}] = {best_block, best_chunk};
+        buckets_[bi].block = best.block;
+        buckets_[bi].chunk = best.chunk;
+        buckets_[bi].xys = best.xys;
+        buckets_[bi].lds = bucket_lds(descs_, buckets_[bi], best.xys);
+        tuned_[{buckets_[bi].kind, buckets_[bi].key}] = best;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);

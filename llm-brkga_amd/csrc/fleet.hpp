@@ -28,6 +28,7 @@ struct FleetBucket {
     int n_min, n_max, max_Pi;
     int lds;           // largest step LDS footprint among the members
     int block, chunk;  // launch shape; results do not depend on it
+    bool xys;          // coordinates staged in LDS (coordinate kind only)
     int item_begin, item_count, grid;
     double work;       // sum of population x n over the members
     int lane;          // stream lane of the bucket's launch (0 = caller's stream)
@@ -81,8 +82,13 @@ private:
     std::vector<FleetBucket> buckets_;
     std::vector<FleetGroup> groups_;
     int n_brkga_ = 0, total_islands_ = 0, max_Pi_ = 0;
-    // launch shape per (kind, key), kept across member changes
-    std::map<std::pair<int, int>, std::pair<int, int>> tuned_;
+    // launch shape (block, chunk, staged coordinates) per (kind, key), kept
+    // across member changes
+    struct Shape {
+        int block, chunk;
+        bool xys;
+    };
+    std::map<std::pair<int, int>, Shape> tuned_;
 
     InstDesc* d_descs_ = nullptr;
     size_t descs_cap_ = 0;

@@ -182,9 +182,11 @@ __device__ __forceinline__ void partition_body(const InstDesc& d, int island, in
         for (int i = threadIdx.x; i < Pi; i += B) perm[i] = (int)(sk[i] & 0xFFFFFFFFull);
     }
     if (advance) {
+        // The counter only has to move once every island block is done; no
+        // block reads what another one wrote (the next kernel sees the rank
+        // orders through the kernel boundary), so no fence is needed here.
         __syncthreads();
         if (threadIdx.x == 0) {
-            __threadfence();
             const int old = atomicAdd(d.done, 1);
             if (old == d.islands - 1) {
                 *d.done = 0;

@@ -17,14 +17,16 @@ namespace brkga {
 // t >= gen_limit leaves the instance untouched.  queue[0] hands out items
 // and queue[1] counts retired workgroups; the last workgroup re-arms both,
 // so the kernel can be relaunched (or replayed from a graph) as is.
-template <int KIND, int BLOCK>
+// XYS: Euclidean coordinates are staged in LDS per item (otherwise the
+// decoder reads them through L2, leaving LDS to more resident workgroups).
+template <int KIND, int BLOCK, bool XYS>
 __global__ __launch_bounds__(BLOCK) void k_step_queue(const InstDesc* __restrict__ descs,
                                                       const int4* __restrict__ items, int num_items,
                                                       int* __restrict__ queue)
 {
     extern __shared__ __align__(16) char lds[];
     __shared__ int s_item;
-    constexpr bool kXY = KIND == KIND_TSP_COORDS;
+    constexpr bool kXY = KIND == KIND_TSP_COORDS && XYS;
     for (;;) {
         __syncthreads();  // the previous item is done with LDS and s_item
         if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
@@ -40,7 +42,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_queue(const InstDesc* __restrict
         const int nxt = init ? 0 : cur ^ 1;
         const int Pi = d.Pi, n = d.prob.n, M = d.prob.num_obj, ld = d.ld, nb = d.nb;
         const int island = w.y / Pi;
-        const StepLayout sl = step_layout(KIND, n, d.L, nb, Pi);
+        const StepLayout sl = step_layout(KIND, n, d.L, nb, Pi, kXY);
         float4* kl = reinterpret_cast<float4*>(lds + sl.dec.keys_off);
         float2* xy = reinterpret_cast<float2*>(lds + sl.xy_off);
         int* order = reinterpret_cast<int*>(lds + sl.isl_off);
@@ -65,7 +67,8 @@ __global__ __launch_bounds__(BLOCK) void k_step_queue(const InstDesc* __restrict
                 build_child<BLOCK>(d, island, s, cur, init, t, staged ? order : perm_cur, out, kl, fit_out);
             if (!child) continue;
             __syncthreads();
-            decode_block<KIND, BLOCK, true, kXY>(d.prob, lds, sl.dec, nb, fit_out, nullptr, xy, nullptr);
+            decode_block<KIND, BLOCK, true, kXY>(d.prob, lds, sl.dec, nb, fit_out, nullptr, kXY ? xy : nullptr,
+                                                 nullptr);
         }
     }
     if (threadIdx.x == 0) {
@@ -97,39 +100,40 @@ void ensure_lds(F func, int bytes)
                                   bytes);
 }
 
-// Kernel symbol of the queue step for (kind, block); nullptr when the pair
-// has no instantiation.
-template <int KIND>
+// Kernel symbol of the queue step for (kind, block, coordinate staging);
+// nullptr when there is no such instantiation.
+template <int KIND, bool XYS>
 const void* queue_kernel_kind(int block)
 {
     switch (block) {
-    case 64: return reinterpret_cast<const void*>(k_step_queue<KIND, 64>);
-    case 128: return reinterpret_cast<const void*>(k_step_queue<KIND, 128>);
-    case 256: return reinterpret_cast<const void*>(k_step_queue<KIND, 256>);
-    case 512: return reinterpret_cast<const void*>(k_step_queue<KIND, 512>);
-    case 1024: return reinterpret_cast<const void*>(k_step_queue<KIND, 1024>);
+    case 64: return reinterpret_cast<const void*>(k_step_queue<KIND, 64, XYS>);
+    case 128: return reinterpret_cast<const void*>(k_step_queue<KIND, 128, XYS>);
+    case 256: return reinterpret_cast<const void*>(k_step_queue<KIND, 256, XYS>);
+    case 512: return reinterpret_cast<const void*>(k_step_queue<KIND, 512, XYS>);
+    case 1024: return reinterpret_cast<const void*>(k_step_queue<KIND, 1024, XYS>);
     default: return nullptr;
     }
 }
 
-const void* queue_kernel(int kind, int block)
+const void* queue_kernel(int kind, int block, bool xys)
 {
     switch (kind) {
-    case KIND_TSP_COORDS: return queue_kernel_kind<KIND_TSP_COORDS>(block);
-    case KIND_TSP_MATRIX: return queue_kernel_kind<KIND_TSP_MATRIX>(block);
-    case KIND_KNAP_THRESHOLD: return queue_kernel_kind<KIND_KNAP_THRESHOLD>(block);
-    case KIND_KNAP_GREEDY: return queue_kernel_kind<KIND_KNAP_GREEDY>(block);
-    case KIND_TSPJ: return queue_kernel_kind<KIND_TSPJ>(block);
+    case KIND_TSP_COORDS:
+        return xys ? queue_kernel_kind<KIND_TSP_COORDS, true>(block) : queue_kernel_kind<KIND_TSP_COORDS, false>(block);
+    case KIND_TSP_MATRIX: return xys ? nullptr : queue_kernel_kind<KIND_TSP_MATRIX, false>(block);
+    case KIND_KNAP_THRESHOLD: return xys ? nullptr : queue_kernel_kind<KIND_KNAP_THRESHOLD, false>(block);
+    case KIND_KNAP_GREEDY: return xys ? nullptr : queue_kernel_kind<KIND_KNAP_GREEDY, false>(block);
+    case KIND_TSPJ: return xys ? nullptr : queue_kernel_kind<KIND_TSPJ, false>(block);
     default: return nullptr;
     }
 }
 
 }  // namespace
 
-hipError_t launch_step_queue(int kind, int block, const InstDesc* d_descs, const int4* d_items, int num_items,
-                             int* d_queue, int grid, int lds, hipStream_t s)
+hipError_t launch_step_queue(int kind, int block, bool xys, const InstDesc* d_descs, const int4* d_items,
+                             int num_items, int* d_queue, int grid, int lds, hipStream_t s)
 {
-    const void* f = queue_kernel(kind, block);
+    const void* f = queue_kernel(kind, block, xys);
     if (!f) return hipErrorInvalidValue;
     if (num_items <= 0 || grid <= 0) return hipSuccess;
     ensure_lds(f, lds);
@@ -137,9 +141,9 @@ hipError_t launch_step_queue(int kind, int block, const InstDesc* d_descs, const
     return hipLaunchKernel(f, dim3(grid), dim3(block), args, (size_t)lds, s);
 }
 
-int step_queue_blocks_per_cu(int kind, int block, int lds)
+int step_queue_blocks_per_cu(int kind, int block, bool xys, int lds)
 {
-    const void* f = queue_kernel(kind, block);
+    const void* f = queue_kernel(kind, block, xys);
     if (!f) return 0;
     ensure_lds(f, lds);
     int nb = 0;

@@ -25,12 +25,13 @@ hipError_t launch_partition_many(const InstDesc* d_descs, int num_inst, int bloc
                                  hipStream_t s);
 
 // Persistent queue step over work items {instance, first slot, end slot, -}
-// of one problem kind (batched / served populations).  d_queue holds two
+// of one problem kind (batched / served populations).  xys: stage the
+// Euclidean coordinates in LDS (coordinate kind only).  d_queue holds two
 // counters that must be zero at the first launch; the kernel re-arms them.
-hipError_t launch_step_queue(int kind, int block, const InstDesc* d_descs, const int4* d_items, int num_items,
-                             int* d_queue, int grid, int lds, hipStream_t s);
+hipError_t launch_step_queue(int kind, int block, bool xys, const InstDesc* d_descs, const int4* d_items,
+                             int num_items, int* d_queue, int grid, int lds, hipStream_t s);
 // Resident workgroups of the queue step per CU (0: no such kernel).
-int step_queue_blocks_per_cu(int kind, int block, int lds);
+int step_queue_blocks_per_cu(int kind, int block, bool xys, int lds);
 // Step counter := gen and arrival counter := 0 for `num` instances.
 hipError_t launch_set_counters(const InstDesc* d_descs, int num, int gen, hipStream_t s);
 

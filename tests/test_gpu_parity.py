@@ -174,6 +174,30 @@ def test_graph_replay_matches_plain(engine):
     assert b.generation == 15
 
 
+@pytest.mark.parametrize("kind,n,P,E,M", [("tsp", 300, 64, 10, 8), ("knap_greedy", 500, 64, 10, 8),
+                                          ("bitsp", 60, 64, 0, 0)])
+def test_autotune_keeps_trajectory(engine, kind, n, P, E, M):
+    """Workgroup-size tuning times launches that rewrite the same rows: the
+    trajectory afterwards equals the untuned one."""
+    print(f"autotune trajectory {kind} n={n}", flush=True)
+    gp, _ = make_problem(engine, kind, n, inst=9)
+    prm = brkga_amd.Params(P, E, M, 0.7, 1, 0, 0, 1234)
+    a = brkga_amd.Population(gp, prm)
+    b = brkga_amd.Population(gp, prm)
+    a.initialize()
+    b.initialize()
+    a.evolve(2)
+    b.evolve(2)
+    blk = a.autotune(3)
+    assert blk in (64, 128, 256, 512, 1024) and blk * 16 >= n
+    a.evolve(4)
+    a.evolve(3, graph=True)
+    b.evolve(7)
+    ra, rb = a.read(), b.read()
+    for key in ra:
+        assert np.array_equal(ra[key], rb[key]), key
+
+
 def test_best_and_write(engine):
     gp, op = make_problem(engine, "tsp", 80, inst=4)
     prm = brkga_amd.Params(64, 12, 6, 0.7, 2, 0, 0, 5)
