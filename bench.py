@@ -200,8 +200,8 @@ def percentiles(lat):
     return round(1000.0 * pick(0.5), 3), round(1000.0 * pick(0.99), 3)
 
 
-def served_run(brkga_amd, eng, reqs, arrivals, budget):
-    srv = brkga_amd.Server(eng, hbm_budget_bytes=budget, ticks_per_round=4)
+def served_run(brkga_amd, eng, reqs, arrivals, budget, tune=0):
+    srv = brkga_amd.Server(eng, hbm_budget_bytes=budget, ticks_per_round=4, autotune_reps=tune)
     ids, t_sub = [], []
     t0 = time.perf_counter()
     for (kind, _, data, prm), ta in zip(reqs, arrivals):
@@ -242,8 +242,9 @@ def plain_run(brkga_amd, eng, reqs, arrivals):
 
 def served_leg(brkga_amd, workloads, eng, count, rates, budget):
     reqs = serve_requests(brkga_amd, workloads, count)
-    # warm-up of every kernel shape the mix uses
-    served_run(brkga_amd, eng, reqs[:16], [0.0] * 16, budget)
+    # warm-up: the whole mix at once, tuning the launch shape of every
+    # (kind, size class) it meets; the engine keeps the shapes
+    served_run(brkga_amd, eng, reqs, [0.0] * count, budget, tune=2)
     rows, all_same = [], True
     for rate in rates:
         arrivals = workloads.poisson_arrivals(count, rate) if rate > 0 else [0.0] * count

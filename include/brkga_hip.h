@@ -252,9 +252,11 @@ brkga_status brkga_batch_destroy(brkga_batch_t batch);
 brkga_status brkga_batch_initialize(brkga_batch_t batch, int32_t mode, void* stream);
 brkga_status brkga_batch_evolve(brkga_batch_t batch, int32_t generations, int32_t mode, void* stream);
 
-/* Picks the workgroup size and slots per work item of each size bucket by
- * timing `reps` step launches per candidate on the current populations
- * (call after initialize).  Does not advance or change any population. */
+/* Picks the workgroup size, slots per work item and coordinate staging of
+ * each size bucket by timing `reps` step launches per candidate on the
+ * current populations (call after initialize).  Does not advance or change
+ * any population.  The choices are kept per engine and (problem kind, size
+ * class) and reused by later batches and servers of the engine. */
 brkga_status brkga_batch_autotune(brkga_batch_t batch, int32_t reps, void* stream);
 
 /* Member i as a population handle owned by the batch (read, get_best,
@@ -284,6 +286,8 @@ typedef struct {
     uint64_t hbm_budget_bytes; /* device arena for resident populations (0: 8 GiB) */
     int32_t max_resident;      /* most requests on the device at once (0: no cap) */
     int32_t ticks_per_round;   /* generations per resident request between admission rounds (0: 4) */
+    int32_t autotune_reps;     /* > 0: tune the launch shape of each (kind, size class) the engine has
+                                  not tuned yet, when it first appears, timing this many launches */
 } brkga_server_config;
 
 typedef struct {

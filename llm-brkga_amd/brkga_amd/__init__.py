@@ -84,7 +84,7 @@ class Params:
 
 class CServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_resident", ctypes.c_int32),
-                ("ticks_per_round", ctypes.c_int32)]
+                ("ticks_per_round", ctypes.c_int32), ("autotune_reps", ctypes.c_int32)]
 
 
 class CRequestInfo(ctypes.Structure):
@@ -475,9 +475,9 @@ class Server:
     deadline.  Populations are admitted while they fit `hbm_budget_bytes`."""
 
     def __init__(self, engine: Engine, hbm_budget_bytes: int = 0, max_resident: int = 0,
-                 ticks_per_round: int = 0):
+                 ticks_per_round: int = 0, autotune_reps: int = 0):
         self.engine = engine
-        cfg = CServerConfig(hbm_budget_bytes, max_resident, ticks_per_round)
+        cfg = CServerConfig(hbm_budget_bytes, max_resident, ticks_per_round, autotune_reps)
         h = _P()
         _check(lib().brkga_server_start(engine.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h

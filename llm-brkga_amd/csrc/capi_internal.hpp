@@ -47,6 +47,14 @@ struct brkga_engine_s {
     int device;
     size_t lds_limit;  // bytes of LDS one workgroup may use
     int cu_count;
+    // launch shapes the fleet autotuner picked, per (problem kind, size
+    // class); shared by every batch and server of the engine
+    struct Shape {
+        int block, chunk;
+        bool xys;
+    };
+    std::map<std::pair<int, int>, Shape> tuned;
+    std::mutex tuned_mu;
 };
 
 struct brkga_problem_s {

@@ -112,8 +112,9 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
                 b.first = (int)i;
                 b.n_min = n;
                 b.n_max = n;
-                auto it = tuned_.find({b.kind, key});
-                if (it != tuned_.end()) {
+                std::lock_guard<std::mutex> lock(eng_->tuned_mu);
+                auto it = eng_->tuned.find({b.kind, key});
+                if (it != eng_->tuned.end()) {
                     b.block = it->second.block;
                     b.chunk = it->second.chunk;
                     b.xys = it->second.xys;
@@ -322,7 +323,27 @@ static int bucket_lds(const std::vector<InstDesc>& descs, const FleetBucket& b, 
 
 brkga_status Fleet::autotune(int reps, hipStream_t s)
 {
+    return tune(reps, false, s);
+}
+
+brkga_status Fleet::autotune_new(int reps, hipStream_t s)
+{
+    return tune(reps, true, s);
+}
+
+brkga_status Fleet::tune(int reps, bool only_new, hipStream_t s)
+{
     DeviceGuard g(eng_->device);
+    std::vector<bool> todo(buckets_.size(), true);
+    bool any = false;
+    {
+        std::lock_guard<std::mutex> lock(eng_->tuned_mu);
+        for (size_t bi = 0; bi < buckets_.size(); ++bi) {
+            if (only_new && eng_->tuned.count({buckets_[bi].kind, buckets_[bi].key})) todo[bi] = false;
+            any = any || todo[bi];
+        }
+    }
+    if (!any) return BRKGA_OK;
     if (reps < 1) reps = 1;
     hipEvent_t a = nullptr, b = nullptr;
     BRKGA_HIP_TRY(hipEventCreate(&a));
@@ -333,6 +354,7 @@ brkga_status Fleet::autotune(int reps, hipStream_t s)
     }
     brkga_status st = BRKGA_OK;
     for (size_t bi = 0; bi < buckets_.size() && st == BRKGA_OK; ++bi) {
+        if (!todo[bi]) continue;
         float best_ms = 0.0f;
         Shape best{buckets_[bi].block, buckets_[bi].chunk, buckets_[bi].xys};
         bool have = false;
@@ -380,7 +402,8 @@ brkga_status Fleet::autotune(int reps, hipStream_t s)
         buckets_[bi].chunk = best.chunk;
         buckets_[bi].xys = best.xys;
         buckets_[bi].lds = bucket_lds(descs_, buckets_[bi], best.xys);
-        tuned_[{buckets_[bi].kind, buckets_[bi].key}] = best;
+        std::lock_guard<std::mutex> lock(eng_->tuned_mu);
+        eng_->tuned[{buckets_[bi].kind, buckets_[bi].key}] = best;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);

@@ -64,6 +64,9 @@ public:
     // step launches only, which rewrite the same next-generation buffers
     // from unchanged inputs, so the populations are not advanced.
     brkga_status autotune(int reps, hipStream_t s);
+    // Same, for the buckets whose (kind, size class) the engine has no
+    // tuned shape for yet.
+    brkga_status autotune_new(int reps, hipStream_t s);
 
     const std::vector<FleetBucket>& buckets() const { return buckets_; }
     const std::vector<brkga_pop_s*>& members() const { return members_; }
@@ -71,6 +74,7 @@ public:
 
 private:
     brkga_status build_items(hipStream_t s);
+    brkga_status tune(int reps, bool only_new, hipStream_t s);
     brkga_status reserve(void** p, size_t* cap, size_t bytes, const char* what);
     brkga_status ensure_lanes();
     void assign_lanes();
@@ -82,13 +86,7 @@ private:
     std::vector<FleetBucket> buckets_;
     std::vector<FleetGroup> groups_;
     int n_brkga_ = 0, total_islands_ = 0, max_Pi_ = 0;
-    // launch shape (block, chunk, staged coordinates) per (kind, key), kept
-    // across member changes
-    struct Shape {
-        int block, chunk;
-        bool xys;
-    };
-    std::map<std::pair<int, int>, Shape> tuned_;
+    using Shape = brkga_engine_s::Shape;
 
     InstDesc* d_descs_ = nullptr;
     size_t descs_cap_ = 0;

@@ -268,6 +268,8 @@ void brkga_server_s::run()
             std::vector<brkga_pop_s*> pops;
             for (auto& r : resident) pops.push_back(r->pop);
             st = fleet->set_members(pops, stream);
+            // step launches alone leave every population as it is
+            if (st == BRKGA_OK && cfg.autotune_reps > 0) st = fleet->autotune_new(cfg.autotune_reps, stream);
         }
         if (st == BRKGA_OK && !resident.empty()) {
             for (int k = 0; k < cfg.ticks_per_round && st == BRKGA_OK; ++k) st = fleet->tick(stream);

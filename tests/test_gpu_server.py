@@ -58,13 +58,13 @@ def check(result, ref, ctx):
     assert np.array_equal(dec, rdec), f"decoded solution differs ({ctx})"
 
 
-@pytest.mark.parametrize("budget_mib,max_resident,ticks", [(512, 0, 3), (24, 3, 1), (512, 2, 5)])
-def test_server_results_match_solo(engine, budget_mib, max_resident, ticks):
-    print(f"server mix budget={budget_mib} MiB max_resident={max_resident} ticks={ticks}", flush=True)
+@pytest.mark.parametrize("budget_mib,max_resident,ticks,tune", [(512, 0, 3, 0), (24, 3, 1, 0), (512, 2, 5, 1)])
+def test_server_results_match_solo(engine, budget_mib, max_resident, ticks, tune):
+    print(f"server mix budget={budget_mib} MiB max_resident={max_resident} ticks={ticks} tune={tune}", flush=True)
     reqs = build_requests(engine)
     refs = [solo_best(gp, prm, G) for gp, prm, G in reqs]
     srv = brkga_amd.Server(engine, hbm_budget_bytes=budget_mib << 20, max_resident=max_resident,
-                           ticks_per_round=ticks)
+                           ticks_per_round=ticks, autotune_reps=tune)
     ids = []
     for i, (gp, prm, G) in enumerate(reqs):
         ids.append(srv.submit(gp, prm, G))
