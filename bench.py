@@ -256,7 +256,9 @@ def served_leg(brkga_amd, workloads, eng, count, rates, budget):
         p50, p99 = percentiles(p_lat)
         rows.append({"rate_req_per_s": rate if rate > 0 else "all at t=0", "requests": count,
                      "served": {"solves_per_s": round(count / s_span, 2), "p50_ms": s50, "p99_ms": s99,
-                                "peak_resident_bytes": st["peak_bytes"], "ticks": st["ticks"]},
+                                "peak_resident_bytes": st["peak_bytes"], "ticks": st["ticks"],
+                                "rounds": st["rounds"], "worker_wait_ms": round(st["wait_us"] / 1000, 2),
+                                "worker_host_ms": round(st["host_us"] / 1000, 2)},
                      "plain": {"solves_per_s": round(count / p_span, 2), "p50_ms": p50, "p99_ms": p99},
                      "speedup_solves_per_s": round(p_span / s_span, 2), "results_identical": bool(same)})
     return {"workload": f"Poisson arrivals of {count} mixed requests per rate: 60% TSP n log-uniform [20,5000], "

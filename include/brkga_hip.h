@@ -305,6 +305,8 @@ typedef struct {
     uint64_t bytes_in_use, peak_bytes, budget_bytes;
     uint64_t ticks;   /* batched ticks run */
     uint64_t rounds;  /* admission rounds */
+    double wait_us;   /* worker time spent waiting for ticks to finish */
+    double host_us;   /* worker time spent on admission, membership and results */
 } brkga_server_stats;
 
 /* Starts the worker thread (config may be NULL for defaults). */

@@ -96,7 +96,8 @@ class CServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("admitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
                 ("rejected", ctypes.c_uint64), ("resident", ctypes.c_int32), ("pending", ctypes.c_int32),
                 ("bytes_in_use", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64),
-                ("budget_bytes", ctypes.c_uint64), ("ticks", ctypes.c_uint64), ("rounds", ctypes.c_uint64)]
+                ("budget_bytes", ctypes.c_uint64), ("ticks", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
+                ("wait_us", ctypes.c_double), ("host_us", ctypes.c_double)]
 
 
 _lib = None

@@ -357,6 +357,26 @@ void destroy_population(brkga_pop_s* pop)
     delete pop;
 }
 
+int pick_best(const InstDesc& d, int nsga2, const float* fit, const int* order_or_rank, const float* crowd)
+{
+    int best = -1;
+    if (nsga2) {
+        for (int i = 0; i < d.pop; ++i)
+            if (order_or_rank[i] == 0 && (best < 0 || crowd[i] > crowd[best])) best = i;
+        return best;
+    }
+    for (int i = 0; i < d.islands; ++i) {
+        const int cand = i * d.Pi + order_or_rank[(size_t)i * d.Pi];
+        if (best < 0) {
+            best = cand;
+            continue;
+        }
+        const float a = fit[cand], c = fit[best];
+        if (d.prob.maximize ? a > c : a < c) best = cand;
+    }
+    return best;
+}
+
 }  // namespace brkga
 
 using namespace brkga;
@@ -783,27 +803,16 @@ brkga_status brkga_get_best(brkga_pop_t pop, float* fitness, float* keys, int32_
     const int P = d.pop, M = d.prob.num_obj;
     std::vector<float> fit((size_t)P * M);
     BRKGA_HIP_TRY(hipMemcpy(fit.data(), d.fit[b], sizeof(float) * fit.size(), hipMemcpyDeviceToHost));
-    int best = -1;
+    std::vector<int> ord(P);
+    std::vector<float> crowd;
     if (pop->nsga2) {
-        std::vector<int> rank(P);
-        std::vector<float> crowd(P);
-        BRKGA_HIP_TRY(hipMemcpy(rank.data(), d.rank[b], sizeof(int) * P, hipMemcpyDeviceToHost));
+        crowd.resize(P);
+        BRKGA_HIP_TRY(hipMemcpy(ord.data(), d.rank[b], sizeof(int) * P, hipMemcpyDeviceToHost));
         BRKGA_HIP_TRY(hipMemcpy(crowd.data(), d.crowd[b], sizeof(float) * P, hipMemcpyDeviceToHost));
-        for (int i = 0; i < P; ++i)
-            if (rank[i] == 0 && (best < 0 || crowd[i] > crowd[best])) best = i;
     } else {
-        std::vector<int> perm(P);
-        BRKGA_HIP_TRY(hipMemcpy(perm.data(), d.perm[b], sizeof(int) * P, hipMemcpyDeviceToHost));
-        for (int i = 0; i < d.islands; ++i) {
-            const int cand = i * d.Pi + perm[(size_t)i * d.Pi];
-            if (best < 0) {
-                best = cand;
-                continue;
-            }
-            const float a = fit[cand], c = fit[best];
-            if (d.prob.maximize ? a > c : a < c) best = cand;
-        }
+        BRKGA_HIP_TRY(hipMemcpy(ord.data(), d.perm[b], sizeof(int) * P, hipMemcpyDeviceToHost));
     }
+    const int best = pick_best(d, pop->nsga2, fit.data(), ord.data(), crowd.data());
     if (best < 0) return set_error(BRKGA_ERR_DEVICE, "no best individual found");
     if (fitness)
         for (int m = 0; m < M; ++m) fitness[m] = fit[(size_t)best * M + m];

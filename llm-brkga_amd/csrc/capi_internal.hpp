@@ -115,4 +115,10 @@ brkga_status create_population(brkga_problem_s* pr, const brkga_params* q, Devic
 // for arena-backed populations).
 void destroy_population(brkga_pop_s* pop);
 
+// Slot of the best individual (Population::get_best): BRKGA, the better of
+// the islands' rank-0 slots (order_or_rank = rank order); NSGA-II, the
+// front-0 member with the largest crowding distance (order_or_rank = front
+// ranks), first index on ties.
+int pick_best(const InstDesc& d, int nsga2, const float* fit, const int* order_or_rank, const float* crowd);
+
 }  // namespace brkga

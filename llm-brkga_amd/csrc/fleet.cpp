@@ -76,6 +76,8 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
     DeviceGuard g(eng_->device);
     for (auto* p : pops)
         if (!p) return set_error(BRKGA_ERR_INVALID_ARG, "null population in fleet");
+    // earlier uploads read descs_ / items_ asynchronously
+    BRKGA_HIP_TRY(hipStreamSynchronize(s));
     // BRKGA members by (kind, size class, n descending), NSGA-II by (kind, n descending)
     std::vector<brkga_pop_s*> order(pops);
     std::stable_sort(order.begin(), order.end(), [](const brkga_pop_s* a, const brkga_pop_s* b) {
@@ -172,7 +174,10 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
 
 brkga_status Fleet::build_items(hipStream_t s)
 {
-    std::vector<int4> items;
+    // the previous upload from items_ must be done before it is rewritten
+    BRKGA_HIP_TRY(hipStreamSynchronize(s));
+    std::vector<int4>& items = items_;
+    items.clear();
     for (auto& b : buckets_) {
         b.item_begin = (int)items.size();
         for (int m = b.first; m < b.first + b.count; ++m) {
@@ -191,8 +196,6 @@ brkga_status Fleet::build_items(hipStream_t s)
     if (st != BRKGA_OK) return st;
     if (!items.empty())
         BRKGA_HIP_TRY(hipMemcpyAsync(d_items_, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, s));
-    // the host vectors die here: wait for the copies
-    BRKGA_HIP_TRY(hipStreamSynchronize(s));
     return BRKGA_OK;
 }
 

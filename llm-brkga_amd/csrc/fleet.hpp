@@ -50,7 +50,8 @@ public:
     Fleet& operator=(const Fleet&) = delete;
 
     // Replaces the member list (populations are borrowed).  Uploads the
-    // descriptors and work items on stream s and waits for the upload.
+    // descriptors and work items on stream s; the host copies are reused
+    // only by the next call, which the caller orders after s has drained.
     brkga_status set_members(const std::vector<brkga_pop_s*>& pops, hipStream_t s);
     // Sets every member's step counter to `gen` (-1: the next tick builds
     // generation 0).
@@ -83,6 +84,7 @@ private:
     brkga_engine_s* eng_;
     std::vector<brkga_pop_s*> members_;  // BRKGA members (bucket order), then NSGA-II groups
     std::vector<InstDesc> descs_;
+    std::vector<int4> items_;  // host copy; kept until the next upload has been ordered after this one
     std::vector<FleetBucket> buckets_;
     std::vector<FleetGroup> groups_;
     int n_brkga_ = 0, total_islands_ = 0, max_Pi_ = 0;

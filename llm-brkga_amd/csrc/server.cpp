@@ -4,13 +4,17 @@
 //   1. admits pending requests first-come first-served while each one's
 //      population (one contiguous slab of the HBM arena) fits the budget and
 //      the resident cap; admitted populations start at step -1, so the next
-//      tick builds their generation 0,
-//   2. runs `ticks_per_round` ticks of every resident request on its own
+//      tick builds their generation 0; requests that reached their deadline
+//      in the previous round leave the fleet,
+//   2. enqueues `ticks_per_round` ticks of every resident request on its
 //      stream (members at their deadline are skipped by the kernels),
-//   3. retires the requests that reached their deadline: best individual to
-//      host, slab back to the arena.
-// Arena slabs are never returned to the driver while the server runs, so no
-// round waits on a device-wide free.
+//   3. while those run, collects the results of the requests that left:
+//      best individual to host on a second stream, slab back to the arena,
+//   4. waits for the ticks and moves the requests now at their deadline to
+//      the leaving set.
+// Arena slabs are never returned to the driver while the server runs, and
+// problems handed over with a request are destroyed only after the round's
+// ticks are done, so no round waits on a device-wide free.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -23,6 +27,7 @@
 
 #include "capi_internal.hpp"
 #include "fleet.hpp"
+#include "launch.hpp"
 
 using namespace brkga;
 
@@ -141,7 +146,10 @@ struct brkga_server_s {
     brkga_server_config cfg{};
     Arena arena;
     std::unique_ptr<Fleet> fleet;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;     // ticks and population set-up
+    hipStream_t collect_stream = nullptr;  // results of leaving requests
+    char* staging = nullptr;               // pinned host staging of results
+    size_t staging_bytes = 0;
     std::chrono::steady_clock::time_point t0;
     std::thread worker;
 
@@ -158,13 +166,17 @@ struct brkga_server_s {
 
     // worker-only state
     std::vector<std::unique_ptr<Request>> resident;
+    std::vector<std::unique_ptr<Request>> leaving;  // at their deadline, results not collected yet
+    std::vector<brkga_problem_s*> doomed;           // owned problems to destroy after the round
 
     void run();
     bool admit();
-    brkga_status retire_done();
+    void collect_leaving();
     void finish(std::unique_ptr<Request> r, brkga_status st);
 };
 
+// Completes a request: releases its population and slab; an owned problem
+// is queued for destruction after the round.
 void brkga_server_s::finish(std::unique_ptr<Request> r, brkga_status st)
 {
     r->info.status = st;
@@ -178,7 +190,7 @@ void brkga_server_s::finish(std::unique_ptr<Request> r, brkga_status st)
         r->slab = nullptr;
     }
     if (r->own_problem && r->problem) {
-        (void)brkga_problem_destroy(r->problem);
+        doomed.push_back(r->problem);
         r->problem = nullptr;
     }
     std::lock_guard<std::mutex> lock(mu);
@@ -227,30 +239,82 @@ bool brkga_server_s::admit()
     return changed;
 }
 
-brkga_status brkga_server_s::retire_done()
+// Results of the requests that left the fleet this round: their buffers
+// are final (the previous round's ticks were waited for), so they are read
+// on the collect stream while the current ticks run.  All copies go through
+// one pinned staging area: two synchronisations per round, whatever the
+// number of requests.
+void brkga_server_s::collect_leaving()
 {
-    std::vector<std::unique_ptr<Request>> keep;
-    bool changed = false;
-    for (auto& r : resident) {
-        // tick 1 builds generation 0, tick g + 1 completes step g
-        if (r->ticks < r->generations + 1) {
-            keep.push_back(std::move(r));
-            continue;
+    if (leaving.empty()) return;
+    // layout per request: fitness [P][M], order or rank [P], crowding [P]
+    std::vector<size_t> off(leaving.size() + 1, 0);
+    for (size_t i = 0; i < leaving.size(); ++i) {
+        const InstDesc& d = leaving[i]->pop->host;
+        off[i + 1] = off[i] + (size_t)d.pop * (d.prob.num_obj + 2) * 4 + (size_t)d.L * 4 + 16;
+        off[i + 1] = (off[i + 1] + 15) & ~(size_t)15;
+    }
+    brkga_status st = BRKGA_OK;
+    if (off.back() > staging_bytes) {
+        if (staging) (void)hipHostFree(staging);
+        staging = nullptr;
+        staging_bytes = 0;
+        hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&staging), off.back(), hipHostMallocDefault);
+        if (e != hipSuccess) st = hip_error(e, "hipHostMalloc(result staging)");
+        else staging_bytes = off.back();
+    }
+    // a request that reached its deadline holds generation `generations`
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < leaving.size() && st == BRKGA_OK && e == hipSuccess; ++i) {
+        const brkga_pop_s* p = leaving[i]->pop;
+        const InstDesc& d = p->host;
+        const int b = leaving[i]->generations & 1;
+        const size_t P = (size_t)d.pop, M = (size_t)d.prob.num_obj;
+        char* base = staging + off[i];
+        e = hipMemcpyAsync(base, d.fit[b], P * M * 4, hipMemcpyDeviceToHost, collect_stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(base + P * M * 4, p->nsga2 ? (const void*)d.rank[b] : (const void*)d.perm[b], P * 4,
+                               hipMemcpyDeviceToHost, collect_stream);
+        if (e == hipSuccess && p->nsga2)
+            e = hipMemcpyAsync(base + P * (M + 1) * 4, d.crowd[b], P * 4, hipMemcpyDeviceToHost, collect_stream);
+    }
+    if (e == hipSuccess && st == BRKGA_OK) e = hipStreamSynchronize(collect_stream);
+    std::vector<int> best(leaving.size(), -1);
+    for (size_t i = 0; i < leaving.size() && e == hipSuccess && st == BRKGA_OK; ++i) {
+        const brkga_pop_s* p = leaving[i]->pop;
+        const InstDesc& d = p->host;
+        const int b = leaving[i]->generations & 1;
+        const size_t P = (size_t)d.pop, M = (size_t)d.prob.num_obj;
+        char* base = staging + off[i];
+        const float* fit = reinterpret_cast<const float*>(base);
+        best[i] = pick_best(d, p->nsga2, fit, reinterpret_cast<const int*>(base + P * M * 4),
+                            reinterpret_cast<const float*>(base + P * (M + 1) * 4));
+        if (best[i] < 0) continue;
+        leaving[i]->fitness.assign(fit + (size_t)best[i] * M, fit + (size_t)best[i] * M + M);
+        // decode the best row into the population's scratch row, then stage it
+        e = launch_evaluate(d.prob, d.keys[b] + (size_t)best[i] * d.ld, d.ld, 1, p->d_fit_row, p->d_decode_row,
+                            p->block, collect_stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(base + P * (M + 2) * 4, p->d_decode_row, (size_t)d.L * 4, hipMemcpyDeviceToHost,
+                               collect_stream);
+    }
+    if (e == hipSuccess && st == BRKGA_OK) e = hipStreamSynchronize(collect_stream);
+    if (e != hipSuccess && st == BRKGA_OK) st = hip_error(e, "collecting results");
+    for (size_t i = 0; i < leaving.size(); ++i) {
+        auto& r = leaving[i];
+        brkga_status rs = st;
+        if (rs == BRKGA_OK && best[i] < 0) rs = set_error(BRKGA_ERR_DEVICE, "no best individual found");
+        if (rs == BRKGA_OK) {
+            const InstDesc& d = r->pop->host;
+            const int32_t* dec =
+                reinterpret_cast<const int32_t*>(staging + off[i] + (size_t)d.pop * (d.prob.num_obj + 2) * 4);
+            r->decoded.assign(dec, dec + d.L);
         }
-        const int M = r->problem->desc.num_obj, L = r->problem->desc.chrom_len;
-        r->fitness.assign(M, 0.0f);
-        r->decoded.assign(L, 0);
-        brkga_status st = brkga_get_best(r->pop, r->fitness.data(), nullptr, r->decoded.data(), stream);
         r->info.generations = r->generations;
         r->info.device_bytes = r->bytes;
-        finish(std::move(r), st);
-        changed = true;
+        finish(std::move(r), rs);
     }
-    resident.swap(keep);
-    if (!changed) return BRKGA_OK;
-    std::vector<brkga_pop_s*> pops;
-    for (auto& r : resident) pops.push_back(r->pop);
-    return fleet->set_members(pops, stream);
+    leaving.clear();
 }
 
 void brkga_server_s::run()
@@ -259,41 +323,71 @@ void brkga_server_s::run()
     for (;;) {
         {
             std::unique_lock<std::mutex> lock(mu);
-            cv_work.wait(lock, [&] { return stopping || !pending.empty() || !resident.empty(); });
-            if (stopping && pending.empty() && resident.empty()) break;
+            cv_work.wait(lock, [&] { return stopping || !pending.empty() || !resident.empty() || !leaving.empty(); });
+            if (stopping && pending.empty() && resident.empty() && leaving.empty()) break;
             ++stats.rounds;
         }
+        const double h0 = now_us(t0);
+        double waited = 0.0;
         brkga_status st = BRKGA_OK;
-        if (admit()) {
+        const bool admitted = admit();
+        if (admitted || !leaving.empty()) {
             std::vector<brkga_pop_s*> pops;
             for (auto& r : resident) pops.push_back(r->pop);
             st = fleet->set_members(pops, stream);
             // step launches alone leave every population as it is
-            if (st == BRKGA_OK && cfg.autotune_reps > 0) st = fleet->autotune_new(cfg.autotune_reps, stream);
+            if (st == BRKGA_OK && admitted && cfg.autotune_reps > 0)
+                st = fleet->autotune_new(cfg.autotune_reps, stream);
         }
-        if (st == BRKGA_OK && !resident.empty()) {
-            for (int k = 0; k < cfg.ticks_per_round && st == BRKGA_OK; ++k) st = fleet->tick(stream);
-            if (st == BRKGA_OK) {
-                hipError_t e = hipStreamSynchronize(stream);
-                if (e != hipSuccess) st = hip_error(e, "server tick");
-            }
-            if (st == BRKGA_OK) {
-                for (auto& r : resident) r->ticks += cfg.ticks_per_round;
+        const bool ticking = st == BRKGA_OK && !resident.empty();
+        for (int k = 0; ticking && k < cfg.ticks_per_round && st == BRKGA_OK; ++k) st = fleet->tick(stream);
+        collect_leaving();
+        // problems of the requests that left are referenced by no queued
+        // launch; hipFree waits for the device, i.e. for this round's ticks
+        for (auto* p : doomed) (void)brkga_problem_destroy(p);
+        doomed.clear();
+        if (ticking && st == BRKGA_OK) {
+            const double w0 = now_us(t0);
+            hipError_t e = hipStreamSynchronize(stream);
+            waited = now_us(t0) - w0;
+            if (e != hipSuccess) st = hip_error(e, "server tick");
+        }
+        if (ticking && st == BRKGA_OK) {
+            {
                 std::lock_guard<std::mutex> lock(mu);
                 stats.ticks += cfg.ticks_per_round;
             }
+            // tick 1 builds generation 0, tick g + 1 completes step g
+            std::vector<std::unique_ptr<Request>> keep;
+            for (auto& r : resident) {
+                r->ticks += cfg.ticks_per_round;
+                if (r->ticks >= r->generations + 1) leaving.push_back(std::move(r));
+                else keep.push_back(std::move(r));
+            }
+            resident.swap(keep);
         }
-        if (st == BRKGA_OK) st = retire_done();
+        for (auto* p : doomed) (void)brkga_problem_destroy(p);
+        doomed.clear();
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            stats.wait_us += waited;
+            stats.host_us += now_us(t0) - h0 - waited;
+        }
         if (st != BRKGA_OK) {
             // a device failure ends every resident request with that status
             {
                 std::lock_guard<std::mutex> lock(mu);
                 worker_error = brkga_last_error();
             }
+            (void)hipStreamSynchronize(stream);
             std::vector<std::unique_ptr<Request>> rs;
             rs.swap(resident);
+            for (auto& r : leaving) rs.push_back(std::move(r));
+            leaving.clear();
             for (auto& r : rs) finish(std::move(r), st);
             (void)fleet->set_members({}, stream);
+            for (auto* p : doomed) (void)brkga_problem_destroy(p);
+            doomed.clear();
         }
     }
 }
@@ -314,6 +408,7 @@ brkga_status brkga_server_start(brkga_engine_t engine, const brkga_server_config
     brkga_status st = s->arena.reserve(s->cfg.hbm_budget_bytes);
     if (st != BRKGA_OK) return st;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->collect_stream, hipStreamNonBlocking);
     if (e != hipSuccess) return hip_error(e, "hipStreamCreate");
     s->fleet.reset(new Fleet(engine));
     s->stats.budget_bytes = s->cfg.hbm_budget_bytes;
@@ -412,6 +507,8 @@ brkga_status brkga_server_stop(brkga_server_t server)
         (void)hipStreamSynchronize(server->stream);
         server->fleet.reset();
         (void)hipStreamDestroy(server->stream);
+        (void)hipStreamDestroy(server->collect_stream);
+        if (server->staging) (void)hipHostFree(server->staging);
     }
     delete server;
     return BRKGA_OK;
