@@ -6,12 +6,15 @@
 // coordinate sets instead (objective m = tour length in set m).
 #pragma once
 
+#include <cmath>
 #include <memory>
+#include <random>
 #include <string>
 #include <utility>
 #include <vector>
 
 #include "../core/config.hpp"
+#include "tsp_config.hpp"
 
 template <typename T>
 class MultiTSPConfig : public BRKGAConfig<T> {
@@ -76,6 +79,52 @@ public:
             config->max_generations = 1000;
         }
         config->update_cuda_grid_size();
+    }
+
+    // configs/multi_tsp_config.hpp:190-202: TSPLIB coordinates (city 0 is
+    // the depot) -> distance matrix; the time matrix scales every off-
+    // diagonal distance by a speed factor drawn from std::mt19937(42) and
+    // std::uniform_real_distribution<T>(0.7, 1.8) in row-major order.
+    static std::unique_ptr<MultiTSPConfig<T>> load_from_file(const std::string& filename)
+    {
+        const auto dist = load_tsplib(filename).first;
+        return std::make_unique<MultiTSPConfig<T>>(dist, generate_time_matrix(dist, T(0.7), T(1.8)),
+                                                   FileUtils::get_basename(filename));
+    }
+
+    // configs/multi_tsp_config.hpp:204-269: FP32 Euclidean matrix (zero
+    // diagonal) of the TSPLIB coordinates, and the NAME field.
+    static std::pair<std::vector<std::vector<T>>, std::string> load_tsplib(const std::string& filename)
+    {
+        std::string name;
+        std::vector<std::pair<T, T>> xy;
+        TSPConfig<T>::read_tsplib_coords(filename, &name, &xy);
+        const std::size_t n = xy.size();
+        std::vector<std::vector<T>> d(n, std::vector<T>(n, T(0)));
+        for (std::size_t i = 0; i < n; ++i)
+            for (std::size_t j = 0; j < n; ++j)
+                if (i != j) {
+                    const T dx = xy[i].first - xy[j].first;
+                    const T dy = xy[i].second - xy[j].second;
+                    const T sx = dx * dx;
+                    const T sy = dy * dy;
+                    d[i][j] = std::sqrt(sx + sy);
+                }
+        return {d, name};
+    }
+
+    // configs/multi_tsp_config.hpp:165-188
+    static std::vector<std::vector<T>> generate_time_matrix(const std::vector<std::vector<T>>& dist,
+                                                            T speed_min = T(0.8), T speed_max = T(1.5))
+    {
+        const std::size_t n = dist.size();
+        std::vector<std::vector<T>> t(n, std::vector<T>(n, T(0)));
+        std::mt19937 rng(42);
+        std::uniform_real_distribution<T> speed(speed_min, speed_max);
+        for (std::size_t i = 0; i < n; ++i)
+            for (std::size_t j = 0; j < n; ++j)
+                if (i != j) t[i][j] = dist[i][j] * speed(rng);
+        return t;
     }
 
     int get_num_cities() const { return num_cities_; }

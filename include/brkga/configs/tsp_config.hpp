@@ -8,12 +8,15 @@
 // the reference's formula, configs/tsp_config.hpp:366-368, no matrix).
 #pragma once
 
+#include <cctype>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <utility>
 #include <vector>
 
 #include "../core/config.hpp"
+#include "../utils/file_utils.hpp"
 
 template <typename T>
 class TSPConfig : public BRKGAConfig<T> {
@@ -83,9 +86,74 @@ public:
         config->update_cuda_grid_size();
     }
 
+    // configs/tsp_config.hpp:260-265
+    static std::unique_ptr<TSPConfig<T>> load_from_file(const std::string& filename)
+    {
+        if (!FileUtils::file_exists(filename)) throw std::runtime_error("File does not exist: " + filename);
+        return load_from_tsplib(filename);
+    }
+
+    // TSPLIB NODE_COORD_SECTION instance (configs/tsp_config.hpp:301-380):
+    // NAME, DIMENSION, then "id x y" lines with 1-based ids until EOF.  The
+    // reference turns the coordinates into an n x n matrix with the edge
+    // formula the engine evaluates from coordinates directly, so the
+    // instance comes back as a coordinate instance (no matrix).
+    static std::unique_ptr<TSPConfig<T>> load_from_tsplib(const std::string& filename)
+    {
+        std::string name;
+        std::vector<std::pair<T, T>> xy;
+        read_tsplib_coords(filename, &name, &xy);
+        if (xy.empty()) throw std::runtime_error("No coordinates found in TSPLIB file");
+        if (name.empty()) name = FileUtils::get_basename(filename);
+        return from_coordinates(xy, name);
+    }
+
+    // TSPLIB header keyword of a line ("NAME : x" -> "NAME"), or "".
+    static std::string tsplib_key(const std::string& line, std::string* value)
+    {
+        const std::size_t a = line.find_first_not_of(" \t");
+        if (a == std::string::npos) return std::string();
+        std::size_t b = a;
+        while (b < line.size() && (std::isalnum((unsigned char)line[b]) || line[b] == '_')) ++b;
+        const std::string key = line.substr(a, b - a);
+        if (value) {
+            std::size_t c = line.find(':', b);
+            c = c == std::string::npos ? b : c + 1;
+            const std::size_t v0 = line.find_first_not_of(" \t", c);
+            const std::size_t v1 = line.find_last_not_of(" \t");
+            *value = v0 == std::string::npos ? std::string() : line.substr(v0, v1 - v0 + 1);
+        }
+        return key;
+    }
+
+    // Coordinates of a TSPLIB file, index = id - 1 (ids outside 1..DIMENSION
+    // are ignored, missing ids stay at the origin).
+    static void read_tsplib_coords(const std::string& filename, std::string* name, std::vector<std::pair<T, T>>* xy)
+    {
+        bool coords = false;
+        for (const std::string& line : FileUtils::read_lines(filename)) {
+            std::string value;
+            const std::string key = tsplib_key(line, &value);
+            if (key.empty()) continue;
+            if (key == "EOF") break;
+            if (!coords) {
+                if (key == "NAME") *name = value;
+                else if (key == "DIMENSION") xy->assign(std::stoi(value), std::pair<T, T>(T(0), T(0)));
+                else if (key == "NODE_COORD_SECTION") coords = true;
+                continue;
+            }
+            std::istringstream in(line);
+            int id;
+            T x, y;
+            if (in >> id >> x >> y && id > 0 && id <= (int)xy->size()) (*xy)[id - 1] = {x, y};
+        }
+    }
+
     int get_num_cities() const { return num_cities_; }
     const std::string& get_instance_name() const { return instance_name_; }
     bool is_gpu_available() const { return true; }
+    // Coordinates of a coordinate instance (empty for a matrix instance).
+    const std::vector<std::pair<T, T>>& get_coordinates() const { return coords_; }
 
     void print_instance_info() const
     {

@@ -8,12 +8,15 @@
 // engine: has_local_search() is false.
 #pragma once
 
+#include <fstream>
+#include <iomanip>
 #include <memory>
 #include <string>
 #include <utility>
 #include <vector>
 
 #include "../core/config.hpp"
+#include "../utils/file_utils.hpp"
 
 template <typename T>
 class TSPJConfig : public BRKGAConfig<T> {
@@ -85,8 +88,81 @@ public:
         config->update_cuda_grid_size();
     }
 
+    // configs/tspj_config.hpp:727-769: a pair of CSV matrices named
+    // <name>_TT.csv / <name>_JT.csv or <name>_cost_table_by_coordinates.csv /
+    // <name>_tasktime_table.csv; either file of the pair may be given.
+    static std::unique_ptr<TSPJConfig<T>> load_from_file(const std::string& filename)
+    {
+        const std::string base = FileUtils::get_basename(filename);
+        const std::string dir = FileUtils::get_directory(filename);
+        std::string tt, jt, name;
+        std::size_t at;
+        if ((at = base.find("_TT")) != std::string::npos) {
+            name = base.substr(0, at);
+            tt = filename;
+            jt = dir + "/" + name + "_JT.csv";
+        } else if ((at = base.find("_JT")) != std::string::npos) {
+            name = base.substr(0, at);
+            jt = filename;
+            tt = dir + "/" + name + "_TT.csv";
+        } else if ((at = base.find("_cost_table_by_coordinates")) != std::string::npos) {
+            name = base.substr(0, at);
+            tt = filename;
+            jt = dir + "/" + name + "_tasktime_table.csv";
+        } else if ((at = base.find("_tasktime_table")) != std::string::npos) {
+            name = base.substr(0, at);
+            jt = filename;
+            tt = dir + "/" + name + "_cost_table_by_coordinates.csv";
+        } else {
+            throw std::runtime_error("TSPJ CSV files must be named with '_TT'/'_JT' or "
+                                     "'_cost_table_by_coordinates'/'_tasktime_table' suffix");
+        }
+        if (!FileUtils::file_exists(tt)) throw std::runtime_error("Travel times file not found: " + tt);
+        if (!FileUtils::file_exists(jt)) throw std::runtime_error("Job times file not found: " + jt);
+        return std::make_unique<TSPJConfig<T>>(load_csv_matrix(tt), load_csv_matrix(jt), name);
+    }
+
+    // configs/tspj_config.hpp:771-802: comma-separated rows ('#' comments);
+    // empty cells are skipped, unparsable ones read as 0.
+    static std::vector<std::vector<T>> load_csv_matrix(const std::string& filename)
+    {
+        std::vector<std::vector<T>> m;
+        for (const std::string& line : FileUtils::read_lines(filename)) {
+            if (line.empty() || line[0] == '#') continue;
+            std::vector<T> row;
+            for (const std::string& cell : FileUtils::parse_csv_line(line)) {
+                if (cell.empty()) continue;
+                try {
+                    row.push_back(static_cast<T>(std::stod(cell)));
+                } catch (...) {
+                    row.push_back(T(0));
+                }
+            }
+            if (!row.empty()) m.push_back(row);
+        }
+        return m;
+    }
+
+    // configs/tspj_config.hpp:695-725
+    void export_solution(const Individual<T>& individual, const std::string& filename) override
+    {
+        const auto s = decode_solution_internal(individual);
+        FileUtils::ensure_directory(FileUtils::get_directory(filename));
+        std::ofstream f(filename);
+        if (!f) throw std::runtime_error("Cannot create solution file: " + filename);
+        f << "TSPJ Solution for " << instance_name_ << std::endl;
+        f << "Makespan: " << std::fixed << std::setprecision(6) << individual.fitness << std::endl;
+        f << "City sequence: ";
+        for (std::size_t i = 0; i < s.first.size(); ++i) f << s.first[i] << (i + 1 < s.first.size() ? " " : "");
+        f << std::endl << "Job sequence: ";
+        for (std::size_t i = 0; i < s.second.size(); ++i) f << s.second[i] << (i + 1 < s.second.size() ? " " : "");
+        f << std::endl;
+    }
+
     int get_num_cities() const { return num_cities_; }
     int get_num_jobs() const { return num_jobs_; }
+    const std::vector<std::vector<T>>& get_travel_times() const { return travel_; }
+    const std::vector<std::vector<T>>& get_job_durations() const { return jobs_; }
     const std::string& get_instance_name() const { return instance_name_; }
 
     void print_instance_info() const

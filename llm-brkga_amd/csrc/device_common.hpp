@@ -381,15 +381,21 @@ __device__ void block_argsort(const float* keys, int n, int nb, const SortScratc
             s.items[s.cnt[bkt[i]] + slot[i]] = ((unsigned long long)key_order(keys[e]) << 32) | (unsigned)e;
     }
     __syncthreads();
+    // the first four items of every bucket are read as independent loads
+    // (buckets rarely hold more); longer buckets finish in a loop
 #pragma unroll
     for (int i = 0; i < kSortItems; ++i) {
         const int e = tid + i * BLOCK;
         if (e < n) {
             const int st = s.cnt[bkt[i]];
-            const int en = s.cnt[bkt[i] + 1];
+            const int sz = s.cnt[bkt[i] + 1] - st;
             const unsigned long long mine = s.items[st + slot[i]];
-            int r = 0;
-            for (int q = st; q < en; ++q) r += s.items[q] < mine ? 1 : 0;
+            const unsigned long long x1 = sz > 1 ? s.items[st + 1] : ~0ull;
+            const unsigned long long x2 = sz > 2 ? s.items[st + 2] : ~0ull;
+            const unsigned long long x3 = sz > 3 ? s.items[st + 3] : ~0ull;
+            const unsigned long long x0 = s.items[st];
+            int r = (x0 < mine) + (x1 < mine) + (x2 < mine) + (x3 < mine);
+            for (int q = st + 4; q < st + sz; ++q) r += s.items[q] < mine ? 1 : 0;
             perm[st + r] = e;
         }
     }

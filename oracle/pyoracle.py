@@ -309,5 +309,9 @@ def ref_lib():
         L.ref_knapsack_threshold.argtypes = [_PF, _PF, ctypes.c_int, ctypes.c_float, _PF, ctypes.c_int, _PF, _PI]
         L.ref_multitsp.argtypes = [_PF, _PF, ctypes.c_int, _PF, ctypes.c_int, _PF, _PI]
         L.ref_dominates.argtypes = [_PF, ctypes.c_int, ctypes.c_int, _PI]
+        L.ref_knapsack_load.argtypes = [ctypes.c_char_p, _PF, _PF, ctypes.c_int, _PF]
+        L.ref_knapsack_load.restype = ctypes.c_int
+        L.ref_multitsp_load_objectives.argtypes = [ctypes.c_char_p, _PF, ctypes.c_int, _PF]
+        L.ref_multitsp_load_objectives.restype = ctypes.c_int
         _ref = L
     return _ref

@@ -8,6 +8,8 @@
 //   KnapsackConfig<float>::decode_to_solution          configs/knapsack_config.hpp:64-75
 //   MultiTSPConfig<float> objectives + decode_tour     configs/multi_tsp_config.hpp:54-103
 //   Individual<float>::dominates                       core/individual.hpp:99-114
+//   KnapsackConfig<float>::load_from_file              configs/knapsack_config.hpp:180-304
+//   MultiTSPConfig<float>::load_from_file              configs/multi_tsp_config.hpp:165-269
 // Nothing here is part of the engine.
 #include "configs/knapsack_config.hpp"
 #include "configs/multi_tsp_config.hpp"
@@ -77,6 +79,36 @@ void ref_dominates(const float* obj, int count, int nobj, int32_t* out)
     }
     for (int i = 0; i < count; ++i)
         for (int j = 0; j < count; ++j) out[(size_t)i * count + j] = pop[i].dominates(pop[j]) ? 1 : 0;
+}
+
+// Items of a knapsack file as the reference loads it; returns the item
+// count (at most max_items are copied).
+int ref_knapsack_load(const char* path, float* w, float* v, int max_items, float* cap)
+{
+    auto cfg = KnapsackConfig<float>::load_from_file(path);
+    const auto& ww = cfg->get_weights();
+    const auto& vv = cfg->get_values();
+    for (int i = 0; i < (int)ww.size() && i < max_items; ++i) {
+        w[i] = ww[i];
+        v[i] = vv[i];
+    }
+    *cap = cfg->get_capacity();
+    return (int)ww.size();
+}
+
+// Objectives [rows][2] of key rows [rows][n] under the bi-objective TSP the
+// reference builds from a TSPLIB file (distance matrix and the generated
+// time matrix); returns n (cities after the depot).
+int ref_multitsp_load_objectives(const char* path, const float* keys, int rows, float* obj)
+{
+    auto cfg = MultiTSPConfig<float>::load_from_file(path);
+    const int n = cfg->get_num_cities();
+    for (int r = 0; r < rows; ++r) {
+        auto ind = make_individual(keys + (size_t)r * n, n);
+        obj[2 * r] = cfg->objective_functions[0](ind);
+        obj[2 * r + 1] = cfg->objective_functions[1](ind);
+    }
+    return n;
 }
 
 }  // extern "C"

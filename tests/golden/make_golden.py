@@ -83,6 +83,32 @@ def dominance(ref, out):
     np.savez(out, objectives=obj, dominates=res)
 
 
+def loaders(ref, out):
+    """The reference's own knapsack and bi-objective TSP file loaders on the
+    fixture files in data/ (the TSP and TSPJ loaders need CUDA headers)."""
+    data = os.path.join(HERE, "data")
+    res = {}
+    for name in ("knap_small.txt", "knap_cap.csv", "knap_nocap.csv"):
+        w = np.zeros(64, np.float32)
+        v = np.zeros(64, np.float32)
+        cap = np.zeros(1, np.float32)
+        cnt = ref.ref_knapsack_load(os.path.join(data, name).encode(), _f(w), _f(v), 64, _f(cap))
+        key = name.replace(".", "_")
+        res[key + "_w"] = w[:cnt]
+        res[key + "_v"] = v[:cnt]
+        res[key + "_cap"] = cap
+    g = np.random.default_rng(20240104)
+    n = 7  # tiny8.tsp: depot + 7 cities
+    keys = (g.integers(0, 1 << 24, (24, n)) * 2.0 ** -24).astype(np.float32)
+    keys[0] = np.arange(n, dtype=np.float32) / n  # identity tour
+    obj = np.zeros((24, 2), np.float32)
+    got = ref.ref_multitsp_load_objectives(os.path.join(data, "tiny8.tsp").encode(), _f(keys), 24, _f(obj))
+    assert got == n
+    res["tiny8_keys"] = keys
+    res["tiny8_objectives"] = obj
+    np.savez(out, **res)
+
+
 def _euclid(xy):
     dx = (xy[:, None, 0] - xy[None, :, 0]).astype(np.float32)
     dy = (xy[:, None, 1] - xy[None, :, 1]).astype(np.float32)
@@ -100,6 +126,7 @@ def main():
     knapsack_threshold(ref, os.path.join(HERE, "knapsack_threshold_ref.npz"))
     multitsp(ref, os.path.join(HERE, "multitsp_ref.npz"))
     dominance(ref, os.path.join(HERE, "dominance_ref.npz"))
+    loaders(ref, os.path.join(HERE, "loaders_ref.npz"))
     print("fixtures written to", HERE)
 
 
