@@ -4,8 +4,9 @@
 // TSPJConfig (configs/tspj_config.hpp:27-870): travel[dim][dim] and
 // jobs[dim][dim] with row / column 0 the depot, chromosome = n city keys |
 // n job keys, makespan minimised (configs/tspj_config.hpp:555-642).  The
-// TSPJ local search (configs/tspj_config.hpp:77-135) is not part of the
-// engine: has_local_search() is false.
+// local search (configs/tspj_config.hpp:77-135: random 2-opt moves on the
+// elites, kept when the makespan improves) runs on the device inside the
+// generation loop; the solver's options set its schedule.
 #pragma once
 
 #include <fstream>
@@ -172,6 +173,7 @@ public:
     }
 
     int native_kind() const override { return BRKGA_TSPJ; }
+    bool has_local_search() const override { return true; }
 
 protected:
     brkga_problem_t create_native_problem(brkga_engine_t engine) override

@@ -25,6 +25,12 @@ struct BRKGAEngineOptions {
     int num_migrants = 0;        // solver.hpp:164
     uint64_t seed = 0x5EED;      // the reference seeds from the clock
     bool use_graph = true;       // replay generation loops from a HIP graph
+    // local search of configs with has_local_search() (TSPJ), with the
+    // reference's defaults (solver.hpp:166)
+    bool local_search = true;
+    int ls_interval = 2;
+    int ls_individuals = 100;
+    int ls_moves = 5;
 };
 
 template <typename T>
@@ -45,6 +51,11 @@ public:
         p.migration_interval = opt.migration_interval;
         p.num_migrants = opt.num_migrants;
         p.seed = opt.seed;
+        if (opt.local_search && cfg->has_local_search()) {
+            p.ls_interval = opt.ls_interval;
+            p.ls_individuals = opt.ls_individuals;
+            p.ls_moves = opt.ls_moves;
+        }
         brkga_facade::check(brkga_population_create(problem_, &p, &pop_), "brkga_population_create");
     }
 

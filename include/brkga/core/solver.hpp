@@ -40,6 +40,14 @@ public:
         options_.num_migrants = migrants;
     }
     void set_use_graph(bool on) { options_.use_graph = on; }
+    // local search of configs that have one (solver.hpp:74-77,166)
+    void set_local_search(bool on, int interval = 2, int individuals = 100, int moves = 5)
+    {
+        options_.local_search = on;
+        options_.ls_interval = interval;
+        options_.ls_individuals = individuals;
+        options_.ls_moves = moves;
+    }
 
     void initialize()
     {

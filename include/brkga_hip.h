@@ -77,6 +77,15 @@ typedef struct {
     int32_t migration_interval; /* generations between ring migrations (0 = never) */
     int32_t num_migrants;       /* top-k rows sent to the next island, k <= elite per island */
     uint64_t seed;              /* Philox key */
+    /* TSPJ local search (tspj_config.hpp:77-135, solver.hpp:458-462,562-647):
+     * after every ls_interval-th generation, ls_moves random 2-opt moves on
+     * the decoded city tour of the first ls_individuals (at most the elite)
+     * of every island, kept when the makespan strictly improves.  0 = off
+     * (the reference runs it every 2 generations on up to 100 elites with 5
+     * moves).  TSPJ problems only. */
+    int32_t ls_interval;
+    int32_t ls_individuals;
+    int32_t ls_moves;
 } brkga_params;
 
 /* ------------------------------------------------------------ engine ---- */

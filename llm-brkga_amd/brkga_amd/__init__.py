@@ -59,6 +59,9 @@ class CParams(ctypes.Structure):
         ("migration_interval", ctypes.c_int32),
         ("num_migrants", ctypes.c_int32),
         ("seed", ctypes.c_uint64),
+        ("ls_interval", ctypes.c_int32),
+        ("ls_individuals", ctypes.c_int32),
+        ("ls_moves", ctypes.c_int32),
     ]
 
 
@@ -75,11 +78,14 @@ class Params:
     migration_interval: int = 0
     num_migrants: int = 0
     seed: int = 0
+    ls_interval: int = 0      # TSPJ local search (tspj_config.hpp:77-135); 0 = off
+    ls_individuals: int = 0
+    ls_moves: int = 0
 
     def to_c(self) -> CParams:
         return CParams(self.population_size, self.elite_size, self.mutant_size, float(self.elite_prob),
                        self.num_islands, self.migration_interval, self.num_migrants,
-                       self.seed & 0xFFFFFFFFFFFFFFFF)
+                       self.seed & 0xFFFFFFFFFFFFFFFF, self.ls_interval, self.ls_individuals, self.ls_moves)
 
 
 class CServerConfig(ctypes.Structure):

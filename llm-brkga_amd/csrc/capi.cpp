@@ -199,6 +199,15 @@ brkga_status fill_inst_desc(const brkga_problem_s* pr, const brkga_params* q, In
         return set_error(BRKGA_ERR_INVALID_ARG, "migration settings must be >= 0");
     if (I > 1 && q->num_migrants > Ei)
         return set_error(BRKGA_ERR_INVALID_ARG, "num_migrants must not exceed the elite per island");
+    if (q->ls_interval < 0 || q->ls_individuals < 0 || q->ls_moves < 0)
+        return set_error(BRKGA_ERR_INVALID_ARG, "local search settings must be >= 0");
+    if (q->ls_interval > 0 && q->ls_individuals > 0 && p.kind != KIND_TSPJ)
+        return set_error(BRKGA_ERR_INVALID_ARG, "the local search is defined for TSPJ problems only");
+    if (q->ls_interval > 0 && q->ls_individuals > 0) {
+        d->ls_interval = q->ls_interval;
+        d->ls_k = std::min(q->ls_individuals, Ei);
+        d->ls_moves = q->ls_moves;
+    }
     d->islands = I;
     d->Pi = Pi;
     d->Ei = Ei;
@@ -234,9 +243,10 @@ brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, DeviceAllocator* a, std:
         }
     }
     void* c;
-    if ((st = get(&c, 2 * sizeof(int), "counters")) != BRKGA_OK) return st;
+    if ((st = get(&c, 4 * sizeof(int), "counters")) != BRKGA_OK) return st;
     d->gen = (int*)c;
     d->done = (int*)c + 1;
+    d->ls_flag = (int*)c + 2;
     return BRKGA_OK;
 }
 
@@ -329,7 +339,7 @@ brkga_status create_population(brkga_problem_s* problem, const brkga_params* par
         hipError_t e = hipSuccess;
         for (int b = 0; b < 2 && e == hipSuccess; ++b)
             e = hipMemsetAsync(d.keys[b], 0, rows * d.ld * sizeof(float), s);
-        if (e == hipSuccess) e = hipMemsetAsync(d.gen, 0, 2 * sizeof(int), s);
+        if (e == hipSuccess) e = hipMemsetAsync(d.gen, 0, 4 * sizeof(int), s);
         if (e == hipSuccess && gen0 < 0) e = hipMemsetAsync(d.gen, 0xFF, sizeof(int), s);
         if (e == hipSuccess) e = hipMemcpyAsync(pop->d_desc, &pop->host, sizeof(InstDesc), hipMemcpyHostToDevice, s);
         if (e != hipSuccess) st = hip_error(e, "population upload");
@@ -401,6 +411,13 @@ brkga_status pop_step(brkga_pop_t pop, int init, int cur, hipStream_t s)
     } else {
         e = launch_step_one(d, pop->block, cur, init, s);
         if (e == hipSuccess) e = launch_partition_one(d, init ? 0 : cur ^ 1, init ? 0 : 1, s);
+        // TSPJ local search: the kernels check on the device whether the
+        // generation just built is due, so graphs replay at any generation
+        if (e == hipSuccess && !init && d.ls_interval > 0) {
+            e = launch_tspj_local_search(pop->d_desc, 1, d.islands * d.ls_k, pop->block,
+                                         tspj_ls_lds_bytes(d.prob.n, d.nb), s);
+            if (e == hipSuccess) e = launch_partition_one(d, -1, 2, s);
+        }
     }
     if (e != hipSuccess) return hip_error(e, "kernel launch");
     return BRKGA_OK;
@@ -683,7 +700,7 @@ brkga_status brkga_population_initialize(brkga_pop_t pop, void* stream)
     if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
     DeviceGuard g(pop->problem->engine->device);
     hipStream_t s = as_stream(stream);
-    BRKGA_HIP_TRY(hipMemsetAsync(pop->host.gen, 0, 2 * sizeof(int), s));
+    BRKGA_HIP_TRY(hipMemsetAsync(pop->host.gen, 0, 4 * sizeof(int), s));
     pop->host_t = 0;
     return pop_step(pop, 1, 0, s);
 }
@@ -781,7 +798,7 @@ brkga_status brkga_population_write(brkga_pop_t pop, const float* keys, void* st
     BRKGA_HIP_TRY(hipStreamSynchronize(s));
     BRKGA_HIP_TRY(hipMemcpy2D(d.keys[0], sizeof(float) * d.ld, keys, sizeof(float) * d.L, sizeof(float) * d.L,
                               (size_t)d.pop, hipMemcpyHostToDevice));
-    BRKGA_HIP_TRY(hipMemsetAsync(d.gen, 0, 2 * sizeof(int), s));
+    BRKGA_HIP_TRY(hipMemsetAsync(d.gen, 0, 4 * sizeof(int), s));
     pop->host_t = 0;
     hipError_t e = launch_evaluate(d.prob, d.keys[0], d.ld, d.pop, d.fit[0], nullptr, pop->block, s);
     if (e != hipSuccess) return hip_error(e, "evaluate launch");

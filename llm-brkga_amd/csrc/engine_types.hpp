@@ -24,7 +24,8 @@ enum RngTag : uint32_t {
     TAG_COINS = 1,
     TAG_MUTANT = 2,
     TAG_INIT = 3,
-    TAG_MUTCOIN = 4
+    TAG_MUTCOIN = 4,
+    TAG_LSEARCH = 5
 };
 
 struct ProblemDesc {
@@ -55,12 +56,14 @@ struct InstDesc {
     float* crowd[2];  // NSGA-II crowding distance per slot
     int* gen;         // completed generation steps
     int* done;        // arrival counter of the per-island partition blocks
+    int* ls_flag;     // 1 while a local search of the current generation is due
     int L, ld;
     int pop, Pi, Ei, Mi, islands;
     int mig_interval, mig_k;
     uint32_t seed_lo, seed_hi;
     float rho;
     float mut_rate;   // NSGA-II per-gene resampling probability
+    int ls_interval, ls_k, ls_moves;  // TSPJ local search (0 = off)
     int nb;           // argsort buckets for n keys
     int slot_begin;   // first block of this instance in a step launch
     int seg_begin;    // first block of this instance in a partition launch

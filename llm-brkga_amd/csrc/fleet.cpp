@@ -44,6 +44,7 @@ Fleet::~Fleet()
     }
     if (fork_) (void)hipEventDestroy(fork_);
     if (d_descs_) (void)hipFree(d_descs_);
+    if (d_ls_descs_) (void)hipFree(d_ls_descs_);
     if (d_items_) (void)hipFree(d_items_);
     if (d_queues_) (void)hipFree(d_queues_);
 }
@@ -151,6 +152,16 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
         }
         descs_[i] = d;
     }
+    ls_descs_.clear();
+    ls_items_ = ls_block_ = ls_lds_ = 0;
+    for (int i = 0; i < n_brkga_; ++i) {
+        const InstDesc& d = descs_[i];
+        if (d.ls_interval <= 0 || d.ls_k <= 0) continue;
+        ls_descs_.push_back(d);
+        ls_items_ = std::max(ls_items_, d.islands * d.ls_k);
+        ls_block_ = std::max(ls_block_, default_block(d.prob.n));
+        ls_lds_ = std::max(ls_lds_, tspj_ls_lds_bytes(d.prob.n, d.nb));
+    }
     for (auto& b : buckets_) {
         // the decoders hold up to 16 keys per thread
         while (b.block * kSortItems < b.n_max) b.block <<= 1;
@@ -168,6 +179,13 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
     if (!descs_.empty())
         BRKGA_HIP_TRY(hipMemcpyAsync(d_descs_, descs_.data(), sizeof(InstDesc) * descs_.size(),
                                      hipMemcpyHostToDevice, s));
+    if (!ls_descs_.empty()) {
+        st = reserve(reinterpret_cast<void**>(&d_ls_descs_), &ls_descs_cap_, sizeof(InstDesc) * ls_descs_.size(),
+                     "fleet local-search descriptors");
+        if (st != BRKGA_OK) return st;
+        BRKGA_HIP_TRY(hipMemcpyAsync(d_ls_descs_, ls_descs_.data(), sizeof(InstDesc) * ls_descs_.size(),
+                                     hipMemcpyHostToDevice, s));
+    }
     BRKGA_HIP_TRY(hipMemsetAsync(d_queues_, 0, queues_cap_, s));
     return build_items(s);
 }
@@ -276,6 +294,12 @@ brkga_status Fleet::tick(hipStream_t s)
         }
     if (e == hipSuccess && n_brkga_ > 0)
         e = launch_partition_many(d_descs_, n_brkga_, total_islands_, max_Pi_, -1, 1, s);
+    if (e == hipSuccess && !ls_descs_.empty()) {
+        // members whose new generation is due run their local search, then
+        // are sorted again (both kernels skip everyone else)
+        e = launch_tspj_local_search(d_ls_descs_, (int)ls_descs_.size(), ls_items_, ls_block_, ls_lds_, s);
+        if (e == hipSuccess) e = launch_partition_many(d_descs_, n_brkga_, total_islands_, max_Pi_, -1, 2, s);
+    }
     if (e == hipSuccess && !groups_.empty() && nsga_lane_ != 0) {
         e = hipEventRecord(join_[nsga_lane_], lanes_[nsga_lane_]);
         if (e == hipSuccess) e = hipStreamWaitEvent(s, join_[nsga_lane_], 0);

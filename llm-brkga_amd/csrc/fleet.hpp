@@ -92,6 +92,11 @@ private:
 
     InstDesc* d_descs_ = nullptr;
     size_t descs_cap_ = 0;
+    // members with the TSPJ local search (descriptor copies)
+    std::vector<InstDesc> ls_descs_;
+    InstDesc* d_ls_descs_ = nullptr;
+    size_t ls_descs_cap_ = 0;
+    int ls_items_ = 0, ls_block_ = 0, ls_lds_ = 0;
     int4* d_items_ = nullptr;
     size_t items_cap_ = 0;
     int* d_queues_ = nullptr;  // two counters per bucket, 128 bytes apart

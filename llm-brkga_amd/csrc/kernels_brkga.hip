@@ -123,9 +123,11 @@ constexpr int kPartBlock = 1024;
 // kPartBlock individuals: every key (fitness order key, index) is unique,
 // so the rank of a key is the number of smaller keys; G = kPartBlock / Pn
 // threads count one key's rank over interleaved quarters and combine with
-// wave shuffles.  Larger islands use a bitonic network.  With `advance`,
-// the last island block of the population bumps the generation counter.
-__device__ __forceinline__ void partition_body(const InstDesc& d, int island, int buf, int advance, int t,
+// wave shuffles.  Larger islands use a bitonic network.  mode 1: the last
+// island block of the population bumps the generation counter to t + 1 and
+// raises the local-search flag when generation t + 1 is due; mode 2: the
+// last block lowers the flag.
+__device__ __forceinline__ void partition_body(const InstDesc& d, int island, int buf, int mode, int t,
                                                char* lds)
 {
     constexpr int B = kPartBlock;
@@ -181,8 +183,8 @@ __device__ __forceinline__ void partition_body(const InstDesc& d, int island, in
         }
         for (int i = threadIdx.x; i < Pi; i += B) perm[i] = (int)(sk[i] & 0xFFFFFFFFull);
     }
-    if (advance) {
-        // The counter only has to move once every island block is done; no
+    if (mode != 0) {
+        // The counters only change once every island block is done; no
         // block reads what another one wrote (the next kernel sees the rank
         // orders through the kernel boundary), so no fence is needed here.
         __syncthreads();
@@ -190,37 +192,137 @@ __device__ __forceinline__ void partition_body(const InstDesc& d, int island, in
             const int old = atomicAdd(d.done, 1);
             if (old == d.islands - 1) {
                 *d.done = 0;
-                *d.gen = t + 1;
+                if (mode == 1) {
+                    const int g = t + 1;
+                    *d.ls_flag = d.ls_interval > 0 && g > 0 && g % d.ls_interval == 0 ? 1 : 0;
+                    *d.gen = g;
+                } else {
+                    *d.ls_flag = 0;
+                }
             }
         }
     }
 }
 
-__global__ __launch_bounds__(kPartBlock) void k_partition_one(InstDesc d, int buf, int advance)
+__global__ __launch_bounds__(kPartBlock) void k_partition_one(InstDesc d, int buf, int mode)
 {
     extern __shared__ __align__(16) char lds[];
-    const int t = advance ? *d.gen : 0;
-    partition_body(d, blockIdx.x, buf, advance, t, lds);
+    int t = 0;
+    if (mode == 1) {
+        t = *d.gen;
+    } else if (mode == 2) {
+        if (*d.ls_flag == 0) return;
+        buf = *d.gen & 1;
+    }
+    partition_body(d, blockIdx.x, buf, mode, t, lds);
 }
 
 __global__ __launch_bounds__(kPartBlock) void k_partition_many(const InstDesc* __restrict__ descs, int num_inst,
-                                                               int buf, int advance)
+                                                               int buf, int mode)
 {
     extern __shared__ __align__(16) char lds[];
     const InstDesc& d = descs[find_instance_by_seg(descs, num_inst, blockIdx.x)];
     const int island = blockIdx.x - d.seg_begin;
     if (island >= d.islands) return;
     int t = 0, b = buf;
-    if (advance) {
+    if (mode == 1) {
         // t = -1: generation 0 was just built into buffer 0; the counter
         // moves to 0 like after any other step
         t = *d.gen;
         if (t >= d.gen_limit) return;
         if (b < 0) b = (t + 1) & 1;
+    } else if (mode == 2) {
+        if (*d.ls_flag == 0) return;
+        b = *d.gen & 1;
     } else if (b < 0) {
         b = 0;
     }
-    partition_body(d, island, b, advance, t, lds);
+    partition_body(d, island, b, mode, t, lds);
+}
+
+// TSPJ local search: one workgroup per (population, island, rank r < ls_k)
+// of every population whose local-search flag is raised (its current
+// generation g is due).  The individual's city tour (stable argsort of the
+// city keys) takes ls_moves random 2-opt moves: Philox words 2m, 2m+1 of
+// (island, g, r, LSEARCH) give positions x = u0 % n, y = u1 % n; when they
+// are at least two apart the tour between them is reversed.  The city keys,
+// sorted, are then handed to the new tour in order, and the candidate
+// replaces the individual only if its makespan is strictly smaller.
+// Reference: tspj_config.hpp:77-135, cuda_kernels.cuh:166-246 (which
+// reverses key values by position and so does not reverse the decoded
+// tour; the tour reversal is what is done here).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_tspj_local_search(const InstDesc* __restrict__ descs, int num_inst)
+{
+    extern __shared__ __align__(16) char lds[];
+    __shared__ float s_fit;
+    const InstDesc& d = descs[blockIdx.y];
+    if (d.prob.kind != KIND_TSPJ || d.ls_k <= 0) return;
+    const int item = blockIdx.x;
+    if (item >= d.islands * d.ls_k) return;
+    if (*d.ls_flag == 0) return;
+    const int g = *d.gen;
+    const int b = g & 1;
+    const int island = item / d.ls_k, r = item - island * d.ls_k;
+    const int n = d.prob.n, nb = d.nb;
+    const int slot = island * d.Pi + d.perm[b][island * d.Pi + r];
+    float* row = d.keys[b] + (size_t)slot * d.ld;
+    const DecodeLayout lay = decode_layout(KIND_TSPJ, n, 2 * n, nb);
+    float* keys = reinterpret_cast<float*>(lds + lay.keys_off);
+    int* tour = reinterpret_cast<int*>(lds + lay.bytes);
+    float* vals = reinterpret_cast<float*>(tour + n);
+    float* cand = vals + n;
+    SortScratch ss;
+    ss.cnt = reinterpret_cast<int*>(lds + lay.cnt_off);
+    ss.items = reinterpret_cast<unsigned long long*>(lds + lay.items_off);
+    ss.red = reinterpret_cast<long long*>(lds + lay.red_off);
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+        const float k = row[i];
+        keys[i] = k;
+        cand[i] = k;
+    }
+    __syncthreads();
+    // decoded tour into the key area (the argsort overlays it)
+    block_argsort<BLOCK>(keys, n, nb, ss, reinterpret_cast<int*>(keys));
+    const int* order = reinterpret_cast<const int*>(keys);
+    for (int p = threadIdx.x; p < n; p += BLOCK) {
+        const int c = order[p];
+        tour[p] = c;
+        vals[p] = cand[c];
+    }
+    __syncthreads();
+    for (int m = 0; m < d.ls_moves; ++m) {
+        const U4 w = rng_block(d.seed_lo, d.seed_hi, (uint32_t)island, (uint32_t)g, (uint32_t)r, TAG_LSEARCH, m >> 1);
+        int x = (int)(((m & 1) ? w.z : w.x) % (uint32_t)n);
+        int y = (int)(((m & 1) ? w.w : w.y) % (uint32_t)n);
+        if (x > y) {
+            const int tmp = x;
+            x = y;
+            y = tmp;
+        }
+        if (y - x < 2) continue;  // uniform across the block
+        const int half = (y - x + 1) >> 1;
+        for (int q = threadIdx.x; q < half; q += BLOCK) {
+            const int a = tour[x + q];
+            tour[x + q] = tour[y - q];
+            tour[y - q] = a;
+        }
+        __syncthreads();
+    }
+    for (int p = threadIdx.x; p < n; p += BLOCK) cand[tour[p]] = vals[p];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += BLOCK) {
+        keys[i] = cand[i];
+        keys[n + i] = row[n + i];
+    }
+    __syncthreads();
+    decode_block<KIND_TSPJ, BLOCK>(d.prob, lds, lay, nb, &s_fit, nullptr);
+    __syncthreads();
+    const float old = d.fit[b][slot];
+    if (s_fit < old) {
+        for (int i = threadIdx.x; i < n; i += BLOCK) row[i] = cand[i];
+        if (threadIdx.x == 0) d.fit[b][slot] = s_fit;
+    }
 }
 
 int partition_lds_bytes(int Pi)
@@ -324,22 +426,53 @@ hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipS
     }
 }
 
-hipError_t launch_partition_one(const InstDesc& d, int buf, int advance, hipStream_t s)
+hipError_t launch_partition_one(const InstDesc& d, int buf, int mode, hipStream_t s)
 {
     const int lds = partition_lds_bytes(d.Pi);
     ensure_lds(k_partition_one, lds);
-    hipLaunchKernelGGL(k_partition_one, dim3(d.islands), dim3(kPartBlock), lds, s, d, buf, advance);
+    hipLaunchKernelGGL(k_partition_one, dim3(d.islands), dim3(kPartBlock), lds, s, d, buf, mode);
     return hipGetLastError();
 }
 
-hipError_t launch_partition_many(const InstDesc* d_descs, int num_inst, int blocks, int max_Pi, int buf, int advance,
+hipError_t launch_partition_many(const InstDesc* d_descs, int num_inst, int blocks, int max_Pi, int buf, int mode,
                                  hipStream_t s)
 {
     if (blocks <= 0) return hipSuccess;
     const int lds = partition_lds_bytes(max_Pi);
     ensure_lds(k_partition_many, lds);
-    hipLaunchKernelGGL(k_partition_many, dim3(blocks), dim3(kPartBlock), lds, s, d_descs, num_inst, buf, advance);
+    hipLaunchKernelGGL(k_partition_many, dim3(blocks), dim3(kPartBlock), lds, s, d_descs, num_inst, buf, mode);
     return hipGetLastError();
+}
+
+int tspj_ls_lds_bytes(int n, int nb)
+{
+    return decode_layout(KIND_TSPJ, n, 2 * n, nb).bytes + 12 * n + 16;
+}
+
+namespace {
+
+template <int BLOCK>
+hipError_t ls_one(const InstDesc* d, int num, int items, int lds, hipStream_t s)
+{
+    ensure_lds(k_tspj_local_search<BLOCK>, lds);
+    hipLaunchKernelGGL((k_tspj_local_search<BLOCK>), dim3(items, num), dim3(BLOCK), lds, s, d, num);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_tspj_local_search(const InstDesc* d_descs, int num_inst, int items, int block, int lds,
+                                    hipStream_t s)
+{
+    if (num_inst <= 0 || items <= 0) return hipSuccess;
+    switch (block) {
+    case 64: return ls_one<64>(d_descs, num_inst, items, lds, s);
+    case 128: return ls_one<128>(d_descs, num_inst, items, lds, s);
+    case 256: return ls_one<256>(d_descs, num_inst, items, lds, s);
+    case 512: return ls_one<512>(d_descs, num_inst, items, lds, s);
+    case 1024: return ls_one<1024>(d_descs, num_inst, items, lds, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace brkga

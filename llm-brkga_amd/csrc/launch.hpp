@@ -17,12 +17,21 @@ hipError_t launch_evaluate(const ProblemDesc& p, const float* keys, long long ld
 // (Philox keys + evaluation) in buffer 0.
 hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s);
 
-// Stable per-island sort of the fitness in buffer `buf` into the rank order;
-// with advance=1 it closes a generation step and bumps the step counter.
-hipError_t launch_partition_one(const InstDesc& d, int buf, int advance, hipStream_t s);
+// Stable per-island sort of the fitness in buffer `buf` into the rank order.
+// mode 0: sort only; 1: closes a generation step (bumps the step counter and
+// raises the local-search flag when the new generation is due); 2: sorts the
+// current generation again after a local search, only when the flag is
+// raised, and lowers it (buf is then taken from the step counter).
+hipError_t launch_partition_one(const InstDesc& d, int buf, int mode, hipStream_t s);
 // buf < 0: buffer from each population's step counter.
-hipError_t launch_partition_many(const InstDesc* d_descs, int num_inst, int blocks, int max_Pi, int buf, int advance,
+hipError_t launch_partition_many(const InstDesc* d_descs, int num_inst, int blocks, int max_Pi, int buf, int mode,
                                  hipStream_t s);
+
+// TSPJ local search of the current generation of every listed population
+// whose local-search flag is raised; `items` = largest islands x ls_k.
+hipError_t launch_tspj_local_search(const InstDesc* d_descs, int num_inst, int items, int block, int lds,
+                                    hipStream_t s);
+int tspj_ls_lds_bytes(int n, int nb);
 
 // Persistent queue step over work items {instance, first slot, end slot, -}
 // of one problem kind (batched / served populations).  xys: stage the

@@ -15,6 +15,8 @@ Runs, in order:
      [20, 5000], 25% greedy knapsack n in [100, 2000], 15% bi-objective TSP
      with NSGA-II population 512; 200-generation deadline) submitted at once
      to the request loop.
+With --tune the batched and served parts first autotune their launch shapes
+(as the benchmark does).
 
 Example:
   rocprofv3 --kernel-trace --stats --output-format csv -d out -o run -- \
@@ -35,6 +37,7 @@ def main():
     ap.add_argument("--n", type=int, default=1000)
     ap.add_argument("--batch", type=int, default=0)
     ap.add_argument("--serve", type=int, default=0)
+    ap.add_argument("--tune", action="store_true", help="autotune launch shapes before the profiled part")
     args = ap.parse_args()
 
     import torch
@@ -64,10 +67,12 @@ def main():
         params = [brkga_amd.Params(256, 38, 38, 0.7, 1, 0, 0, 0xB5EED + i) for i in range(512)]
         batch = brkga_amd.Batch(eng, probs, params)
         batch.initialize("launch")
+        if args.tune:
+            batch.autotune(2)
         batch.evolve(args.batch, "launch")
         print(f"batched generations: {batch.member(0).generation} of 512 TSP instances", flush=True)
     if args.serve > 0:
-        srv = brkga_amd.Server(eng, hbm_budget_bytes=8 << 30, ticks_per_round=4)
+        srv = brkga_amd.Server(eng, hbm_budget_bytes=8 << 30, ticks_per_round=4, autotune_reps=2 if args.tune else 0)
         ids = []
         for i, (kind, n) in enumerate(workloads.request_mix(args.serve)):
             inst, seed = 5000 + i, 0xB5EED + 0x10000 + i

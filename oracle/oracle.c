@@ -452,6 +452,59 @@ void orc_brkga_step(const orc_problem* p, const orc_params* q, int t,
     }
     orc_evaluate(p, keys_out, q->pop, fitness_out, NULL);
     orc_partition(fitness_out, I, Pi, q->maximize, perm_out);
+    orc_tspj_local_search(p, q, t + 1, keys_out, fitness_out, perm_out);
+}
+
+/* TSPJ local search (the hook of tspj_config.hpp:77-135, applied as
+ * solver.hpp:458-462 does: generations g > 0 with g % ls_interval == 0, the
+ * first min(ls_count, Ei) individuals of every island's rank order).  Each
+ * of ls_moves random 2-opt moves draws positions x = u0 % n, y = u1 % n
+ * (Philox words 2m, 2m+1 of (island, g, rank, LSEARCH)), orders them and,
+ * unless y - x < 2, reverses the decoded city tour between them.  The
+ * individual's city keys, sorted ascending, are then handed to the new tour
+ * in order (job keys unchanged), and the candidate replaces the individual
+ * only if its makespan is strictly smaller (cuda_kernels.cuh:224-246).  The
+ * reference reverses key values by position instead (cuda_kernels.cuh:
+ * 208-216), which does not reverse the decoded tour; that is not
+ * reproduced.  Every island is partitioned again afterwards. */
+void orc_tspj_local_search(const orc_problem* p, const orc_params* q, int g,
+                           float* keys, float* fitness, int32_t* perm)
+{
+    if (p->kind != ORC_TSPJ || q->ls_interval <= 0 || q->ls_count <= 0 || g <= 0 || g % q->ls_interval != 0)
+        return;
+    int n = p->n, L = 2 * n, I = q->islands;
+    int Pi = q->pop / I, Ei = q->elite / I;
+    int k = q->ls_count < Ei ? q->ls_count : Ei;
+    int32_t* tour = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    float* vals = (float*)malloc(sizeof(float) * (size_t)n);
+    float* cand = (float*)malloc(sizeof(float) * (size_t)L);
+    for (int i = 0; i < I; ++i) {
+        for (int r = 0; r < k; ++r) {
+            int row = i * Pi + perm[(size_t)i * Pi + r];
+            float* key = keys + (size_t)row * L;
+            orc_argsort_keys(key, n, tour);
+            for (int pos = 0; pos < n; ++pos) vals[pos] = key[tour[pos]];
+            for (int m = 0; m < q->ls_moves; ++m) {
+                uint32_t u0 = orc_philox_word(q->seed, (uint32_t)i, (uint32_t)g, (uint32_t)r, ORC_TAG_LSEARCH,
+                                              (uint32_t)(2 * m));
+                uint32_t u1 = orc_philox_word(q->seed, (uint32_t)i, (uint32_t)g, (uint32_t)r, ORC_TAG_LSEARCH,
+                                              (uint32_t)(2 * m + 1));
+                int x = (int)(u0 % (uint32_t)n), y = (int)(u1 % (uint32_t)n);
+                if (x > y) { int tmp = x; x = y; y = tmp; }
+                if (y - x < 2) continue;
+                while (x < y) { int tmp = tour[x]; tour[x] = tour[y]; tour[y] = tmp; ++x; --y; }
+            }
+            memcpy(cand, key, sizeof(float) * (size_t)L);
+            for (int pos = 0; pos < n; ++pos) cand[tour[pos]] = vals[pos];
+            float f = tspj_fitness(p, cand, NULL);
+            if (f < fitness[row]) {
+                memcpy(key, cand, sizeof(float) * (size_t)n);
+                fitness[row] = f;
+            }
+        }
+    }
+    free(tour); free(vals); free(cand);
+    orc_partition(fitness, I, Pi, q->maximize, perm);
 }
 
 /* Init + `generations` steps; returns the best fitness over islands

@@ -36,7 +36,8 @@ enum {
     ORC_TAG_COINS = 1,
     ORC_TAG_MUTANT = 2,
     ORC_TAG_INIT = 3,
-    ORC_TAG_MUTCOIN = 4
+    ORC_TAG_MUTCOIN = 4,
+    ORC_TAG_LSEARCH = 5
 };
 
 enum {
@@ -70,6 +71,9 @@ typedef struct {
     int mig_count;
     uint64_t seed;
     int maximize;
+    int ls_interval;  /* TSPJ local search every ls_interval generations (0 = off) */
+    int ls_count;     /* individuals per island (top of the rank order), capped at the elite */
+    int ls_moves;     /* random 2-opt moves per individual */
 } orc_params;
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
@@ -101,6 +105,10 @@ void orc_brkga_step(const orc_problem* p, const orc_params* q, int t,
                     float* keys_out, float* fitness_out, int32_t* perm_out);
 double orc_brkga_run(const orc_problem* p, const orc_params* q, int generations,
                      float* keys, float* fitness, int32_t* perm);
+/* TSPJ local search of generation g in place (keys, fitness, rank order);
+ * orc_brkga_step applies it to the generation it builds. */
+void orc_tspj_local_search(const orc_problem* p, const orc_params* q, int g,
+                           float* keys, float* fitness, int32_t* perm);
 
 void orc_nds_ranks(const float* obj, int count, int nobj, int32_t* rank);
 void orc_crowding(const float* obj, const int32_t* rank, int count, int nobj, float* crowd);

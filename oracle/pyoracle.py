@@ -48,6 +48,9 @@ class OrcParams(ctypes.Structure):
         ("mig_count", ctypes.c_int),
         ("seed", ctypes.c_uint64),
         ("maximize", ctypes.c_int),
+        ("ls_interval", ctypes.c_int),
+        ("ls_count", ctypes.c_int),
+        ("ls_moves", ctypes.c_int),
     ]
 
 
@@ -85,6 +88,8 @@ def lib():
                                      _PF, _PI, _PF, _PF, _PI]
         L.orc_brkga_run.argtypes = [ctypes.POINTER(OrcProblem), ctypes.POINTER(OrcParams), ctypes.c_int,
                                     _PF, _PF, _PI]
+        L.orc_tspj_local_search.argtypes = [ctypes.POINTER(OrcProblem), ctypes.POINTER(OrcParams), ctypes.c_int,
+                                            _PF, _PF, _PI]
         L.orc_brkga_run.restype = ctypes.c_double
         L.orc_nds_ranks.argtypes = [_PF, ctypes.c_int, ctypes.c_int, _PI]
         L.orc_crowding.argtypes = [_PF, _PI, ctypes.c_int, ctypes.c_int, _PF]
@@ -203,11 +208,13 @@ class Problem:
         return (fit[:, 0] if self.num_obj == 1 else fit), dec
 
 
-def params(pop, elite, mutants, rho, islands=1, mig_interval=0, mig_count=0, seed=0, maximize=0):
+def params(pop, elite, mutants, rho, islands=1, mig_interval=0, mig_count=0, seed=0, maximize=0,
+           ls_interval=0, ls_count=0, ls_moves=0):
     q = OrcParams()
     q.pop, q.elite, q.mutants, q.rho = pop, elite, mutants, rho
     q.islands, q.mig_interval, q.mig_count = islands, mig_interval, mig_count
     q.seed, q.maximize = seed & 0xFFFFFFFFFFFFFFFF, maximize
+    q.ls_interval, q.ls_count, q.ls_moves = ls_interval, ls_count, ls_moves
     return q
 
 
@@ -215,6 +222,12 @@ def init_keys(seed, stream, slots, length):
     k = np.empty((slots, length), np.float32)
     lib().orc_init_keys(seed, stream, slots, length, _f(k))
     return k
+
+
+def tspj_local_search(problem, q, g, keys, fitness, perm):
+    """In place: the local search the oracle applies to generation g."""
+    assert keys.dtype == np.float32 and fitness.dtype == np.float32 and perm.dtype == np.int32
+    lib().orc_tspj_local_search(ctypes.byref(problem.c), ctypes.byref(q), g, _f(keys), _f(fitness), _i(perm))
 
 
 def partition(fitness, segments, seg_len, maximize):

@@ -52,6 +52,11 @@ static float abi_best(BRKGAConfig<float>* cfg, uint64_t seed, int G)
     p.elite_prob = cfg->elite_prob;
     p.num_islands = 1;
     p.seed = seed;
+    if (cfg->has_local_search()) {  // the facade's defaults (reference solver.hpp:166)
+        p.ls_interval = 2;
+        p.ls_individuals = 100;
+        p.ls_moves = 5;
+    }
     brkga_pop_t pop = nullptr;
     brkga_facade::check(brkga_population_create(cfg->native_problem(0), &p, &pop), "create");
     brkga_facade::check(brkga_population_initialize(pop, nullptr), "init");

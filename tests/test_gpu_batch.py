@@ -30,17 +30,21 @@ MEMBERS = [
     ("knap_greedy", 100, 64, 12, 6, 0.7, 1, 0, 0),
     ("knap_threshold", 120, 64, 12, 6, 0.7, 2, 2, 3),
     ("tspj", 60, 64, 12, 6, 0.75, 1, 0, 0),
+    ("tspj_ls", 45, 64, 12, 6, 0.75, 2, 3, 2),
     ("bitsp", 30, 32, 0, 0, 0.7, 1, 0, 0),
     ("bitsp", 80, 64, 0, 0, 0.7, 1, 0, 0),
 ]
 
 
 def build(engine):
+    """tspj_ls: TSPJ with the local search every 2 generations (top 3 per
+    island, 5 moves)."""
     probs, params = [], []
     for i, (kind, n, P, E, M, rho, I, iv, k) in enumerate(MEMBERS):
-        gp, _ = make_problem(engine, kind, n, inst=100 + i)
+        ls = (2, 3, 5) if kind == "tspj_ls" else (0, 0, 0)
+        gp, _ = make_problem(engine, "tspj" if kind == "tspj_ls" else kind, n, inst=100 + i)
         probs.append(gp)
-        params.append(brkga_amd.Params(P, E, M, rho, I, iv, k, 0xBA7C4 + 7 * i))
+        params.append(brkga_amd.Params(P, E, M, rho, I, iv, k, 0xBA7C4 + 7 * i, *ls))
     return probs, params
 
 

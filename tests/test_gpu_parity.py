@@ -133,6 +133,30 @@ def test_brkga_trajectory_parity(engine, case):
     assert pop.generation == G
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_tspj_local_search_trajectory_parity(engine, graph):
+    """TSPJ with the local search every 2 generations on the top 4 of each
+    of 2 islands, 6 random 2-opt moves each: bit-exact to the oracle."""
+    print(f"tspj local search trajectory graph={graph}", flush=True)
+    gp, op = make_problem(engine, "tspj", 40, inst=11)
+    seed = 0x15EA
+    pop = brkga_amd.Population(gp, brkga_amd.Params(64, 12, 6, 0.75, 2, 3, 2, seed, 2, 4, 6))
+    pop.initialize()
+    ost = pyoracle.BrkgaState(op, pyoracle.params(64, 12, 6, 0.75, 2, 3, 2, seed, 0, 2, 4, 6))
+    compare_brkga(pop, ost, "after init")
+    for t in range(9):
+        pop.evolve(1, graph=graph)
+        ost.step()
+        compare_brkga(pop, ost, f"after step {t}")
+
+
+def test_local_search_rejected_for_other_kinds(engine):
+    gp, _ = make_problem(engine, "tsp", 30)
+    with pytest.raises(brkga_amd.BrkgaError) as e:
+        brkga_amd.Population(gp, brkga_amd.Params(32, 6, 4, 0.7, 1, 0, 0, 1, 2, 4, 5))
+    assert e.value.status == brkga_amd.ERR_INVALID_ARG
+
+
 def test_full_size_tsp_islands_parity(engine):
     """n=1000, P=1024 as 4 islands x 256, E=M=38 per island, ring migration
     of the top 2 every 50 generations: bit-exact over 52 generations, which

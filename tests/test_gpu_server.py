@@ -25,6 +25,7 @@ REQUESTS = [
     ("tsp_matrix", 50, 48, 9, 5, 0.65, 1, 0, 0, 7),
     ("knap_threshold", 90, 64, 12, 6, 0.7, 1, 0, 0, 0),
     ("tspj", 40, 64, 12, 6, 0.75, 1, 0, 0, 1),
+    ("tspj_ls", 35, 64, 12, 6, 0.75, 2, 2, 1, 7),
     ("tsp", 64, 64, 10, 10, 0.7, 1, 0, 0, 13),
     ("bitsp", 120, 128, 0, 0, 0.7, 1, 0, 0, 3),
     ("knap_greedy", 700, 128, 20, 12, 0.7, 1, 0, 0, 5),
@@ -44,8 +45,9 @@ def solo_best(gp, prm, G):
 def build_requests(engine):
     out = []
     for i, (kind, n, P, E, M, rho, I, iv, k, G) in enumerate(REQUESTS):
-        gp, _ = make_problem(engine, kind, n, inst=200 + i)
-        prm = brkga_amd.Params(P, E, M, rho, I, iv, k, 0x5E4E + 11 * i)
+        ls = (2, 4, 5) if kind == "tspj_ls" else (0, 0, 0)
+        gp, _ = make_problem(engine, "tspj" if kind == "tspj_ls" else kind, n, inst=200 + i)
+        prm = brkga_amd.Params(P, E, M, rho, I, iv, k, 0x5E4E + 11 * i, *ls)
         out.append((gp, prm, G))
     return out
 
