@@ -18,8 +18,9 @@ namespace brkga {
 
 // CLEARED: the argsort counters were cleared (and a barrier passed) by the
 // caller.  XY_LDS: coordinates were staged in LDS at xy0 / xy1 (coordinate
-// kinds only); otherwise they are read from p.a / p.b.
-template <int KIND, int BLOCK, bool CLEARED = false, bool XY_LDS = false>
+// kinds only); otherwise they are read from p.a / p.b.  ITEMS: keys per
+// thread of the sorting decoders, n <= ITEMS * BLOCK.
+template <int KIND, int BLOCK, bool CLEARED = false, bool XY_LDS = false, int ITEMS = kSortItems>
 __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout& lay, int nb,
                              float* fit_out, int* decoded, const float2* xy0 = nullptr,
                              const float2* xy1 = nullptr)
@@ -34,37 +35,71 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
     ss.items = reinterpret_cast<unsigned long long*>(lds + lay.items_off);
     ss.red = red;
 
-    if constexpr (KIND == KIND_TSP_COORDS || KIND == KIND_TSP_MATRIX) {
-        block_argsort<BLOCK, CLEARED>(keys, n, nb, ss, perm);
+    if constexpr (KIND == KIND_TSP_COORDS) {
+        // Each key writes its city's coordinates at its tour position, over
+        // the sort items once the ranking is done; edges are then summed
+        // from consecutive positions.
+        // With few keys per thread the coordinate loads are issued before
+        // the ranking; with many they would hold too many registers.
+        constexpr bool kEarly = ITEMS <= 8;
+        const float2* xy = XY_LDS ? xy0 : reinterpret_cast<const float2*>(p.a);
+        float2 c[ITEMS];
+        int pos[ITEMS];
+        auto load = [&]() {
+#pragma unroll
+            for (int i = 0; i < ITEMS; ++i) {
+                const int e = tid + i * BLOCK;
+                if (e < n) c[i] = xy[e];
+            }
+        };
+        if constexpr (kEarly)
+            block_rank<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, pos, nullptr, load);
+        else
+            block_rank<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, pos);
+        float2* txy = reinterpret_cast<float2*>(lds + lay.items_off);
+        if constexpr (!kEarly) load();
+        __syncthreads();  // every ranking read of the items is done
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int e = tid + i * BLOCK;
+            if (e < n) {
+                txy[pos[i]] = c[i];
+                if (decoded) decoded[pos[i]] = e;
+            }
+        }
+        __syncthreads();
+        FxAcc acc;
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int q = tid + i * BLOCK;
+            if (q < n) {
+                const float2 a = txy[q];
+                const float2 b = txy[q + 1 < n ? q + 1 : 0];
+                fx_add(acc, edge_xy(a.x, a.y, b.x, b.y), p.scale0);
+            }
+        }
+        const long long tot = block_sum_fx<BLOCK>(fx_value(acc), ss);
+        if (tid == 0 && fit_out) fit_out[0] = fx_to_float(tot, p.inv0);
+    } else if constexpr (KIND == KIND_TSP_MATRIX) {
+        block_argsort<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, perm);
         // thread t sums the edges leaving positions [t*chunk, t*chunk+chunk)
         const int chunk = (n + BLOCK - 1) / BLOCK;
         const int p0 = tid * chunk;
         const int m = p0 < n ? (p0 + chunk < n ? chunk : n - p0) : 0;
-        int pv[kSortItems + 1];
+        int pv[ITEMS + 1];
 #pragma unroll
-        for (int k = 0; k <= kSortItems; ++k) {
+        for (int k = 0; k <= ITEMS; ++k) {
             const int p = p0 + k;
             if (k <= m && m > 0) pv[k] = perm[p < n ? p : p - n];
         }
         FxAcc acc;
-        if constexpr (KIND == KIND_TSP_COORDS) {
-            const float2* xy = XY_LDS ? xy0 : reinterpret_cast<const float2*>(p.a);
-            float2 c[kSortItems + 1];
+        float dd[ITEMS];
 #pragma unroll
-            for (int k = 0; k <= kSortItems; ++k)
-                if (k <= m && m > 0) c[k] = xy[pv[k]];
+        for (int k = 0; k < ITEMS; ++k)
+            if (k < m) dd[k] = p.a[(size_t)pv[k] * n + pv[k + 1]];
 #pragma unroll
-            for (int k = 0; k < kSortItems; ++k)
-                if (k < m) fx_add(acc, edge_xy(c[k].x, c[k].y, c[k + 1].x, c[k + 1].y), p.scale0);
-        } else {
-            float dd[kSortItems];
-#pragma unroll
-            for (int k = 0; k < kSortItems; ++k)
-                if (k < m) dd[k] = p.a[(size_t)pv[k] * n + pv[k + 1]];
-#pragma unroll
-            for (int k = 0; k < kSortItems; ++k)
-                if (k < m) fx_add(acc, dd[k], p.scale0);
-        }
+        for (int k = 0; k < ITEMS; ++k)
+            if (k < m) fx_add(acc, dd[k], p.scale0);
         const long long tot = block_sum_fx<BLOCK>(fx_value(acc), ss);
         if (tid == 0 && fit_out) fit_out[0] = fx_to_float(tot, p.inv0);
         if (decoded)
@@ -86,7 +121,7 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
             fit_out[0] = __ll2float_rn(f);
         }
     } else if constexpr (KIND == KIND_KNAP_GREEDY) {
-        block_argsort<BLOCK, CLEARED>(keys, n, nb, ss, perm);
+        block_argsort<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, perm);
         // With weights >= 0 the greedy fill takes every item up to the first
         // position f where the running weight in key order exceeds the
         // capacity (a block prefix scan finds f), rejects item f, and from
@@ -145,8 +180,8 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
     } else if constexpr (KIND == KIND_TSPJ) {
         int* tour = perm;
         int* jobs = perm + n;
-        block_argsort<BLOCK, CLEARED>(keys, n, nb, ss, tour);
-        block_argsort<BLOCK>(keys + n, n, nb, ss, jobs);
+        block_argsort<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, tour);
+        block_argsort<BLOCK, false, ITEMS>(keys + n, n, nb, ss, jobs);
         const int dim = p.dim;
         const int chunk = (n + BLOCK - 1) / BLOCK;
         const int lo = tid * chunk;
@@ -177,7 +212,7 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
                 decoded[n + q] = jobs[q] + 1;
             }
     } else if constexpr (KIND == KIND_BITSP_COORDS || KIND == KIND_BITSP_MATRIX) {
-        block_argsort<BLOCK, CLEARED>(keys, n, nb, ss, perm);
+        block_argsort<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, perm);
         const int dim = p.dim;
         long long acc0 = 0, acc1 = 0;
         for (int q = tid; q <= n; q += BLOCK) {

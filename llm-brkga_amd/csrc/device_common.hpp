@@ -341,9 +341,12 @@ __device__ __forceinline__ void scan_bucket_counts(int* cnt, int nb, int n, int*
     __syncthreads();
 }
 
-// perm[p] = index of the element at position p of the stable ascending order
-// of keys[0..n) (equal keys keep ascending index order, i.e. std::sort on
-// pair<float,int>(key, index) as configs/tsp_config.hpp:209-216 does).
+// Sorted positions of keys[0..n) in the stable ascending order (equal keys
+// keep ascending index order, i.e. std::sort on pair<float,int>(key, index)
+// as configs/tsp_config.hpp:209-216 does): on return pos[i] is the position
+// of key tid + i * BLOCK, and perm[pos[i]] = tid + i * BLOCK when perm is
+// given.  Thread tid holds the keys tid + i * BLOCK, i < ITEMS: n <= ITEMS *
+// BLOCK.
 //
 // Bucket sort: keys are counted into nb value buckets (LDS atomics), the
 // counts are scanned into bucket ranges and every key drops its
@@ -351,54 +354,85 @@ __device__ __forceinline__ void scan_bucket_counts(int* cnt, int nb, int n, int*
 // final position is its bucket start plus the number of smaller items in
 // its bucket (buckets hold about two items on average); every key ranks
 // itself, with no per-bucket leader and no divergent sorting networks.
-// keys and perm live in LDS and may overlay each other: keys are last read
-// before the barrier that precedes the first write of perm.  The result is
-// ready after the final barrier.
-// CLEARED: the caller already ran argsort_clear and a barrier.
-template <int BLOCK, bool CLEARED = false>
-__device__ void block_argsort(const float* keys, int n, int nb, const SortScratch& s, int* perm)
+// Each thread keeps its keys' order keys and bucket ranges in registers, so
+// the ranking reads only the other items of the bucket.  keys are read once,
+// before the first barrier, so perm may overlay them.  No barrier follows
+// the ranking: callers synchronise before they read perm or reuse s.items.
+// CLEARED: the caller already ran argsort_clear and a barrier.  pre_rank()
+// runs between the placement and the ranking (loads issued there overlap
+// the ranking without holding registers through the whole sort).
+struct NoHook {
+    __device__ void operator()() const {}
+};
+
+template <int BLOCK, bool CLEARED = false, int ITEMS = kSortItems, typename Hook = NoHook>
+__device__ __forceinline__ void block_rank(const float* keys, int n, int nb, const SortScratch& s,
+                                           int (&pos)[ITEMS], int* perm = nullptr, Hook pre_rank = Hook())
 {
     const int tid = threadIdx.x;
     if (!CLEARED) {
         argsort_clear<BLOCK>(s, nb);
         __syncthreads();
     }
-    int bkt[kSortItems], slot[kSortItems];
+    // rng[i]: the bucket, then (bucket start | bucket size << 16); both are
+    // below 2^15 since n <= ITEMS * BLOCK <= kSortItems * 1024
+    int rng[ITEMS], slot[ITEMS];
+    uint32_t ok[ITEMS];
 #pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
+    for (int i = 0; i < ITEMS; ++i) {
         const int e = tid + i * BLOCK;
         if (e < n) {
-            bkt[i] = key_bucket(keys[e], nb);
-            slot[i] = atomicAdd(&s.cnt[bkt[i]], 1);
+            const float k = keys[e];
+            ok[i] = key_order(k);
+            rng[i] = key_bucket(k, nb);
+            slot[i] = atomicAdd(&s.cnt[rng[i]], 1);
         }
     }
     __syncthreads();
     scan_bucket_counts<BLOCK>(s.cnt, nb, n, reinterpret_cast<int*>(s.red));
 #pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
-        const int e = tid + i * BLOCK;
-        if (e < n)
-            s.items[s.cnt[bkt[i]] + slot[i]] = ((unsigned long long)key_order(keys[e]) << 32) | (unsigned)e;
-    }
-    __syncthreads();
-    // the first four items of every bucket are read as independent loads
-    // (buckets rarely hold more); longer buckets finish in a loop
-#pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
+    for (int i = 0; i < ITEMS; ++i) {
         const int e = tid + i * BLOCK;
         if (e < n) {
-            const int st = s.cnt[bkt[i]];
-            const int sz = s.cnt[bkt[i] + 1] - st;
-            const unsigned long long mine = s.items[st + slot[i]];
-            const unsigned long long x1 = sz > 1 ? s.items[st + 1] : ~0ull;
-            const unsigned long long x2 = sz > 2 ? s.items[st + 2] : ~0ull;
-            const unsigned long long x3 = sz > 3 ? s.items[st + 3] : ~0ull;
-            const unsigned long long x0 = s.items[st];
-            int r = (x0 < mine) + (x1 < mine) + (x2 < mine) + (x3 < mine);
-            for (int q = st + 4; q < st + sz; ++q) r += s.items[q] < mine ? 1 : 0;
-            perm[st + r] = e;
+            const int st = s.cnt[rng[i]];
+            const int en = s.cnt[rng[i] + 1];
+            s.items[st + slot[i]] = ((unsigned long long)ok[i] << 32) | (unsigned)e;
+            rng[i] = st | ((en - st) << 16);
         }
     }
+    __syncthreads();
+    pre_rank();
+    // the first four items of every bucket are read as independent loads
+    // (buckets rarely hold more); longer buckets finish in a loop.  A key
+    // compared with itself counts nothing.
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const int e = tid + i * BLOCK;
+        if (e < n) {
+            const int st = rng[i] & 0xFFFF;
+            const int sz = rng[i] >> 16;
+            const unsigned long long mine = ((unsigned long long)ok[i] << 32) | (unsigned)e;
+            const unsigned long long* b = s.items + st;
+            const unsigned long long x0 = b[0];
+            const unsigned long long x1 = sz > 1 ? b[1] : ~0ull;
+            const unsigned long long x2 = sz > 2 ? b[2] : ~0ull;
+            const unsigned long long x3 = sz > 3 ? b[3] : ~0ull;
+            int r = (x0 < mine) + (x1 < mine) + (x2 < mine) + (x3 < mine);
+            for (int q = 4; q < sz; ++q) r += b[q] < mine ? 1 : 0;
+            pos[i] = st + r;
+            if (perm) perm[st + r] = e;
+        }
+    }
+}
+
+// perm[p] = index of the element at position p of the stable ascending
+// order of keys[0..n) (block_rank).  keys and perm live in LDS and may
+// overlay each other.  The result is ready after the final barrier.
+template <int BLOCK, bool CLEARED = false, int ITEMS = kSortItems>
+__device__ void block_argsort(const float* keys, int n, int nb, const SortScratch& s, int* perm)
+{
+    int pos[ITEMS];
+    block_rank<BLOCK, CLEARED, ITEMS>(keys, n, nb, s, pos, perm);
     __syncthreads();
 }
 

@@ -120,7 +120,10 @@ __host__ __device__ inline DecodeLayout decode_layout(int kind, int n, int L, in
 }
 
 // Keys one decode thread holds: a workgroup of B threads decodes n <= 16 B.
+// Kernels also come in a compact form for n <= 4 B, which holds fewer
+// registers.
 constexpr int kSortItems = 16;
+constexpr int kCompactItems = 4;
 
 // Rank orders longer than this are read from global memory, not staged.
 constexpr int kMaxStagedOrder = 2048;

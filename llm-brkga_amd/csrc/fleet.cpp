@@ -205,7 +205,7 @@ brkga_status Fleet::build_items(hipStream_t s)
                     items.push_back(make_int4(m, isl * d.Pi + s0, isl * d.Pi + std::min(s0 + b.chunk, d.Pi), 0));
         }
         b.item_count = (int)items.size() - b.item_begin;
-        const int per_cu = step_queue_blocks_per_cu(b.kind, b.block, b.xys, b.lds);
+        const int per_cu = step_queue_blocks_per_cu(b.kind, b.block, b.xys, b.n_max, b.lds);
         if (per_cu <= 0) return set_error(BRKGA_ERR_DEVICE, "queue step kernel does not fit on a CU");
         b.grid = std::min(b.item_count, per_cu * eng_->cu_count);
     }
@@ -275,7 +275,7 @@ brkga_status Fleet::tick(hipStream_t s)
         }
     for (size_t i = 0; i < buckets_.size() && e == hipSuccess; ++i) {
         const FleetBucket& b = buckets_[i];
-        e = launch_step_queue(b.kind, b.block, b.xys, d_descs_, d_items_ + b.item_begin, b.item_count,
+        e = launch_step_queue(b.kind, b.block, b.xys, b.n_max, d_descs_, d_items_ + b.item_begin, b.item_count,
                               d_queues_ + kQueueStride * i, b.grid, b.lds, lane(b.lane));
     }
     for (size_t i = 0; i < groups_.size() && e == hipSuccess; ++i) {
@@ -392,7 +392,7 @@ brkga_status Fleet::tune(int reps, bool only_new, hipStream_t s)
             if ((size_t)lds > eng_->lds_limit) continue;
             for (int blk : kBlocks) {
                 if (blk * kSortItems < buckets_[bi].n_max) continue;
-                if (step_queue_blocks_per_cu(buckets_[bi].kind, blk, xys, lds) <= 0) continue;
+                if (step_queue_blocks_per_cu(buckets_[bi].kind, blk, xys, buckets_[bi].n_max, lds) <= 0) continue;
                 for (int ch : kChunks) {
                     if (ch > buckets_[bi].max_Pi) continue;
                     buckets_[bi].block = blk;
@@ -402,11 +402,11 @@ brkga_status Fleet::tune(int reps, bool only_new, hipStream_t s)
                     if ((st = build_items(s)) != BRKGA_OK) break;
                     const FleetBucket& bk = buckets_[bi];
                     int* q = d_queues_ + kQueueStride * bi;
-                    e = launch_step_queue(bk.kind, bk.block, bk.xys, d_descs_, d_items_ + bk.item_begin,
+                    e = launch_step_queue(bk.kind, bk.block, bk.xys, bk.n_max, d_descs_, d_items_ + bk.item_begin,
                                           bk.item_count, q, bk.grid, bk.lds, s);
                     if (e == hipSuccess) e = hipEventRecord(a, s);
                     for (int r = 0; r < reps && e == hipSuccess; ++r)
-                        e = launch_step_queue(bk.kind, bk.block, bk.xys, d_descs_, d_items_ + bk.item_begin,
+                        e = launch_step_queue(bk.kind, bk.block, bk.xys, bk.n_max, d_descs_, d_items_ + bk.item_begin,
                                               bk.item_count, q, bk.grid, bk.lds, s);
                     if (e == hipSuccess) e = hipEventRecord(b, s);
                     if (e == hipSuccess) e = hipEventSynchronize(b);

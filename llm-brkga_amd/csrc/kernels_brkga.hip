@@ -44,10 +44,10 @@ __global__ __launch_bounds__(BLOCK) void k_evaluate(ProblemDesc p, const float* 
 }
 
 // One block of a generation step: slot gs of the population described by d.
-template <int KIND, int BLOCK>
+// n <= ITEMS * BLOCK.
+template <int KIND, int BLOCK, int ITEMS>
 __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, int init, char* lds)
 {
-    constexpr bool kXY = KIND == KIND_TSP_COORDS;
     const int Pi = d.Pi, Ei = d.Ei, I = d.islands;
     const int island = gs / Pi;
     const int s = gs - island * Pi;
@@ -72,26 +72,15 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
 
     const int t = init ? 0 : *d.gen;
     const int nb = d.nb;
-    const StepLayout sl = step_layout(KIND, n, L, nb, Pi);
+    const StepLayout sl = step_layout(KIND, n, L, nb, Pi, false);
     float4* kl = reinterpret_cast<float4*>(lds + sl.dec.keys_off);
-    float2* xy = reinterpret_cast<float2*>(lds + sl.xy_off);
     int* order = reinterpret_cast<int*>(lds + sl.isl_off);
     const bool staged = Pi <= kMaxStagedOrder;
     SortScratch ss;
     ss.cnt = reinterpret_cast<int*>(lds + sl.dec.cnt_off);
     ss.red = reinterpret_cast<long long*>(lds + sl.dec.red_off);
 
-    // independent loads first: coordinates (kept in registers until the
-    // keys are built), island rank order; then clear the argsort counters
-    float2 cxy[kXY ? kSortItems : 1];
-    if constexpr (kXY) {
-        const float2* gxy = reinterpret_cast<const float2*>(d.prob.a);
-#pragma unroll
-        for (int k = 0; k < kSortItems; ++k) {
-            const int i = threadIdx.x + k * BLOCK;
-            if (i < n) cxy[k] = gxy[i];
-        }
-    }
+    // stage the island rank order, clear the argsort counters
     if (!init && staged)
         for (int i = threadIdx.x; i < Pi; i += BLOCK) order[i] = perm_cur[i];
     argsort_clear<BLOCK>(ss, nb);
@@ -99,22 +88,15 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
 
     const bool child = build_child<BLOCK>(d, island, s, cur, init, t, staged ? order : perm_cur, out, kl, fit_out);
     if (!child) return;
-    if constexpr (kXY) {
-#pragma unroll
-        for (int k = 0; k < kSortItems; ++k) {
-            const int i = threadIdx.x + k * BLOCK;
-            if (i < n) xy[i] = cxy[k];
-        }
-    }
     __syncthreads();
-    decode_block<KIND, BLOCK, true, kXY>(d.prob, lds, sl.dec, nb, fit_out, nullptr, xy, nullptr);
+    decode_block<KIND, BLOCK, true, false, ITEMS>(d.prob, lds, sl.dec, nb, fit_out, nullptr);
 }
 
-template <int KIND, int BLOCK>
+template <int KIND, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_step_one(InstDesc d, int cur, int init)
 {
     extern __shared__ __align__(16) char lds[];
-    step_body<KIND, BLOCK>(d, blockIdx.x, cur, init, lds);
+    step_body<KIND, BLOCK, ITEMS>(d, blockIdx.x, cur, init, lds);
 }
 
 constexpr int kPartBlock = 1024;
@@ -370,8 +352,14 @@ hipError_t eval_kind(int block, const ProblemDesc& p, const float* keys, long lo
 template <int KIND, int BLOCK>
 hipError_t step_single(const InstDesc& d, int lds, int cur, int init, hipStream_t s)
 {
-    ensure_lds(k_step_one<KIND, BLOCK>, lds);
-    hipLaunchKernelGGL((k_step_one<KIND, BLOCK>), dim3(d.pop), dim3(BLOCK), lds, s, d, cur, init);
+    if (d.prob.n <= kCompactItems * BLOCK) {
+        ensure_lds(k_step_one<KIND, BLOCK, kCompactItems>, lds);
+        hipLaunchKernelGGL((k_step_one<KIND, BLOCK, kCompactItems>), dim3(d.pop), dim3(BLOCK), lds, s, d, cur,
+                           init);
+    } else {
+        ensure_lds(k_step_one<KIND, BLOCK, kSortItems>, lds);
+        hipLaunchKernelGGL((k_step_one<KIND, BLOCK, kSortItems>), dim3(d.pop), dim3(BLOCK), lds, s, d, cur, init);
+    }
     return hipGetLastError();
 }
 
@@ -410,7 +398,7 @@ hipError_t launch_evaluate(const ProblemDesc& p, const float* keys, long long ld
 
 int step_lds_bytes(const InstDesc& d)
 {
-    return step_layout(d.prob.kind, d.prob.n, d.L, d.nb, d.Pi).bytes;
+    return step_layout(d.prob.kind, d.prob.n, d.L, d.nb, d.Pi, false).bytes;
 }
 
 hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s)

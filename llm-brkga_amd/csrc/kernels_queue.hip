@@ -19,7 +19,7 @@ namespace brkga {
 // so the kernel can be relaunched (or replayed from a graph) as is.
 // XYS: Euclidean coordinates are staged in LDS per item (otherwise the
 // decoder reads them through L2, leaving LDS to more resident workgroups).
-template <int KIND, int BLOCK, bool XYS>
+template <int KIND, int BLOCK, bool XYS, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_step_queue(const InstDesc* __restrict__ descs,
                                                       const int4* __restrict__ items, int num_items,
                                                       int* __restrict__ queue)
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_queue(const InstDesc* __restrict
                 build_child<BLOCK>(d, island, s, cur, init, t, staged ? order : perm_cur, out, kl, fit_out);
             if (!child) continue;
             __syncthreads();
-            decode_block<KIND, BLOCK, true, kXY>(d.prob, lds, sl.dec, nb, fit_out, nullptr, kXY ? xy : nullptr,
+            decode_block<KIND, BLOCK, true, kXY, ITEMS>(d.prob, lds, sl.dec, nb, fit_out, nullptr, kXY ? xy : nullptr,
                                                  nullptr);
         }
     }
@@ -100,40 +100,56 @@ void ensure_lds(F func, int bytes)
                                   bytes);
 }
 
-// Kernel symbol of the queue step for (kind, block, coordinate staging);
-// nullptr when there is no such instantiation.
-template <int KIND, bool XYS>
-const void* queue_kernel_kind(int block)
+// Kernel symbol of the queue step for (kind, block, coordinate staging,
+// largest n); nullptr when there is no such instantiation.  The kernel
+// holds the fewest keys per thread (4, 8 or 16) that the bucket's largest n
+// allows: fewer keys, fewer registers, more resident workgroups.
+template <int KIND, bool XYS, int ITEMS>
+const void* queue_kernel_items(int block)
 {
     switch (block) {
-    case 64: return reinterpret_cast<const void*>(k_step_queue<KIND, 64, XYS>);
-    case 128: return reinterpret_cast<const void*>(k_step_queue<KIND, 128, XYS>);
-    case 256: return reinterpret_cast<const void*>(k_step_queue<KIND, 256, XYS>);
-    case 512: return reinterpret_cast<const void*>(k_step_queue<KIND, 512, XYS>);
-    case 1024: return reinterpret_cast<const void*>(k_step_queue<KIND, 1024, XYS>);
+    case 64: return reinterpret_cast<const void*>(k_step_queue<KIND, 64, XYS, ITEMS>);
+    case 128: return reinterpret_cast<const void*>(k_step_queue<KIND, 128, XYS, ITEMS>);
+    case 256: return reinterpret_cast<const void*>(k_step_queue<KIND, 256, XYS, ITEMS>);
+    case 512: return reinterpret_cast<const void*>(k_step_queue<KIND, 512, XYS, ITEMS>);
+    case 1024: return reinterpret_cast<const void*>(k_step_queue<KIND, 1024, XYS, ITEMS>);
     default: return nullptr;
     }
 }
 
-const void* queue_kernel(int kind, int block, bool xys)
+template <int KIND, bool XYS>
+const void* queue_kernel_kind(int block, int n_max)
+{
+    if (n_max > kSortItems * block) return nullptr;
+    if constexpr (KIND == KIND_KNAP_THRESHOLD) {
+        return queue_kernel_items<KIND, XYS, kSortItems>(block);  // no sort, no per-thread keys
+    } else {
+        if (n_max <= kCompactItems * block) return queue_kernel_items<KIND, XYS, kCompactItems>(block);
+        if (n_max <= 2 * kCompactItems * block) return queue_kernel_items<KIND, XYS, 2 * kCompactItems>(block);
+        return queue_kernel_items<KIND, XYS, kSortItems>(block);
+    }
+}
+
+const void* queue_kernel(int kind, int block, bool xys, int n_max)
 {
     switch (kind) {
     case KIND_TSP_COORDS:
-        return xys ? queue_kernel_kind<KIND_TSP_COORDS, true>(block) : queue_kernel_kind<KIND_TSP_COORDS, false>(block);
-    case KIND_TSP_MATRIX: return xys ? nullptr : queue_kernel_kind<KIND_TSP_MATRIX, false>(block);
-    case KIND_KNAP_THRESHOLD: return xys ? nullptr : queue_kernel_kind<KIND_KNAP_THRESHOLD, false>(block);
-    case KIND_KNAP_GREEDY: return xys ? nullptr : queue_kernel_kind<KIND_KNAP_GREEDY, false>(block);
-    case KIND_TSPJ: return xys ? nullptr : queue_kernel_kind<KIND_TSPJ, false>(block);
+        return xys ? queue_kernel_kind<KIND_TSP_COORDS, true>(block, n_max)
+                   : queue_kernel_kind<KIND_TSP_COORDS, false>(block, n_max);
+    case KIND_TSP_MATRIX: return xys ? nullptr : queue_kernel_kind<KIND_TSP_MATRIX, false>(block, n_max);
+    case KIND_KNAP_THRESHOLD: return xys ? nullptr : queue_kernel_kind<KIND_KNAP_THRESHOLD, false>(block, n_max);
+    case KIND_KNAP_GREEDY: return xys ? nullptr : queue_kernel_kind<KIND_KNAP_GREEDY, false>(block, n_max);
+    case KIND_TSPJ: return xys ? nullptr : queue_kernel_kind<KIND_TSPJ, false>(block, n_max);
     default: return nullptr;
     }
 }
 
 }  // namespace
 
-hipError_t launch_step_queue(int kind, int block, bool xys, const InstDesc* d_descs, const int4* d_items,
+hipError_t launch_step_queue(int kind, int block, bool xys, int n_max, const InstDesc* d_descs, const int4* d_items,
                              int num_items, int* d_queue, int grid, int lds, hipStream_t s)
 {
-    const void* f = queue_kernel(kind, block, xys);
+    const void* f = queue_kernel(kind, block, xys, n_max);
     if (!f) return hipErrorInvalidValue;
     if (num_items <= 0 || grid <= 0) return hipSuccess;
     ensure_lds(f, lds);
@@ -141,9 +157,9 @@ hipError_t launch_step_queue(int kind, int block, bool xys, const InstDesc* d_de
     return hipLaunchKernel(f, dim3(grid), dim3(block), args, (size_t)lds, s);
 }
 
-int step_queue_blocks_per_cu(int kind, int block, bool xys, int lds)
+int step_queue_blocks_per_cu(int kind, int block, bool xys, int n_max, int lds)
 {
-    const void* f = queue_kernel(kind, block, xys);
+    const void* f = queue_kernel(kind, block, xys, n_max);
     if (!f) return 0;
     ensure_lds(f, lds);
     int nb = 0;

@@ -34,13 +34,14 @@ hipError_t launch_tspj_local_search(const InstDesc* d_descs, int num_inst, int i
 int tspj_ls_lds_bytes(int n, int nb);
 
 // Persistent queue step over work items {instance, first slot, end slot, -}
-// of one problem kind (batched / served populations).  xys: stage the
-// Euclidean coordinates in LDS (coordinate kind only).  d_queue holds two
-// counters that must be zero at the first launch; the kernel re-arms them.
-hipError_t launch_step_queue(int kind, int block, bool xys, const InstDesc* d_descs, const int4* d_items,
+// of one problem kind (batched / served populations) whose problem sizes are
+// at most n_max.  xys: stage the Euclidean coordinates in LDS (coordinate
+// kind only).  d_queue holds two counters that must be zero at the first
+// launch; the kernel re-arms them.
+hipError_t launch_step_queue(int kind, int block, bool xys, int n_max, const InstDesc* d_descs, const int4* d_items,
                              int num_items, int* d_queue, int grid, int lds, hipStream_t s);
 // Resident workgroups of the queue step per CU (0: no such kernel).
-int step_queue_blocks_per_cu(int kind, int block, bool xys, int lds);
+int step_queue_blocks_per_cu(int kind, int block, bool xys, int n_max, int lds);
 // Step counter := gen and arrival counter := 0 for `num` instances.
 hipError_t launch_set_counters(const InstDesc* d_descs, int num, int gen, hipStream_t s);
 
