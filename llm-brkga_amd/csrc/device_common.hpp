@@ -292,11 +292,13 @@ struct SortScratch {
     long long* red;             // [17] per-wave partials; red[16] = block sum
 };
 
-// Clears the bucket counters and the block-sum slot; a barrier must follow.
+// Clears the bucket counters (nb + 4 of them, 16-byte aligned) and the
+// block-sum slot; a barrier must follow.
 template <int BLOCK>
 __device__ __forceinline__ void argsort_clear(const SortScratch& s, int nb)
 {
-    for (int i = threadIdx.x; i <= nb; i += BLOCK) s.cnt[i] = 0;
+    int4* c4 = reinterpret_cast<int4*>(s.cnt);
+    for (int i = threadIdx.x; i <= (nb >> 2); i += BLOCK) c4[i] = make_int4(0, 0, 0, 0);
     if (threadIdx.x == 0) s.red[16] = 0;
 }
 
@@ -331,10 +333,24 @@ __device__ __forceinline__ void scan_bucket_counts(int* cnt, int nb, int n, int*
     }
     int run = base + incl - local;
     if (lo < nb) {
-        for (int k = 0; k < per; ++k) {
-            const int c = cnt[lo + k];
-            cnt[lo + k] = run;
-            run += c;
+        if ((per & 3) == 0) {
+            int4* c4 = reinterpret_cast<int4*>(cnt + lo);
+            for (int k = 0; k < (per >> 2); ++k) {
+                const int4 v = c4[k];
+                int4 o;
+                o.x = run;
+                o.y = run + v.x;
+                o.z = o.y + v.y;
+                o.w = o.z + v.z;
+                run = o.w + v.w;
+                c4[k] = o;
+            }
+        } else {
+            for (int k = 0; k < per; ++k) {
+                const int c = cnt[lo + k];
+                cnt[lo + k] = run;
+                run += c;
+            }
         }
     }
     if (tid == 0) cnt[nb] = n;
@@ -352,7 +368,7 @@ __device__ __forceinline__ void scan_bucket_counts(int* cnt, int nb, int n, int*
 // counts are scanned into bucket ranges and every key drops its
 // (order key, index) item into its range.  Items are unique, so each key's
 // final position is its bucket start plus the number of smaller items in
-// its bucket (buckets hold about two items on average); every key ranks
+// its bucket (buckets hold about half a key on average); every key ranks
 // itself, with no per-bucket leader and no divergent sorting networks.
 // Each thread keeps its keys' order keys and bucket ranges in registers, so
 // the ranking reads only the other items of the bucket.  keys are read once,
@@ -402,23 +418,26 @@ __device__ __forceinline__ void block_rank(const float* keys, int n, int nb, con
     }
     __syncthreads();
     pre_rank();
-    // the first four items of every bucket are read as independent loads
-    // (buckets rarely hold more); longer buckets finish in a loop.  A key
-    // compared with itself counts nothing.
+    // A key alone in its bucket reads nothing; otherwise the first four
+    // items of the bucket are read as independent loads and longer buckets
+    // finish in a loop.  A key compared with itself counts nothing.
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         const int e = tid + i * BLOCK;
         if (e < n) {
             const int st = rng[i] & 0xFFFF;
             const int sz = rng[i] >> 16;
-            const unsigned long long mine = ((unsigned long long)ok[i] << 32) | (unsigned)e;
-            const unsigned long long* b = s.items + st;
-            const unsigned long long x0 = b[0];
-            const unsigned long long x1 = sz > 1 ? b[1] : ~0ull;
-            const unsigned long long x2 = sz > 2 ? b[2] : ~0ull;
-            const unsigned long long x3 = sz > 3 ? b[3] : ~0ull;
-            int r = (x0 < mine) + (x1 < mine) + (x2 < mine) + (x3 < mine);
-            for (int q = 4; q < sz; ++q) r += b[q] < mine ? 1 : 0;
+            int r = 0;
+            if (sz > 1) {
+                const unsigned long long mine = ((unsigned long long)ok[i] << 32) | (unsigned)e;
+                const unsigned long long* b = s.items + st;
+                const unsigned long long x0 = b[0];
+                const unsigned long long x1 = b[1];
+                const unsigned long long x2 = sz > 2 ? b[2] : ~0ull;
+                const unsigned long long x3 = sz > 3 ? b[3] : ~0ull;
+                r = (x0 < mine) + (x1 < mine) + (x2 < mine) + (x3 < mine);
+                for (int q = 4; q < sz; ++q) r += b[q] < mine ? 1 : 0;
+            }
             pos[i] = st + r;
             if (perm) perm[st + r] = e;
         }
@@ -456,11 +475,13 @@ __host__ __device__ inline int pow2_at_least(int v)
     return p;
 }
 
-// Buckets used to argsort n keys: the power of two >= n / 2, at least 64.
+// Buckets used to argsort n keys: the power of two >= 2n, at least 64 and
+// at most 2048.  With about half a key per bucket most keys rank without
+// reading any other item.
 __host__ __device__ inline int sort_buckets(int n)
 {
-    const int p = pow2_at_least((n + 1) / 2);
-    return p < 64 ? 64 : p;
+    const int p = pow2_at_least(2 * n);
+    return p < 64 ? 64 : (p > 2048 ? 2048 : p);
 }
 
 }  // namespace brkga

@@ -12,6 +12,12 @@ namespace {
 
 constexpr int kQueueStride = 32;  // ints between two buckets' counters
 constexpr int kBlocks[] = {64, 128, 256, 512, 1024};
+
+// Lane balancing charges a child n + kChildFixedWork: the build, the
+// barriers and the reductions of a child cost about as much as ranking a
+// thousand keys, so buckets of small instances take longer than their key
+// count says.
+constexpr double kChildFixedWork = 1024.0;
 constexpr int kChunks[] = {1, 2, 4, 8, 16};
 
 // Default slots per work item: enough decode work per item to hide the
@@ -130,7 +136,7 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
             }
             FleetBucket& b = buckets_.back();
             ++b.count;
-            b.work += (double)d.pop * n;
+            b.work += (double)d.pop * (n + kChildFixedWork);
             b.n_min = std::min(b.n_min, n);
             b.n_max = std::max(b.n_max, n);
             b.max_Pi = std::max(b.max_Pi, d.Pi);

@@ -101,16 +101,25 @@ __global__ __launch_bounds__(BLOCK) void k_step_one(InstDesc d, int cur, int ini
 
 constexpr int kPartBlock = 1024;
 
+// Workgroups that rank one island together: islands of 128 to kPartBlock
+// individuals are split into slices of 64 keys, one workgroup each.
+__host__ __device__ inline int partition_parts(int Pi)
+{
+    const int Pn = pow2_at_least(Pi < 2 ? 2 : Pi);
+    return (Pn < 128 || Pn > kPartBlock) ? 1 : Pn / 64;
+}
+
 // Stable rank order of island `island`'s fitness in buffer `buf`.  Up to
 // kPartBlock individuals: every key (fitness order key, index) is unique,
-// so the rank of a key is the number of smaller keys; G = kPartBlock / Pn
-// threads count one key's rank over interleaved quarters and combine with
-// wave shuffles.  Larger islands use a bitonic network.  mode 1: the last
-// island block of the population bumps the generation counter to t + 1 and
+// so the rank of a key is the number of smaller keys; workgroup `part` of
+// `parts` ranks the keys of its slice, G = kPartBlock / slice threads per
+// key counting over interleaved shares and combining with wave shuffles.
+// Larger islands use a bitonic network (parts = 1).  mode 1: the last
+// workgroup of the population bumps the generation counter to t + 1 and
 // raises the local-search flag when generation t + 1 is due; mode 2: the
-// last block lowers the flag.
-__device__ __forceinline__ void partition_body(const InstDesc& d, int island, int buf, int mode, int t,
-                                               char* lds)
+// last workgroup lowers the flag.
+__device__ __forceinline__ void partition_body(const InstDesc& d, int island, int part, int parts, int buf,
+                                               int mode, int t, char* lds)
 {
     constexpr int B = kPartBlock;
     unsigned long long* sk = reinterpret_cast<unsigned long long*>(lds);
@@ -125,8 +134,9 @@ __device__ __forceinline__ void partition_body(const InstDesc& d, int island, in
                        : ~0ull;
     __syncthreads();
     if (Pn <= B) {
-        const int G = B / Pn;  // threads per key, power of two <= 512
-        const int e = threadIdx.x / G, sub = threadIdx.x % G;
+        const int slice = Pn / parts;
+        const int G = B / slice;  // threads per key, power of two <= 512
+        const int e = part * slice + threadIdx.x / G, sub = threadIdx.x % G;
         const unsigned long long mine = sk[e];
         const ulonglong2* pairs = reinterpret_cast<const ulonglong2*>(sk);
         int cnt = 0;
@@ -172,7 +182,7 @@ __device__ __forceinline__ void partition_body(const InstDesc& d, int island, in
         __syncthreads();
         if (threadIdx.x == 0) {
             const int old = atomicAdd(d.done, 1);
-            if (old == d.islands - 1) {
+            if (old == d.islands * parts - 1) {
                 *d.done = 0;
                 if (mode == 1) {
                     const int g = t + 1;
@@ -186,9 +196,11 @@ __device__ __forceinline__ void partition_body(const InstDesc& d, int island, in
     }
 }
 
+// Grid: islands x partition_parts(Pi) workgroups.
 __global__ __launch_bounds__(kPartBlock) void k_partition_one(InstDesc d, int buf, int mode)
 {
     extern __shared__ __align__(16) char lds[];
+    const int parts = partition_parts(d.Pi);
     int t = 0;
     if (mode == 1) {
         t = *d.gen;
@@ -196,7 +208,7 @@ __global__ __launch_bounds__(kPartBlock) void k_partition_one(InstDesc d, int bu
         if (*d.ls_flag == 0) return;
         buf = *d.gen & 1;
     }
-    partition_body(d, blockIdx.x, buf, mode, t, lds);
+    partition_body(d, blockIdx.x / parts, blockIdx.x % parts, parts, buf, mode, t, lds);
 }
 
 __global__ __launch_bounds__(kPartBlock) void k_partition_many(const InstDesc* __restrict__ descs, int num_inst,
@@ -219,7 +231,7 @@ __global__ __launch_bounds__(kPartBlock) void k_partition_many(const InstDesc* _
     } else if (b < 0) {
         b = 0;
     }
-    partition_body(d, island, b, mode, t, lds);
+    partition_body(d, island, 0, 1, b, mode, t, lds);
 }
 
 // TSPJ local search: one workgroup per (population, island, rank r < ls_k)
@@ -418,7 +430,8 @@ hipError_t launch_partition_one(const InstDesc& d, int buf, int mode, hipStream_
 {
     const int lds = partition_lds_bytes(d.Pi);
     ensure_lds(k_partition_one, lds);
-    hipLaunchKernelGGL(k_partition_one, dim3(d.islands), dim3(kPartBlock), lds, s, d, buf, mode);
+    hipLaunchKernelGGL(k_partition_one, dim3(d.islands * partition_parts(d.Pi)), dim3(kPartBlock), lds, s, d, buf,
+                       mode);
     return hipGetLastError();
 }
 
