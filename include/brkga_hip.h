@@ -58,8 +58,16 @@ typedef enum {
     BRKGA_KNAPSACK_GREEDY = 3,  /* greedy fill in key order (no reference counterpart) */
     BRKGA_TSPJ = 4,             /* configs/tspj_config.hpp:42-65, 555-642 */
     BRKGA_BITSP_COORDS = 5,     /* configs/multi_tsp_config.hpp, two coordinate sets */
-    BRKGA_BITSP_MATRIX = 6      /* configs/multi_tsp_config.hpp:25-52 (two matrices) */
+    BRKGA_BITSP_MATRIX = 6,     /* configs/multi_tsp_config.hpp:25-52 (two matrices) */
+    BRKGA_CONTINUOUS = 7,       /* configs/continuous_opt_gpu_config.hpp:28-49, 116-160 */
+    BRKGA_JOB_SCHEDULING = 8,   /* configs/job_scheduling_gpu_config.hpp:28-52, 138-158 */
+    BRKGA_ZDT1 = 9,             /* examples/zdt1.cu:14-63 (two objectives) */
+    BRKGA_VRP_DAYS = 10,        /* configs/vrp_blazer_config.hpp:52-296 */
+    BRKGA_MOVRP = 11            /* configs/multi_objective_vrp_config.hpp:57-300 (two objectives) */
 } brkga_problem_kind;
+
+/* Continuous test functions (BRKGA_CONTINUOUS). */
+typedef enum { BRKGA_SPHERE = 0, BRKGA_RASTRIGIN = 1, BRKGA_ROSENBROCK = 2 } brkga_continuous_fn;
 
 typedef struct brkga_engine_s* brkga_engine_t;
 typedef struct brkga_problem_s* brkga_problem_t;
@@ -140,6 +148,49 @@ brkga_status brkga_problem_create_bitsp_coords(brkga_engine_t engine, const floa
                                                int32_t n, brkga_problem_t* out);
 brkga_status brkga_problem_create_bitsp_matrix(brkga_engine_t engine, const float* m0, const float* m1,
                                                int32_t dim, brkga_problem_t* out);
+
+/* Continuous minimisation over n variables x_i = key_i * (upper - lower) +
+ * lower (FP32): sphere sum x^2, Rastrigin 10 n + sum (x^2 - 10 cos(2 pi x)),
+ * Rosenbrock sum 100 (x_{i+1} - x_i^2)^2 + (1 - x_i)^2
+ * (ContinuousOptGPUConfig, configs/continuous_opt_gpu_config.hpp:28-49,
+ * 116-160; the reference evaluates Rosenbrock on the host only).  Decoded
+ * row: the FP32 bits of each x_i. */
+brkga_status brkga_problem_create_continuous(brkga_engine_t engine, int32_t function, int32_t n, float lower,
+                                             float upper, brkga_problem_t* out);
+
+/* Job scheduling: job i (time job_times[i]) runs on machine int(key_i *
+ * machines) (clamped to [0, machines)); fitness = largest machine load,
+ * minimised (JobSchedulingGPUConfig, configs/job_scheduling_gpu_config.hpp:
+ * 28-52, 138-158).  1 <= machines <= 4096.  Decoded row: machine per job. */
+brkga_status brkga_problem_create_job_scheduling(brkga_engine_t engine, const float* job_times, int32_t n,
+                                                 int32_t machines, brkga_problem_t* out);
+
+/* ZDT1 over n variables (the keys): objectives f1 = x_1 and f2 = g (1 -
+ * sqrt(f1 / g)), g = 1 + 9 sum(x_2..x_n) / (n - 1), both minimised; runs
+ * on NSGA-II (ZDT1Config, examples/zdt1.cu:14-63).  Decoded row: the FP32
+ * bits of each x_i. */
+brkga_status brkga_problem_create_zdt1(brkga_engine_t engine, int32_t n, brkga_problem_t* out);
+
+/* Multi-day routes (VrpBlazerConfig, configs/vrp_blazer_config.hpp:
+ * 52-296): time_matrix[dim][dim] with location 0 the depot and n = dim - 1
+ * stops.  Stops in key order fill days of at most max_stops_per_day stops
+ * whose drive time (legs plus return to the depot) stays within
+ * max_drive_minutes; fitness = total drive minutes + 1000 per entry of the
+ * reference's unvisited list (which holds every skipped stop twice),
+ * minimised.  Decoded row: day (1-based) of each stop, 0 if skipped. */
+brkga_status brkga_problem_create_vrp_days(brkga_engine_t engine, const float* time_matrix, int32_t dim,
+                                           int32_t max_stops_per_day, double max_drive_minutes,
+                                           brkga_problem_t* out);
+
+/* Bi-objective capacitated VRP (MultiObjectiveVRPConfig,
+ * configs/multi_objective_vrp_config.hpp:57-300): xy[dim][2] and
+ * demands[dim] with location 0 the depot and n = dim - 1 customers.
+ * Customers in biased key order split greedily into routes by capacity;
+ * objectives: total distance (+100 per unit of overload) and the longest
+ * route, each +500 per route beyond `vehicles`; runs on NSGA-II.  Decoded
+ * row: route (1-based) of each customer. */
+brkga_status brkga_problem_create_movrp(brkga_engine_t engine, const float* xy, const int32_t* demands, int32_t dim,
+                                        int64_t capacity, int32_t vehicles, brkga_problem_t* out);
 
 /* Kind, chromosome length, objectives and direction of a problem. */
 brkga_status brkga_problem_info(brkga_problem_t problem, int32_t* kind, int32_t* chrom_len, int32_t* num_obj,

@@ -41,6 +41,15 @@ KNAPSACK_GREEDY = 3
 TSPJ = 4
 BITSP_COORDS = 5
 BITSP_MATRIX = 6
+CONTINUOUS = 7
+JOB_SCHEDULING = 8
+ZDT1 = 9
+VRP_DAYS = 10
+MOVRP = 11
+
+SPHERE = 0
+RASTRIGIN = 1
+ROSENBROCK = 2
 
 
 class BrkgaError(RuntimeError):
@@ -125,6 +134,12 @@ _SIGNATURES = {
     "brkga_problem_create_tspj": ([_P, _PF, _PF, _I32, ctypes.POINTER(_P)], ctypes.c_int),
     "brkga_problem_create_bitsp_coords": ([_P, _PF, _PF, _I32, ctypes.POINTER(_P)], ctypes.c_int),
     "brkga_problem_create_bitsp_matrix": ([_P, _PF, _PF, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_problem_create_continuous": ([_P, _I32, _I32, ctypes.c_float, ctypes.c_float, ctypes.POINTER(_P)],
+                                        ctypes.c_int),
+    "brkga_problem_create_job_scheduling": ([_P, _PF, _I32, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_problem_create_zdt1": ([_P, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_problem_create_vrp_days": ([_P, _PF, _I32, _I32, ctypes.c_double, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_problem_create_movrp": ([_P, _PF, _PI, _I32, ctypes.c_int64, _I32, ctypes.POINTER(_P)], ctypes.c_int),
     "brkga_problem_info": ([_P, _PI, _PI, _PI, _PI], ctypes.c_int),
     "brkga_problem_destroy": ([_P], ctypes.c_int),
     "brkga_evaluate": ([_P, _P, _P, _I32, _I32, _P], ctypes.c_int),
@@ -277,6 +292,36 @@ class Problem:
         a = np.ascontiguousarray(m0, dtype=np.float32)
         b = np.ascontiguousarray(m1, dtype=np.float32)
         return cls._make(engine, lib().brkga_problem_create_bitsp_matrix, _fptr(a), _fptr(b), a.shape[0])
+
+    @classmethod
+    def continuous(cls, engine, function, n, lower, upper):
+        """ContinuousOptGPUConfig: function SPHERE / RASTRIGIN / ROSENBROCK over n variables."""
+        return cls._make(engine, lib().brkga_problem_create_continuous, int(function), int(n), float(lower),
+                         float(upper))
+
+    @classmethod
+    def job_scheduling(cls, engine, job_times, machines):
+        t = np.ascontiguousarray(job_times, dtype=np.float32)
+        return cls._make(engine, lib().brkga_problem_create_job_scheduling, _fptr(t), t.shape[0], int(machines))
+
+    @classmethod
+    def zdt1(cls, engine, n):
+        return cls._make(engine, lib().brkga_problem_create_zdt1, int(n))
+
+    @classmethod
+    def vrp_days(cls, engine, time_matrix, max_stops_per_day, max_drive_minutes):
+        """VrpBlazerConfig: time_matrix[dim][dim], location 0 the depot."""
+        t = np.ascontiguousarray(time_matrix, dtype=np.float32)
+        return cls._make(engine, lib().brkga_problem_create_vrp_days, _fptr(t), t.shape[0], int(max_stops_per_day),
+                         float(max_drive_minutes))
+
+    @classmethod
+    def movrp(cls, engine, xy, demands, capacity, vehicles):
+        """MultiObjectiveVRPConfig: xy[dim][2], demands[dim], location 0 the depot."""
+        xy = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1, 2)
+        d = np.ascontiguousarray(demands, dtype=np.int32)
+        return cls._make(engine, lib().brkga_problem_create_movrp, _fptr(xy), _iptr(d), xy.shape[0], int(capacity),
+                         int(vehicles))
 
     def evaluate_device(self, d_keys: int, d_fitness: int, pop_size: int, stream: int = 0):
         """evaluate_population_gpu on caller-owned device pointers (ints)."""

@@ -16,6 +16,7 @@ namespace {
 thread_local std::string g_last_error;
 
 constexpr int kMaxGenes = 16000;   // fixed-point sums hold 2^14 terms
+constexpr int kMaxMachines = 4096; // job-scheduling machine loads live in LDS
 constexpr int kMaxIslandPop = 16384;
 constexpr int kMaxNsgaPop = 2048;  // parents + children index in 12 bits
 
@@ -73,9 +74,9 @@ void free_all(std::vector<void*>& v)
     v.clear();
 }
 
-brkga_status check_lds(const brkga_engine_s* eng, int kind, int n, int L)
+brkga_status check_lds(const brkga_engine_s* eng, const ProblemDesc& p)
 {
-    const int bytes = decode_layout(kind, n, L, sort_buckets(n)).bytes;
+    const int bytes = decode_layout(p.kind, p.n, p.chrom_len, sort_buckets(p.n), p.aux).bytes;
     if ((size_t)bytes > eng->lds_limit)
         return set_error(BRKGA_ERR_INVALID_ARG, "chromosome too long: decode needs " + std::to_string(bytes) +
                                                     " bytes of LDS, the device allows " +
@@ -93,20 +94,18 @@ brkga_status new_problem(brkga_engine_t engine, int kind, int n, brkga_problem_s
     std::memset(&pr->desc, 0, sizeof(pr->desc));
     pr->desc.kind = kind;
     pr->desc.n = n;
-    pr->desc.chrom_len = kind == KIND_TSPJ ? 2 * n : n;
-    pr->desc.num_obj = (kind == KIND_BITSP_COORDS || kind == KIND_BITSP_MATRIX) ? 2 : 1;
+    pr->desc.chrom_len = kind_chrom_len(kind, n);
+    pr->desc.num_obj = kind_num_obj(kind);
     pr->desc.maximize = (kind == KIND_KNAP_THRESHOLD || kind == KIND_KNAP_GREEDY) ? 1 : 0;
-    brkga_status st = check_lds(engine, kind, n, pr->desc.chrom_len);
-    if (st != BRKGA_OK) {
-        delete pr;
-        return st;
-    }
     *out = pr;
     return BRKGA_OK;
 }
 
+// Checks that the decoder fits in LDS, then hands the problem out (or frees
+// it on any error).
 brkga_status finish_problem(brkga_problem_s* pr, brkga_status st, brkga_problem_t* out)
 {
+    if (st == BRKGA_OK) st = check_lds(pr->engine, pr->desc);
     if (st != BRKGA_OK) {
         free_all(pr->owned);
         delete pr;
@@ -324,7 +323,7 @@ brkga_status create_population(brkga_problem_s* problem, const brkga_params* par
         const int n = problem->desc.n;
         pop->block = default_block(n);
         pop->host_t = 0;
-        pop->step_lds = pop->nsga2 ? decode_layout(problem->desc.kind, n, pop->host.L, pop->host.nb).bytes
+        pop->step_lds = pop->nsga2 ? decode_layout(problem->desc.kind, n, pop->host.L, pop->host.nb, problem->desc.aux).bytes
                                    : step_lds_bytes(pop->host);
         if ((size_t)partition_lds_bytes(pop->host.Pi) > problem->engine->lds_limit ||
             (size_t)pop->step_lds > problem->engine->lds_limit)
@@ -592,6 +591,109 @@ brkga_status brkga_problem_create_bitsp_matrix(brkga_engine_t engine, const floa
     set_scales(&pr->desc, fixed_shift(max_abs(m0, cells)), fixed_shift(max_abs(m1, cells)));
     st = upload(pr, m0, sizeof(float) * cells, (const void**)&pr->desc.a);
     if (st == BRKGA_OK) st = upload(pr, m1, sizeof(float) * cells, (const void**)&pr->desc.b);
+    return finish_problem(pr, st, out);
+}
+
+brkga_status brkga_problem_create_continuous(brkga_engine_t engine, int32_t function, int32_t n, float lower,
+                                             float upper, brkga_problem_t* out)
+{
+    if (function < FN_SPHERE || function > FN_ROSENBROCK)
+        return set_error(BRKGA_ERR_INVALID_ARG, "function must be 0 (sphere), 1 (rastrigin) or 2 (rosenbrock)");
+    if (!std::isfinite(lower) || !std::isfinite(upper)) return set_error(BRKGA_ERR_INVALID_ARG, "non-finite bounds");
+    brkga_problem_s* pr;
+    brkga_status st = new_problem(engine, KIND_CONTINUOUS, n, &pr);
+    if (st != BRKGA_OK) return st;
+    pr->desc.dim = n;
+    pr->desc.ip0 = function;
+    pr->desc.f0 = lower;
+    pr->desc.f1 = upper;
+    const double m = std::max(std::fabs((double)lower), std::fabs((double)upper));
+    const double bound = function == FN_SPHERE ? m * m
+                         : function == FN_RASTRIGIN ? m * m + 10.0
+                                                    : 100.0 * (m + m * m) * (m + m * m) + (1.0 + m) * (1.0 + m);
+    set_scales(&pr->desc, fixed_shift(bound * (1.0 + 1e-6)), 30);
+    return finish_problem(pr, BRKGA_OK, out);
+}
+
+brkga_status brkga_problem_create_job_scheduling(brkga_engine_t engine, const float* job_times, int32_t n,
+                                                 int32_t machines, brkga_problem_t* out)
+{
+    if (!job_times) return set_error(BRKGA_ERR_INVALID_ARG, "null job times");
+    if (machines < 1 || machines > kMaxMachines)
+        return set_error(BRKGA_ERR_INVALID_ARG, "machines must be in [1, " + std::to_string(kMaxMachines) + "]");
+    if (n > 0 && !all_finite(job_times, (size_t)n)) return set_error(BRKGA_ERR_INVALID_ARG, "non-finite job times");
+    brkga_problem_s* pr;
+    brkga_status st = new_problem(engine, KIND_JOBSHOP, n, &pr);
+    if (st != BRKGA_OK) return st;
+    DeviceGuard g(engine->device);
+    pr->desc.dim = n;
+    pr->desc.ip0 = machines;
+    pr->desc.aux = 8 * machines;
+    set_scales(&pr->desc, fixed_shift(max_abs(job_times, (size_t)n)), 30);
+    st = upload(pr, job_times, sizeof(float) * (size_t)n, (const void**)&pr->desc.a);
+    return finish_problem(pr, st, out);
+}
+
+brkga_status brkga_problem_create_zdt1(brkga_engine_t engine, int32_t n, brkga_problem_t* out)
+{
+    brkga_problem_s* pr;
+    brkga_status st = new_problem(engine, KIND_ZDT1, n, &pr);
+    if (st != BRKGA_OK) return st;
+    pr->desc.dim = n;
+    set_scales(&pr->desc, fixed_shift(1.0), 30);
+    return finish_problem(pr, BRKGA_OK, out);
+}
+
+brkga_status brkga_problem_create_vrp_days(brkga_engine_t engine, const float* time_matrix, int32_t dim,
+                                           int32_t max_stops_per_day, double max_drive_minutes,
+                                           brkga_problem_t* out)
+{
+    if (!time_matrix) return set_error(BRKGA_ERR_INVALID_ARG, "null time matrix");
+    if (dim < 2) return set_error(BRKGA_ERR_INVALID_ARG, "VRP needs dim >= 2 (depot + one stop)");
+    const size_t cells = (size_t)dim * dim;
+    if (!all_finite(time_matrix, cells)) return set_error(BRKGA_ERR_INVALID_ARG, "non-finite matrix entries");
+    if (!std::isfinite(max_drive_minutes)) return set_error(BRKGA_ERR_INVALID_ARG, "non-finite drive limit");
+    brkga_problem_s* pr;
+    brkga_status st = new_problem(engine, KIND_VRP_DAYS, dim - 1, &pr);
+    if (st != BRKGA_OK) return st;
+    DeviceGuard g(engine->device);
+    const int n = dim - 1;
+    pr->desc.dim = dim;
+    pr->desc.ip0 = max_stops_per_day;
+    pr->desc.d0 = max_drive_minutes;
+    pr->desc.aux = 8 * n + 16;
+    const int per_day = std::max(0, std::min(max_stops_per_day, n));
+    const double bound = std::max((per_day + 1.0) * max_abs(time_matrix, cells), 2000.0 * n);
+    set_scales(&pr->desc, fixed_shift(bound), 30);
+    st = upload(pr, time_matrix, sizeof(float) * cells, (const void**)&pr->desc.a);
+    return finish_problem(pr, st, out);
+}
+
+brkga_status brkga_problem_create_movrp(brkga_engine_t engine, const float* xy, const int32_t* demands, int32_t dim,
+                                        int64_t capacity, int32_t vehicles, brkga_problem_t* out)
+{
+    if (!xy || !demands) return set_error(BRKGA_ERR_INVALID_ARG, "null coordinates or demands");
+    if (dim < 2) return set_error(BRKGA_ERR_INVALID_ARG, "VRP needs dim >= 2 (depot + one customer)");
+    if (!all_finite(xy, (size_t)2 * dim)) return set_error(BRKGA_ERR_INVALID_ARG, "non-finite coordinates");
+    if (vehicles < 0) return set_error(BRKGA_ERR_INVALID_ARG, "vehicles must be >= 0");
+    int32_t max_dem = 0;
+    for (int32_t i = 0; i < dim; ++i) {
+        if (demands[i] < 0) return set_error(BRKGA_ERR_INVALID_ARG, "demands must be >= 0");
+        max_dem = std::max(max_dem, demands[i]);
+    }
+    brkga_problem_s* pr;
+    brkga_status st = new_problem(engine, KIND_MOVRP, dim - 1, &pr);
+    if (st != BRKGA_OK) return st;
+    DeviceGuard g(engine->device);
+    const int n = dim - 1;
+    pr->desc.dim = dim;
+    pr->desc.cap = capacity;
+    pr->desc.ip0 = vehicles;
+    pr->desc.aux = 4 * round_up_int(n + 1, 2) + 16 * n + 16;
+    const double bound = std::max(coords_diameter(xy, dim), std::max(100.0 * max_dem, 500.0 * n));
+    set_scales(&pr->desc, fixed_shift(bound), fixed_shift(bound));
+    st = upload(pr, xy, sizeof(float) * 2 * (size_t)dim, (const void**)&pr->desc.a);
+    if (st == BRKGA_OK) st = upload(pr, demands, sizeof(int32_t) * (size_t)dim, (const void**)&pr->desc.w);
     return finish_problem(pr, st, out);
 }
 

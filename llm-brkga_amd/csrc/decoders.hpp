@@ -240,6 +240,227 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
         }
         if (decoded)
             for (int q = tid; q < n; q += BLOCK) decoded[q] = perm[q];
+    } else if constexpr (KIND == KIND_CONTINUOUS) {
+        // sum of per-variable terms (continuous_opt_gpu_config.hpp:132-160);
+        // Rastrigin adds the constant 10 n exactly
+        const float lo = p.f0, hi = p.f1;
+        const int fn = p.ip0;
+        long long acc = 0;
+        for (int i = tid; i < n; i += BLOCK) {
+#pragma clang fp contract(off)
+            const float x = cont_var(keys[i], lo, hi);
+            if (decoded) decoded[i] = __float_as_int(x);
+            float t;
+            if (fn == FN_SPHERE) {
+                t = x * x;
+            } else if (fn == FN_RASTRIGIN) {
+                const float c = cos_det(6.2831854820251465f * x);
+                t = x * x - 10.0f * c;
+            } else {
+                if (i + 1 >= n) continue;
+                const float xn = cont_var(keys[i + 1], lo, hi);
+                const float t1 = xn - x * x;
+                const float t2 = 1.0f - x;
+                t = (100.0f * t1) * t1 + t2 * t2;
+            }
+            acc += fx_term(t, p.scale0);
+        }
+        acc = block_sum_i64<BLOCK>(acc, red);
+        if (tid == 0 && fit_out) {
+            if (fn == FN_RASTRIGIN) acc += fx_term(10.0f * (float)n, p.scale0);
+            fit_out[0] = fx_to_float(acc, p.inv0);
+        }
+    } else if constexpr (KIND == KIND_JOBSHOP) {
+        // machine loads in LDS bins, makespan = largest load, empty machines
+        // count as 0 (job_scheduling_gpu_config.hpp:138-158)
+        long long* bins = reinterpret_cast<long long*>(lds + lay.aux_off);
+        const int M = p.ip0;
+        for (int m = tid; m < M; m += BLOCK) bins[m] = 0;
+        __syncthreads();
+        for (int i = tid; i < n; i += BLOCK) {
+            const int m = job_machine(keys[i], M);
+            atomicAdd(reinterpret_cast<unsigned long long*>(&bins[m]),
+                      (unsigned long long)fx_term(p.a[i], p.scale0));
+            if (decoded) decoded[i] = m;
+        }
+        __syncthreads();
+        long long best = 0;
+        for (int m = tid; m < M; m += BLOCK) best = bins[m] > best ? bins[m] : best;
+        best = block_max_i64<BLOCK>(best, red);
+        if (tid == 0 && fit_out) fit_out[0] = fx_to_float(best, p.inv0);
+    } else if constexpr (KIND == KIND_ZDT1) {
+        // f1 = x1, g = 1 + 9 sum(x2..xn) / (n - 1), f2 = g (1 - sqrt(f1 / g))
+        // (examples/zdt1.cu:43-63); the key sum is exact fixed point
+        long long acc = 0;
+        for (int i = tid; i < n; i += BLOCK) {
+            if (i > 0) acc += fx_term(keys[i], p.scale0);
+            if (decoded) decoded[i] = __float_as_int(keys[i]);
+        }
+        acc = block_sum_i64<BLOCK>(acc, red);
+        if (tid == 0 && fit_out) {
+            const float f1 = keys[0];
+            float g = 1.0f;
+            if (n > 1) {
+                const double sum = (double)acc * (double)p.inv0;
+                g = (float)(1.0 + 9.0 * sum / (double)(n - 1));
+            }
+            const float r = sqrt_rn(__fdiv_rn(f1, g));
+            fit_out[0] = f1;
+            fit_out[1] = (float)((double)g * (1.0 - (double)r));
+        }
+    } else if constexpr (KIND == KIND_VRP_DAYS) {
+        // Stops in key order fill day routes from the depot
+        // (vrp_blazer_config.hpp:192-282): a stop joins the day while the day
+        // has fewer than ip0 stops and cumulative drive + leg + return stays
+        // within d0 minutes (FP32 sums, compared in FP64); a stop that fits
+        // no empty day is skipped.  Every start position walks its own day
+        // in parallel; one thread then chains the days from position 0.
+        // Fitness = sum of day drive times + 1000 per entry of the
+        // reference's unvisited list, which holds each skipped stop twice
+        // (vrp_blazer_config.hpp:265-280, 285-296).
+        block_argsort<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, perm);
+        long long* dfx = reinterpret_cast<long long*>(lds + lay.items_off);
+        int* nxt = reinterpret_cast<int*>(lds + lay.aux_off);
+        int* dnum = nxt + n;
+        const int dim = p.dim, maxs = p.ip0;
+        const double lim = p.d0;
+        for (int q = tid; q < n; q += BLOCK) {
+#pragma clang fp contract(off)
+            int cur = 0, k = 0;
+            float cum = 0.0f;
+            while (q + k < n && k < maxs) {
+                const int st = perm[q + k] + 1;
+                const float tts = p.a[(size_t)cur * dim + st];
+                const float ttd = p.a[(size_t)st * dim];
+                const float tot = (cum + tts) + ttd;
+                if (!((double)tot <= lim)) break;
+                cum = cum + tts;
+                cur = st;
+                ++k;
+            }
+            nxt[q] = k > 0 ? q + k : -(q + 1);
+            dfx[q] = k > 0 ? fx_term(cum + p.a[(size_t)cur * dim], p.scale0) : 0;
+            dnum[q] = 0;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            long long acc = 0;
+            int skipped = 0, day = 0;
+            for (int q = 0; q < n;) {
+                const int nx = nxt[q];
+                if (nx < 0) {
+                    ++skipped;
+                    dnum[q] = -1;
+                    q = -nx;
+                } else {
+                    acc += dfx[q];
+                    dnum[q] = ++day;
+                    q = nx;
+                }
+            }
+            acc += fx_term(2000.0f * (float)skipped, p.scale0);
+            if (fit_out) fit_out[0] = fx_to_float(acc, p.inv0);
+        }
+        if (decoded) {
+            __syncthreads();
+            // the day of a position is that of the nearest chained start at
+            // or before it whose day reaches it
+            for (int q = tid; q < n; q += BLOCK) {
+                int dv = 0;
+                if (dnum[q] >= 0) {
+                    for (int s0 = q; s0 >= 0 && s0 > q - maxs; --s0)
+                        if (dnum[s0] > 0 && nxt[s0] > q) {
+                            dv = dnum[s0];
+                            break;
+                        }
+                }
+                decoded[perm[q]] = dv;
+            }
+        }
+    } else if constexpr (KIND == KIND_MOVRP) {
+        // Customers in biased key order (the bias adds i 1e-5 and a running
+        // FP32 hash of the keys, multi_objective_vrp_config.hpp:216-228)
+        // split greedily into routes by vehicle capacity (a route always
+        // takes its first customer, 233-254).  Objectives: total distance +
+        // 100 per unit of overload, and the longest route; routes beyond the
+        // vehicle count cost 500 each in both (272-300).
+        float* vals = keys;
+        int* dem = reinterpret_cast<int*>(lds + lay.aux_off);
+        int* nxt = dem + round_up_int(n + 1, 2);
+        int* rid = nxt + n;
+        long long* rload = reinterpret_cast<long long*>(rid + n);
+        __shared__ float s_hash;
+        for (int i = tid; i <= n; i += BLOCK) dem[i] = p.w[i];
+        if (tid == 0) {
+#pragma clang fp contract(off)
+            float h = 0.0f;
+            for (int i = 0; i < n; ++i) h = h + keys[i] * (float)(i + 1);
+            s_hash = h;
+        }
+        __syncthreads();
+        {
+#pragma clang fp contract(off)
+            const float hb = s_hash * 0.00001f;
+            for (int i = tid; i < n; i += BLOCK) vals[i] = keys[i] + ((float)i * 0.00001f + hb);
+        }
+        if (CLEARED) __syncthreads();
+        block_argsort<BLOCK, CLEARED, ITEMS>(vals, n, nb, ss, perm);
+        const long long cap = p.cap;
+        for (int q = tid; q < n; q += BLOCK) {
+            long long load = 0;
+            int k = 0;
+            while (q + k < n) {
+                const long long dd = dem[perm[q + k] + 1];
+                if (k > 0 && load + dd > cap) break;
+                load += dd;
+                ++k;
+            }
+            nxt[q] = q + k;
+            rload[q] = load;
+            rid[q] = 0;
+        }
+        long long* rsum = reinterpret_cast<long long*>(lds + lay.items_off);
+        for (int q = tid; q < n; q += BLOCK) rsum[q] = 0;
+        __syncthreads();
+        if (tid == 0) {
+            int r = 0;
+            for (int q = 0; q < n; q = nxt[q]) rid[q] = ++r;
+            red[16] = r;
+        }
+        __syncthreads();
+        const int routes = (int)red[16];
+        // route of each position: count of route starts up to it
+        const int chunk = (n + BLOCK - 1) / BLOCK;
+        const int lo = tid * chunk < n ? tid * chunk : n;
+        const int hi = lo + chunk < n ? lo + chunk : n;
+        long long starts = 0;
+        for (int q = lo; q < hi; ++q) starts += rid[q] > 0 ? 1 : 0;
+        long long run = block_inclusive_scan_i64<BLOCK>(starts, red) - starts;
+        const float2* xy = reinterpret_cast<const float2*>(p.a);
+        long long tot = 0, pen = 0;
+        for (int q = lo; q < hi; ++q) {
+            if (rid[q] > 0) {
+                ++run;
+                if (rload[q] > cap) pen += fx_term((float)(rload[q] - cap) * 100.0f, p.scale0);
+            }
+            const int c = perm[q] + 1;
+            const int prev = rid[q] > 0 ? 0 : perm[q - 1] + 1;
+            long long leg = fx_term(edge_xy(xy[prev].x, xy[prev].y, xy[c].x, xy[c].y), p.scale0);
+            if (q == n - 1 || rid[q + 1] > 0) leg += fx_term(edge_xy(xy[c].x, xy[c].y, xy[0].x, xy[0].y), p.scale0);
+            tot += leg;
+            atomicAdd(reinterpret_cast<unsigned long long*>(&rsum[run - 1]), (unsigned long long)leg);
+            if (decoded) decoded[c - 1] = (int)run;
+        }
+        tot = block_sum_i64<BLOCK>(tot + pen, red);
+        __syncthreads();
+        long long best = 0;
+        for (int r = tid; r < routes; r += BLOCK) best = rsum[r] > best ? rsum[r] : best;
+        best = block_max_i64<BLOCK>(best, red);
+        if (tid == 0 && fit_out) {
+            const long long veh = routes > p.ip0 ? fx_term((float)(routes - p.ip0) * 500.0f, p.scale0) : 0;
+            fit_out[0] = fx_to_float(tot + veh, p.inv0);
+            fit_out[1] = fx_to_float(best + veh, p.inv0);
+        }
     }
 }
 

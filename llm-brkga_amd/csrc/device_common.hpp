@@ -126,6 +126,68 @@ __device__ __forceinline__ float edge_xy(float xa, float ya, float xb, float yb)
     return sqrt_rn(sx + sy);
 }
 
+// Deterministic cosine of an FP32 argument, correctly rounded in all but
+// rare cases: Cody-Waite reduction by pi/2 in FP64 (pi/2 split into a
+// 33-bit head and a tail, so k * head is exact for |k| < 2^20), then the
+// Taylor series of sin / cos on [-pi/4, pi/4] to degree 17 / 16 (truncation
+// below 3e-18), all in separately rounded FP64 operations.  The oracle
+// evaluates the identical sequence on the CPU, so both agree bit for bit.
+// |y| < 2^20 * pi/2; larger arguments return cos(0) of an inexact
+// reduction and are not used by any decoder (|y| <= 2 pi * max bound).
+__host__ __device__ inline float cos_det(float y)
+{
+#pragma clang fp contract(off)
+    const double kTwoOverPi = 0.63661977236758134308;
+    const double kPio2Hi = 1.57079632673412561417e+00;
+    const double kPio2Lo = 6.07710050650619224932e-11;
+    const double yd = (double)y;
+    const double k = __builtin_rint(yd * kTwoOverPi);
+    const double r = (yd - k * kPio2Hi) - k * kPio2Lo;
+    const double z = r * r;
+    double c = 1.0 / 20922789888000.0;        // 1/16!
+    c = c * z - 1.0 / 87178291200.0;          // 1/14!
+    c = c * z + 1.0 / 479001600.0;            // 1/12!
+    c = c * z - 1.0 / 3628800.0;              // 1/10!
+    c = c * z + 1.0 / 40320.0;                // 1/8!
+    c = c * z - 1.0 / 720.0;                  // 1/6!
+    c = c * z + 1.0 / 24.0;                   // 1/4!
+    c = c * z - 0.5;
+    c = c * z + 1.0;
+    double sn = 1.0 / 355687428096000.0;      // 1/17!
+    sn = sn * z - 1.0 / 1307674368000.0;      // 1/15!
+    sn = sn * z + 1.0 / 6227020800.0;         // 1/13!
+    sn = sn * z - 1.0 / 39916800.0;           // 1/11!
+    sn = sn * z + 1.0 / 362880.0;             // 1/9!
+    sn = sn * z - 1.0 / 5040.0;               // 1/7!
+    sn = sn * z + 1.0 / 120.0;                // 1/5!
+    sn = sn * z - 1.0 / 6.0;                  // 1/3!
+    sn = (sn * z) * r + r;
+    const int q = (int)((long long)k & 3);
+    const double v = q == 0 ? c : (q == 1 ? -sn : (q == 2 ? -c : sn));
+    return (float)v;
+}
+
+// Decision variable of a continuous gene: gene * (hi - lo) + lo in FP32
+// (configs/continuous_opt_gpu_config.hpp:136-137).
+__host__ __device__ inline float cont_var(float gene, float lo, float hi)
+{
+#pragma clang fp contract(off)
+    const float w = hi - lo;
+    const float t = gene * w;
+    return t + lo;
+}
+
+// Machine of a job gene: int(gene * M) clamped to [0, M)
+// (configs/job_scheduling_gpu_config.hpp:145-146).
+__host__ __device__ inline int job_machine(float gene, int M)
+{
+    const float v = gene * (float)M;
+    if (!(v >= 0.0f)) return 0;
+    if (v >= (float)M) return M - 1;
+    const int m = (int)v;
+    return m < M ? m : M - 1;
+}
+
 // ------------------------------------------------------------ wave / block --
 
 __device__ __forceinline__ long long wave_sum_i64(long long v)

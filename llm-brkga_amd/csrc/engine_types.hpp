@@ -16,8 +16,33 @@ enum ProblemKind : int {
     KIND_TSPJ = 4,            // configs/tspj_config.hpp:555-642
     KIND_BITSP_COORDS = 5,    // configs/multi_tsp_config.hpp, two coordinate sets
     KIND_BITSP_MATRIX = 6,    // configs/multi_tsp_config.hpp, two matrices
-    KIND_COUNT = 7
+    KIND_CONTINUOUS = 7,      // configs/continuous_opt_gpu_config.hpp:116-160
+    KIND_JOBSHOP = 8,         // configs/job_scheduling_gpu_config.hpp:138-158
+    KIND_ZDT1 = 9,            // examples/zdt1.cu:43-63 (two objectives)
+    KIND_VRP_DAYS = 10,       // configs/vrp_blazer_config.hpp:192-296
+    KIND_MOVRP = 11,          // configs/multi_objective_vrp_config.hpp:208-300 (two objectives)
+    KIND_COUNT = 12
 };
+
+// Kinds whose decoder argsorts keys (and so has per-thread key registers).
+__host__ __device__ constexpr bool kind_sorts(int kind)
+{
+    return !(kind == KIND_KNAP_THRESHOLD || kind == KIND_CONTINUOUS || kind == KIND_JOBSHOP || kind == KIND_ZDT1);
+}
+
+// Continuous test functions (KIND_CONTINUOUS, ProblemDesc::ip0).
+enum ContinuousFn : int { FN_SPHERE = 0, FN_RASTRIGIN = 1, FN_ROSENBROCK = 2 };
+
+__host__ __device__ inline int kind_num_obj(int kind)
+{
+    return (kind == KIND_BITSP_COORDS || kind == KIND_BITSP_MATRIX || kind == KIND_ZDT1 || kind == KIND_MOVRP) ? 2
+                                                                                                               : 1;
+}
+
+__host__ __device__ inline int kind_chrom_len(int kind, int n)
+{
+    return kind == KIND_TSPJ ? 2 * n : n;
+}
 
 enum RngTag : uint32_t {
     TAG_PARENTS = 0,
@@ -39,8 +64,13 @@ struct ProblemDesc {
     const float* b; // second matrix / coordinate set
     const int* w;   // knapsack weights
     const int* v;   // knapsack values
-    long long cap;  // knapsack capacity
+    long long cap;  // knapsack capacity / vehicle capacity
     float scale0, inv0, scale1, inv1;  // fixed-point 2^S and 2^-S per objective
+    int ip0, ip1;   // kind parameters: continuous function, machines, stops
+                    // per day, vehicles
+    float f0, f1;   // continuous bounds [f0, f1]
+    double d0;      // VRP days: drive-time limit per day (minutes)
+    int aux;        // extra LDS bytes the decoder needs (machine bins, ...)
 };
 
 // One resident population (a BRKGA run of `islands` islands, or one NSGA-II
@@ -77,7 +107,7 @@ __host__ __device__ inline int round_up_int(int v, int m)
 
 // LDS carve-up of one decode block.
 struct DecodeLayout {
-    int keys_off, perm_off, items_off, cnt_off, red_off, bytes;
+    int keys_off, perm_off, items_off, cnt_off, red_off, aux_off, bytes;
 };
 
 // LDS of a generation-step block: the decode layout, then the staged
@@ -96,7 +126,8 @@ __host__ __device__ inline int coords_bytes(int kind, int n)
     return 0;
 }
 
-__host__ __device__ inline DecodeLayout decode_layout(int kind, int n, int L, int nb)
+// aux: the problem's extra decoder bytes (ProblemDesc::aux), placed last.
+__host__ __device__ inline DecodeLayout decode_layout(int kind, int n, int L, int nb, int aux)
 {
     DecodeLayout d;
     int off = 0;
@@ -115,6 +146,9 @@ __host__ __device__ inline DecodeLayout decode_layout(int kind, int n, int L, in
     off += key_bytes > perm_bytes ? key_bytes : perm_bytes;
     d.cnt_off = off;
     off += (nb + 4) * 4;
+    off = round_up_int(off, 16);
+    d.aux_off = off;
+    off += aux;
     d.bytes = round_up_int(off, 16);
     return d;
 }
@@ -128,10 +162,10 @@ constexpr int kCompactItems = 4;
 // Rank orders longer than this are read from global memory, not staged.
 constexpr int kMaxStagedOrder = 2048;
 
-__host__ __device__ inline StepLayout step_layout(int kind, int n, int L, int nb, int Pi, bool stage_xy = true)
+__host__ __device__ inline StepLayout step_layout(int kind, int n, int L, int nb, int Pi, bool stage_xy, int aux)
 {
     StepLayout s;
-    s.dec = decode_layout(kind, n, L, nb);
+    s.dec = decode_layout(kind, n, L, nb, aux);
     s.xy_off = s.dec.bytes;
     s.isl_off = s.xy_off + (stage_xy ? round_up_int(coords_bytes(kind, n), 16) : 0);
     s.bytes = s.isl_off + round_up_int(4 * (Pi <= kMaxStagedOrder ? Pi : 0), 16);

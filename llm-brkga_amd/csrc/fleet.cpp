@@ -140,7 +140,7 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
             b.n_min = std::min(b.n_min, n);
             b.n_max = std::max(b.n_max, n);
             b.max_Pi = std::max(b.max_Pi, d.Pi);
-            b.lds = std::max(b.lds, step_layout(d.prob.kind, n, d.L, d.nb, d.Pi, b.xys).bytes);
+            b.lds = std::max(b.lds, step_layout(d.prob.kind, n, d.L, d.nb, d.Pi, b.xys, d.prob.aux).bytes);
         } else {
             if (groups_.empty() || groups_.back().kind != d.prob.kind) {
                 FleetGroup gr{};
@@ -349,7 +349,7 @@ static int bucket_lds(const std::vector<InstDesc>& descs, const FleetBucket& b, 
     int lds = 0;
     for (int m = b.first; m < b.first + b.count; ++m) {
         const InstDesc& d = descs[m];
-        lds = std::max(lds, step_layout(d.prob.kind, d.prob.n, d.L, d.nb, d.Pi, xys).bytes);
+        lds = std::max(lds, step_layout(d.prob.kind, d.prob.n, d.L, d.nb, d.Pi, xys, d.prob.aux).bytes);
     }
     return lds;
 }

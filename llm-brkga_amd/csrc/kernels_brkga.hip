@@ -34,7 +34,7 @@ __global__ __launch_bounds__(BLOCK) void k_evaluate(ProblemDesc p, const float* 
     if (row >= count) return;
     const int L = p.chrom_len;
     const int nb = sort_buckets(p.n);
-    const DecodeLayout lay = decode_layout(KIND, p.n, L, nb);
+    const DecodeLayout lay = decode_layout(KIND, p.n, L, nb, p.aux);
     float* k = reinterpret_cast<float*>(lds + lay.keys_off);
     const float* src = keys + (size_t)row * ld;
     for (int j = threadIdx.x; j < L; j += BLOCK) k[j] = src[j];
@@ -72,7 +72,7 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
 
     const int t = init ? 0 : *d.gen;
     const int nb = d.nb;
-    const StepLayout sl = step_layout(KIND, n, L, nb, Pi, false);
+    const StepLayout sl = step_layout(KIND, n, L, nb, Pi, false, d.prob.aux);
     float4* kl = reinterpret_cast<float4*>(lds + sl.dec.keys_off);
     int* order = reinterpret_cast<int*>(lds + sl.isl_off);
     const bool staged = Pi <= kMaxStagedOrder;
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(BLOCK) void k_tspj_local_search(const InstDesc* __r
     const int n = d.prob.n, nb = d.nb;
     const int slot = island * d.Pi + d.perm[b][island * d.Pi + r];
     float* row = d.keys[b] + (size_t)slot * d.ld;
-    const DecodeLayout lay = decode_layout(KIND_TSPJ, n, 2 * n, nb);
+    const DecodeLayout lay = decode_layout(KIND_TSPJ, n, 2 * n, nb, 0);
     float* keys = reinterpret_cast<float*>(lds + lay.keys_off);
     int* tour = reinterpret_cast<int*>(lds + lay.bytes);
     float* vals = reinterpret_cast<float*>(tour + n);
@@ -364,7 +364,7 @@ hipError_t eval_kind(int block, const ProblemDesc& p, const float* keys, long lo
 template <int KIND, int BLOCK>
 hipError_t step_single(const InstDesc& d, int lds, int cur, int init, hipStream_t s)
 {
-    if (d.prob.n <= kCompactItems * BLOCK) {
+    if (kind_sorts(KIND) && d.prob.n <= kCompactItems * BLOCK) {
         ensure_lds(k_step_one<KIND, BLOCK, kCompactItems>, lds);
         hipLaunchKernelGGL((k_step_one<KIND, BLOCK, kCompactItems>), dim3(d.pop), dim3(BLOCK), lds, s, d, cur,
                            init);
@@ -394,7 +394,7 @@ hipError_t launch_evaluate(const ProblemDesc& p, const float* keys, long long ld
                            int* decoded, int block, hipStream_t s)
 {
     if (count <= 0) return hipSuccess;
-    const int lds = decode_layout(p.kind, p.n, p.chrom_len, sort_buckets(p.n)).bytes;
+    const int lds = decode_layout(p.kind, p.n, p.chrom_len, sort_buckets(p.n), p.aux).bytes;
     switch (p.kind) {
     case KIND_TSP_COORDS: return eval_kind<KIND_TSP_COORDS>(block, p, keys, ld, count, fit, decoded, lds, s);
     case KIND_TSP_MATRIX: return eval_kind<KIND_TSP_MATRIX>(block, p, keys, ld, count, fit, decoded, lds, s);
@@ -404,13 +404,18 @@ hipError_t launch_evaluate(const ProblemDesc& p, const float* keys, long long ld
     case KIND_TSPJ: return eval_kind<KIND_TSPJ>(block, p, keys, ld, count, fit, decoded, lds, s);
     case KIND_BITSP_COORDS: return eval_kind<KIND_BITSP_COORDS>(block, p, keys, ld, count, fit, decoded, lds, s);
     case KIND_BITSP_MATRIX: return eval_kind<KIND_BITSP_MATRIX>(block, p, keys, ld, count, fit, decoded, lds, s);
+    case KIND_CONTINUOUS: return eval_kind<KIND_CONTINUOUS>(block, p, keys, ld, count, fit, decoded, lds, s);
+    case KIND_JOBSHOP: return eval_kind<KIND_JOBSHOP>(block, p, keys, ld, count, fit, decoded, lds, s);
+    case KIND_ZDT1: return eval_kind<KIND_ZDT1>(block, p, keys, ld, count, fit, decoded, lds, s);
+    case KIND_VRP_DAYS: return eval_kind<KIND_VRP_DAYS>(block, p, keys, ld, count, fit, decoded, lds, s);
+    case KIND_MOVRP: return eval_kind<KIND_MOVRP>(block, p, keys, ld, count, fit, decoded, lds, s);
     default: return hipErrorInvalidValue;
     }
 }
 
 int step_lds_bytes(const InstDesc& d)
 {
-    return step_layout(d.prob.kind, d.prob.n, d.L, d.nb, d.Pi, false).bytes;
+    return step_layout(d.prob.kind, d.prob.n, d.L, d.nb, d.Pi, false, d.prob.aux).bytes;
 }
 
 hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s)
@@ -422,6 +427,9 @@ hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipS
     case KIND_KNAP_THRESHOLD: return step_kind<KIND_KNAP_THRESHOLD>(block, d, lds, cur, init, s);
     case KIND_KNAP_GREEDY: return step_kind<KIND_KNAP_GREEDY>(block, d, lds, cur, init, s);
     case KIND_TSPJ: return step_kind<KIND_TSPJ>(block, d, lds, cur, init, s);
+    case KIND_CONTINUOUS: return step_kind<KIND_CONTINUOUS>(block, d, lds, cur, init, s);
+    case KIND_JOBSHOP: return step_kind<KIND_JOBSHOP>(block, d, lds, cur, init, s);
+    case KIND_VRP_DAYS: return step_kind<KIND_VRP_DAYS>(block, d, lds, cur, init, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -447,7 +455,7 @@ hipError_t launch_partition_many(const InstDesc* d_descs, int num_inst, int bloc
 
 int tspj_ls_lds_bytes(int n, int nb)
 {
-    return decode_layout(KIND_TSPJ, n, 2 * n, nb).bytes + 12 * n + 16;
+    return decode_layout(KIND_TSPJ, n, 2 * n, nb, 0).bytes + 12 * n + 16;
 }
 
 namespace {

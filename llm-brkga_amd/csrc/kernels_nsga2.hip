@@ -90,7 +90,7 @@ __global__ __launch_bounds__(BLOCK) void k_nsga2_offspring(const InstDesc* __res
     const int buf = init ? 0 : cur;
     float4* out = reinterpret_cast<float4*>(d.keys[buf] + (size_t)row * ld);
     const int nb = d.nb;
-    const DecodeLayout lay = decode_layout(KIND, d.prob.n, L, nb);
+    const DecodeLayout lay = decode_layout(KIND, d.prob.n, L, nb, d.prob.aux);
     float4* kl = reinterpret_cast<float4*>(lds + lay.keys_off);
 
     if (init) {
@@ -416,6 +416,8 @@ hipError_t launch_nsga2_offspring(int kind, int block, const InstDesc* d_descs, 
     switch (kind) {
     case KIND_BITSP_COORDS: return off_kind<KIND_BITSP_COORDS>(block, d_descs, num_inst, blocks, lds, init, s);
     case KIND_BITSP_MATRIX: return off_kind<KIND_BITSP_MATRIX>(block, d_descs, num_inst, blocks, lds, init, s);
+    case KIND_ZDT1: return off_kind<KIND_ZDT1>(block, d_descs, num_inst, blocks, lds, init, s);
+    case KIND_MOVRP: return off_kind<KIND_MOVRP>(block, d_descs, num_inst, blocks, lds, init, s);
     default: return hipErrorInvalidValue;
     }
 }

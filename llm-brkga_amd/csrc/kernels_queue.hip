@@ -42,7 +42,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_queue(const InstDesc* __restrict
         const int nxt = init ? 0 : cur ^ 1;
         const int Pi = d.Pi, n = d.prob.n, M = d.prob.num_obj, ld = d.ld, nb = d.nb;
         const int island = w.y / Pi;
-        const StepLayout sl = step_layout(KIND, n, d.L, nb, Pi, kXY);
+        const StepLayout sl = step_layout(KIND, n, d.L, nb, Pi, kXY, d.prob.aux);
         float4* kl = reinterpret_cast<float4*>(lds + sl.dec.keys_off);
         float2* xy = reinterpret_cast<float2*>(lds + sl.xy_off);
         int* order = reinterpret_cast<int*>(lds + sl.isl_off);
@@ -121,7 +121,7 @@ template <int KIND, bool XYS>
 const void* queue_kernel_kind(int block, int n_max)
 {
     if (n_max > kSortItems * block) return nullptr;
-    if constexpr (KIND == KIND_KNAP_THRESHOLD) {
+    if constexpr (!kind_sorts(KIND)) {
         return queue_kernel_items<KIND, XYS, kSortItems>(block);  // no sort, no per-thread keys
     } else {
         if (n_max <= kCompactItems * block) return queue_kernel_items<KIND, XYS, kCompactItems>(block);
@@ -140,6 +140,9 @@ const void* queue_kernel(int kind, int block, bool xys, int n_max)
     case KIND_KNAP_THRESHOLD: return xys ? nullptr : queue_kernel_kind<KIND_KNAP_THRESHOLD, false>(block, n_max);
     case KIND_KNAP_GREEDY: return xys ? nullptr : queue_kernel_kind<KIND_KNAP_GREEDY, false>(block, n_max);
     case KIND_TSPJ: return xys ? nullptr : queue_kernel_kind<KIND_TSPJ, false>(block, n_max);
+    case KIND_CONTINUOUS: return xys ? nullptr : queue_kernel_kind<KIND_CONTINUOUS, false>(block, n_max);
+    case KIND_JOBSHOP: return xys ? nullptr : queue_kernel_kind<KIND_JOBSHOP, false>(block, n_max);
+    case KIND_VRP_DAYS: return xys ? nullptr : queue_kernel_kind<KIND_VRP_DAYS, false>(block, n_max);
     default: return nullptr;
     }
 }

@@ -150,7 +150,8 @@ int orc_chrom_len(const orc_problem* p)
 
 int orc_num_obj(const orc_problem* p)
 {
-    return (p->kind == ORC_BITSP_COORDS || p->kind == ORC_BITSP_MATRIX) ? 2 : 1;
+    return (p->kind == ORC_BITSP_COORDS || p->kind == ORC_BITSP_MATRIX || p->kind == ORC_ZDT1 ||
+            p->kind == ORC_MOVRP) ? 2 : 1;
 }
 
 int orc_decoded_len(const orc_problem* p)
@@ -203,6 +204,32 @@ int orc_problem_fixed_shift(const orc_problem* p, int objective)
         return orc_fixed_shift(coords_diameter(objective ? p->b : p->a, n + 1));
     case ORC_BITSP_MATRIX:
         return orc_fixed_shift(max_abs(objective ? p->b : p->a, (size_t)(n + 1) * (n + 1)));
+    case ORC_CONTINUOUS: {
+        if (objective) return 30;
+        double lo = fabs((double)p->f0), hi = fabs((double)p->f1);
+        double m = lo > hi ? lo : hi;
+        double bound = p->ip0 == ORC_SPHERE ? m * m
+                       : p->ip0 == ORC_RASTRIGIN ? m * m + 10.0
+                                                 : 100.0 * (m + m * m) * (m + m * m) + (1.0 + m) * (1.0 + m);
+        return orc_fixed_shift(bound * (1.0 + 1e-6));
+    }
+    case ORC_JOBSHOP: return objective ? 30 : orc_fixed_shift(max_abs(p->a, (size_t)n));
+    case ORC_ZDT1: return objective ? 30 : orc_fixed_shift(1.0);
+    case ORC_VRP_DAYS: {
+        if (objective) return 30;
+        int per_day = p->ip0 < 0 ? 0 : (p->ip0 > n ? n : p->ip0);
+        double a = (per_day + 1.0) * max_abs(p->a, (size_t)(n + 1) * (n + 1)), b = 2000.0 * n;
+        return orc_fixed_shift(a > b ? a : b);
+    }
+    case ORC_MOVRP: {
+        double m = coords_diameter(p->a, n + 1);
+        int32_t md = 0;
+        for (int i = 0; i <= n; ++i)
+            if (p->w[i] > md) md = p->w[i];
+        double pd = 100.0 * md, pv = 500.0 * n;
+        double pm = pd > pv ? pd : pv;
+        return orc_fixed_shift(m > pm ? m : pm);
+    }
     default: return 30;
     }
 }
@@ -313,6 +340,236 @@ static void bitsp_objectives(const orc_problem* p, const float* keys, float* obj
     }
 }
 
+/* Deterministic cosine: y = k pi/2 + r with k = rint(y 2/pi) and pi/2 split
+ * into a 33-bit head (k * head exact for |k| < 2^20) and a tail (fdlibm's
+ * rem_pio2 constants); Taylor series of cos / sin in r to degree 16 / 17;
+ * separately rounded FP64 operations throughout. */
+float orc_cos_det(float y)
+{
+    const double two_over_pi = 0.63661977236758134308;
+    const double pio2_hi = 1.57079632673412561417e+00;
+    const double pio2_lo = 6.07710050650619224932e-11;
+    double yd = (double)y;
+    double k = rint(yd * two_over_pi);
+    double r = (yd - k * pio2_hi) - k * pio2_lo;
+    double z = r * r;
+    double c = 1.0 / 20922789888000.0;
+    c = c * z - 1.0 / 87178291200.0;
+    c = c * z + 1.0 / 479001600.0;
+    c = c * z - 1.0 / 3628800.0;
+    c = c * z + 1.0 / 40320.0;
+    c = c * z - 1.0 / 720.0;
+    c = c * z + 1.0 / 24.0;
+    c = c * z - 0.5;
+    c = c * z + 1.0;
+    double sn = 1.0 / 355687428096000.0;
+    sn = sn * z - 1.0 / 1307674368000.0;
+    sn = sn * z + 1.0 / 6227020800.0;
+    sn = sn * z - 1.0 / 39916800.0;
+    sn = sn * z + 1.0 / 362880.0;
+    sn = sn * z - 1.0 / 5040.0;
+    sn = sn * z + 1.0 / 120.0;
+    sn = sn * z - 1.0 / 6.0;
+    sn = (sn * z) * r + r;
+    int q = (int)((long long)k & 3);
+    double v = q == 0 ? c : (q == 1 ? -sn : (q == 2 ? -c : sn));
+    return (float)v;
+}
+
+static float cont_var(float gene, float lo, float hi)
+{
+    float w = hi - lo;
+    float t = gene * w;
+    return t + lo;
+}
+
+/* Sphere / Rastrigin / Rosenbrock over x_i = key_i (hi - lo) + lo
+ * (continuous_opt_gpu_config.hpp:116-160); Rastrigin's constant 10 n is
+ * added exactly.  dec holds the FP32 bits of each x_i. */
+static float continuous_fitness(const orc_problem* p, const float* keys, int32_t* dec)
+{
+    int n = p->n, fn = p->ip0;
+    int64_t acc = 0;
+    for (int i = 0; i < n; ++i) {
+        float x = cont_var(keys[i], p->f0, p->f1);
+        if (dec) memcpy(&dec[i], &x, sizeof(float));
+        float t;
+        if (fn == ORC_SPHERE) {
+            t = x * x;
+        } else if (fn == ORC_RASTRIGIN) {
+            float arg = 6.2831854820251465f * x;
+            float c = orc_cos_det(arg);
+            float xx = x * x;
+            float tc = 10.0f * c;
+            t = xx - tc;
+        } else {
+            if (i + 1 >= n) continue;
+            float xn = cont_var(keys[i + 1], p->f0, p->f1);
+            float xx = x * x;
+            float t1 = xn - xx;
+            float t2 = 1.0f - x;
+            float a = 100.0f * t1;
+            a = a * t1;
+            float b = t2 * t2;
+            t = a + b;
+        }
+        acc += fx_term(t, p->fx0);
+    }
+    if (fn == ORC_RASTRIGIN) acc += fx_term(10.0f * (float)n, p->fx0);
+    return fx_to_float(acc, p->fx0);
+}
+
+static int job_machine(float gene, int M)
+{
+    float v = gene * (float)M;
+    if (!(v >= 0.0f)) return 0;
+    if (v >= (float)M) return M - 1;
+    int m = (int)v;
+    return m < M ? m : M - 1;
+}
+
+/* Makespan of the machine assignment int(key M) (job_scheduling_gpu_config
+ * .hpp:138-158): largest machine load, empty machines counting 0. */
+static float jobshop_fitness(const orc_problem* p, const float* keys, int32_t* dec)
+{
+    int n = p->n, M = p->ip0;
+    int64_t* load = (int64_t*)calloc((size_t)M, sizeof(int64_t));
+    for (int i = 0; i < n; ++i) {
+        int m = job_machine(keys[i], M);
+        load[m] += fx_term(p->a[i], p->fx0);
+        if (dec) dec[i] = m;
+    }
+    int64_t best = 0;
+    for (int m = 0; m < M; ++m)
+        if (load[m] > best) best = load[m];
+    free(load);
+    return fx_to_float(best, p->fx0);
+}
+
+/* ZDT1 (examples/zdt1.cu:43-63): f1 = x1, g = 1 + 9 sum(x2..xn) / (n - 1)
+ * with the key sum exact, f2 = g (1 - sqrt(f1 / g)). */
+static void zdt1_objectives(const orc_problem* p, const float* keys, float* obj, int32_t* dec)
+{
+    int n = p->n;
+    int64_t acc = 0;
+    for (int i = 1; i < n; ++i) acc += fx_term(keys[i], p->fx0);
+    if (dec)
+        for (int i = 0; i < n; ++i) memcpy(&dec[i], &keys[i], sizeof(float));
+    float f1 = keys[0];
+    float g = 1.0f;
+    if (n > 1) {
+        double sum = (double)acc * ldexp(1.0, -p->fx0);
+        g = (float)(1.0 + 9.0 * sum / (double)(n - 1));
+    }
+    float q = f1 / g;
+    float r = sqrtf(q);
+    obj[0] = f1;
+    obj[1] = (float)((double)g * (1.0 - (double)r));
+}
+
+/* Multi-day routes (vrp_blazer_config.hpp:192-296), the reference's loop as
+ * written: stops in key order join the current day while it has fewer than
+ * ip0 stops and cum + leg + return (FP32) <= d0 (FP64); a stop that fits no
+ * empty day is skipped and lands on the unvisited list twice (once in the
+ * loop, once in the final sweep over unvisited stops).  Fitness = day
+ * drive totals + 1000 per unvisited entry.  dec: day of each stop (0 =
+ * skipped). */
+static float vrp_days_fitness(const orc_problem* p, const float* keys, int32_t* dec)
+{
+    int n = p->n, dim = n + 1;
+    const float* T = p->a;
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    orc_argsort_keys(keys, n, order);
+    if (dec) memset(dec, 0, sizeof(int32_t) * (size_t)n);
+    int64_t acc = 0;
+    int skipped = 0, day = 0, idx = 0;
+    while (idx < n) {
+        int cur = 0, today = 0;
+        float cum = 0.0f;
+        while (idx < n && today < p->ip0) {
+            int st = order[idx] + 1;
+            float tts = T[(size_t)cur * dim + st];
+            float ttd = T[(size_t)st * dim];
+            float tot = cum + tts;
+            tot = tot + ttd;
+            if ((double)tot <= p->d0) {
+                cum = cum + tts;
+                cur = st;
+                ++today;
+                ++idx;
+                if (dec) dec[st - 1] = day + 1;
+            } else {
+                break;
+            }
+        }
+        if (today > 0) {
+            float total = cum + T[(size_t)cur * dim];
+            acc += fx_term(total, p->fx0);
+            ++day;
+        }
+        if (today == 0 && idx < n) {
+            ++skipped;
+            ++idx;
+        }
+    }
+    acc += fx_term(2000.0f * (float)skipped, p->fx0);
+    free(order);
+    return fx_to_float(acc, p->fx0);
+}
+
+/* Bi-objective CVRP (multi_objective_vrp_config.hpp:208-300), as written:
+ * keys biased by i 1e-5 + hash 1e-5 with hash = running FP32 sum of
+ * key_i (i + 1), sorted (ties by customer), split greedily by capacity (a
+ * route always takes its first customer).  Objectives: total distance +
+ * 100 per unit of overload; longest route; both + 500 per route beyond
+ * ip0 vehicles.  dec: route of each customer (1-based). */
+static void movrp_objectives(const orc_problem* p, const float* keys, float* obj, int32_t* dec)
+{
+    int n = p->n;
+    float h = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        float t = keys[i] * (float)(i + 1);
+        h = h + t;
+    }
+    float* vals = (float*)malloc(sizeof(float) * (size_t)(n > 0 ? n : 1));
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    for (int i = 0; i < n; ++i) {
+        float b0 = (float)i * 0.00001f;
+        float b1 = h * 0.00001f;
+        float bias = b0 + b1;
+        vals[i] = keys[i] + bias;
+    }
+    orc_argsort_keys(vals, n, order);
+    int64_t total = 0, pen = 0, longest = 0, rd = 0, load = 0;
+    int routes = 0, last = 0, nonempty = 0;
+    for (int r = 0; r <= n; ++r) {
+        int c = r < n ? order[r] + 1 : 0;
+        int64_t d = r < n ? p->w[c] : 0;
+        if (nonempty && (r == n || load + d > p->cap)) {
+            rd += fx_term(orc_tsp_edge_coords(p->a, last, 0), p->fx0);
+            total += rd;
+            if (rd > longest) longest = rd;
+            if (load > p->cap) pen += fx_term((float)(load - p->cap) * 100.0f, p->fx0);
+            ++routes;
+            rd = 0;
+            load = 0;
+            last = 0;
+            nonempty = 0;
+        }
+        if (r == n) break;
+        rd += fx_term(orc_tsp_edge_coords(p->a, last, c), p->fx0);
+        load += d;
+        last = c;
+        nonempty = 1;
+        if (dec) dec[c - 1] = routes + 1;
+    }
+    int64_t veh = routes > p->ip0 ? fx_term((float)(routes - p->ip0) * 500.0f, p->fx0) : 0;
+    obj[0] = fx_to_float(total + pen + veh, p->fx0);
+    obj[1] = fx_to_float(longest + veh, p->fx0);
+    free(vals);
+    free(order);
+}
+
 void orc_evaluate(const orc_problem* p, const float* keys, int count,
                   float* fitness, int32_t* decoded)
 {
@@ -329,6 +586,11 @@ void orc_evaluate(const orc_problem* p, const float* keys, int count,
         case ORC_TSPJ: fitness[r] = tspj_fitness(p, k, dec); break;
         case ORC_BITSP_COORDS:
         case ORC_BITSP_MATRIX: bitsp_objectives(p, k, fitness + (size_t)r * M, dec); break;
+        case ORC_CONTINUOUS: fitness[r] = continuous_fitness(p, k, dec); break;
+        case ORC_JOBSHOP: fitness[r] = jobshop_fitness(p, k, dec); break;
+        case ORC_ZDT1: zdt1_objectives(p, k, fitness + (size_t)r * M, dec); break;
+        case ORC_VRP_DAYS: fitness[r] = vrp_days_fitness(p, k, dec); break;
+        case ORC_MOVRP: movrp_objectives(p, k, fitness + (size_t)r * M, dec); break;
         default: fitness[r] = 0.0f; break;
         }
     }

@@ -47,8 +47,14 @@ enum {
     ORC_KNAP_GREEDY = 3,
     ORC_TSPJ = 4,
     ORC_BITSP_COORDS = 5,
-    ORC_BITSP_MATRIX = 6
+    ORC_BITSP_MATRIX = 6,
+    ORC_CONTINUOUS = 7,
+    ORC_JOBSHOP = 8,
+    ORC_ZDT1 = 9,
+    ORC_VRP_DAYS = 10,
+    ORC_MOVRP = 11
 };
+enum { ORC_SPHERE = 0, ORC_RASTRIGIN = 1, ORC_ROSENBROCK = 2 };
 
 typedef struct {
     int kind;
@@ -57,8 +63,11 @@ typedef struct {
     const float* b;   /* TSPJ jobs [(n+1)^2] / bi-TSP set 1 */
     const int32_t* w; /* knapsack weights */
     const int32_t* v; /* knapsack values */
-    int64_t cap;      /* knapsack capacity */
+    int64_t cap;      /* knapsack capacity / vehicle capacity */
     int fx0, fx1;     /* fixed-point shifts (objective 0 / 1) */
+    int ip0;          /* continuous function / machines / stops per day / vehicles */
+    float f0, f1;     /* continuous bounds */
+    double d0;        /* VRP days: drive limit per day (minutes) */
 } orc_problem;
 
 typedef struct {
@@ -90,6 +99,9 @@ int orc_maximize(const orc_problem* p);
 
 void orc_argsort_keys(const float* keys, int n, int32_t* perm);
 float orc_tsp_edge_coords(const float* xy, int a, int b);
+/* Deterministic FP32 cosine (FP64 Cody-Waite reduction + Taylor series);
+ * the engine evaluates the identical operation sequence. */
+float orc_cos_det(float y);
 
 void orc_evaluate(const orc_problem* p, const float* keys, int count,
                   float* fitness, int32_t* decoded);

@@ -15,7 +15,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 REF_LIB_PATH = os.path.join(HERE, "_ref", "libref.so")
 
-TSP_COORDS, TSP_MATRIX, KNAP_THRESHOLD, KNAP_GREEDY, TSPJ, BITSP_COORDS, BITSP_MATRIX = range(7)
+(TSP_COORDS, TSP_MATRIX, KNAP_THRESHOLD, KNAP_GREEDY, TSPJ, BITSP_COORDS, BITSP_MATRIX, CONTINUOUS, JOBSHOP,
+ ZDT1, VRP_DAYS, MOVRP) = range(12)
+SPHERE, RASTRIGIN, ROSENBROCK = range(3)
 TAG_PARENTS, TAG_COINS, TAG_MUTANT, TAG_INIT, TAG_MUTCOIN = range(5)
 
 _PF = ctypes.POINTER(ctypes.c_float)
@@ -34,6 +36,10 @@ class OrcProblem(ctypes.Structure):
         ("cap", ctypes.c_int64),
         ("fx0", ctypes.c_int),
         ("fx1", ctypes.c_int),
+        ("ip0", ctypes.c_int),
+        ("f0", ctypes.c_float),
+        ("f1", ctypes.c_float),
+        ("d0", ctypes.c_double),
     ]
 
 
@@ -80,6 +86,8 @@ def lib():
         L.orc_argsort_keys.argtypes = [_PF, ctypes.c_int, _PI]
         L.orc_tsp_edge_coords.argtypes = [_PF, ctypes.c_int, ctypes.c_int]
         L.orc_tsp_edge_coords.restype = ctypes.c_float
+        L.orc_cos_det.argtypes = [ctypes.c_float]
+        L.orc_cos_det.restype = ctypes.c_float
         L.orc_evaluate.argtypes = [ctypes.POINTER(OrcProblem), _PF, ctypes.c_int, _PF, _PI]
         L.orc_init_keys.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, ctypes.c_int, _PF]
         L.orc_partition.argtypes = [_PF, ctypes.c_int, ctypes.c_int, ctypes.c_int, _PI]
@@ -129,6 +137,10 @@ def fixed_shift(m):
     return lib().orc_fixed_shift(float(m))
 
 
+def cos_det(y):
+    return lib().orc_cos_det(float(np.float32(y)))
+
+
 def argsort_keys(keys):
     keys = np.ascontiguousarray(keys, np.float32)
     perm = np.empty(keys.shape[0], np.int32)
@@ -139,7 +151,7 @@ def argsort_keys(keys):
 class Problem:
     """Oracle-side problem: the same data and kinds as the engine's."""
 
-    def __init__(self, kind, n, a=None, b=None, w=None, v=None, cap=0):
+    def __init__(self, kind, n, a=None, b=None, w=None, v=None, cap=0, ip0=0, f0=0.0, f1=0.0, d0=0.0):
         self.kind = kind
         self.n = n
         self._a = None if a is None else np.ascontiguousarray(a, np.float32)
@@ -154,6 +166,10 @@ class Problem:
         self.c.w = _i(self._w) if self._w is not None else _PI()
         self.c.v = _i(self._v) if self._v is not None else _PI()
         self.c.cap = int(cap)
+        self.c.ip0 = int(ip0)
+        self.c.f0 = float(f0)
+        self.c.f1 = float(f1)
+        self.c.d0 = float(d0)
         self.c.fx0 = lib().orc_problem_fixed_shift(ctypes.byref(self.c), 0)
         self.c.fx1 = lib().orc_problem_fixed_shift(ctypes.byref(self.c), 1)
 
@@ -187,13 +203,36 @@ class Problem:
         m0 = np.asarray(m0, np.float32)
         return cls(BITSP_MATRIX, m0.shape[0] - 1, a=m0, b=m1)
 
+    @classmethod
+    def continuous(cls, function, n, lower, upper):
+        return cls(CONTINUOUS, n, ip0=function, f0=np.float32(lower), f1=np.float32(upper))
+
+    @classmethod
+    def job_scheduling(cls, job_times, machines):
+        t = np.asarray(job_times, np.float32)
+        return cls(JOBSHOP, t.shape[0], a=t, ip0=machines)
+
+    @classmethod
+    def zdt1(cls, n):
+        return cls(ZDT1, n)
+
+    @classmethod
+    def vrp_days(cls, time_matrix, max_stops_per_day, max_drive_minutes):
+        t = np.asarray(time_matrix, np.float32)
+        return cls(VRP_DAYS, t.shape[0] - 1, a=t, ip0=max_stops_per_day, d0=max_drive_minutes)
+
+    @classmethod
+    def movrp(cls, xy, demands, capacity, vehicles):
+        xy = np.asarray(xy, np.float32).reshape(-1, 2)
+        return cls(MOVRP, xy.shape[0] - 1, a=xy, w=demands, cap=capacity, ip0=vehicles)
+
     @property
     def chrom_len(self):
         return 2 * self.n if self.kind == TSPJ else self.n
 
     @property
     def num_obj(self):
-        return 2 if self.kind in (BITSP_COORDS, BITSP_MATRIX) else 1
+        return 2 if self.kind in (BITSP_COORDS, BITSP_MATRIX, ZDT1, MOVRP) else 1
 
     @property
     def maximize(self):
@@ -326,5 +365,9 @@ def ref_lib():
         L.ref_knapsack_load.restype = ctypes.c_int
         L.ref_multitsp_load_objectives.argtypes = [ctypes.c_char_p, _PF, ctypes.c_int, _PF]
         L.ref_multitsp_load_objectives.restype = ctypes.c_int
+        L.ref_vrp_days.argtypes = [_PF, ctypes.c_int, ctypes.c_int, ctypes.c_double, _PF, ctypes.c_int, _PF, _PI]
+        L.ref_movrp_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.ref_movrp_info.restype = ctypes.c_int
+        L.ref_movrp.argtypes = [ctypes.c_char_p, _PF, ctypes.c_int, _PF, _PI]
         _ref = L
     return _ref

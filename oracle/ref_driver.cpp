@@ -10,9 +10,13 @@
 //   Individual<float>::dominates                       core/individual.hpp:99-114
 //   KnapsackConfig<float>::load_from_file              configs/knapsack_config.hpp:180-304
 //   MultiTSPConfig<float>::load_from_file              configs/multi_tsp_config.hpp:165-269
+//   VrpBlazerConfig<float> fitness + day routes        configs/vrp_blazer_config.hpp:192-296
+//   MultiObjectiveVRPConfig<float> load + objectives   configs/multi_objective_vrp_config.hpp:69-300
 // Nothing here is part of the engine.
 #include "configs/knapsack_config.hpp"
+#include "configs/multi_objective_vrp_config.hpp"
 #include "configs/multi_tsp_config.hpp"
+#include "configs/vrp_blazer_config.hpp"
 #include "core/individual.hpp"
 
 #include <cstdint>
@@ -109,6 +113,55 @@ int ref_multitsp_load_objectives(const char* path, const float* keys, int rows, 
         obj[2 * r + 1] = cfg->objective_functions[1](ind);
     }
     return n;
+}
+
+// VrpBlazerConfig<float> over time_matrix[dim][dim]: fitness and the day
+// (1-based) of each stop, 0 for stops in no day.
+void ref_vrp_days(const float* tm, int dim, int max_stops, double max_hours, const float* keys, int count,
+                  float* fitness, int32_t* day)
+{
+    VrpBlazerConfig<float> cfg;
+    cfg.max_stops_per_day = max_stops;
+    cfg.max_drive_hours_per_day = max_hours;
+    cfg.update_derived_params();
+    std::vector<std::vector<float>> m(dim, std::vector<float>(dim));
+    for (int i = 0; i < dim; ++i)
+        for (int j = 0; j < dim; ++j) m[i][j] = tm[(size_t)i * dim + j];
+    cfg.set_time_matrix(m);
+    const int n = dim - 1;
+    for (int r = 0; r < count; ++r) {
+        const float* k = keys + (size_t)r * n;
+        auto ind = make_individual(k, n);
+        fitness[r] = cfg.calculate_fitness(ind);
+        auto sol = cfg.decode_solution(k);
+        for (int i = 0; i < n; ++i) day[(size_t)r * n + i] = 0;
+        for (const auto& d : sol.days)
+            for (int st : d.stop_indices) day[(size_t)r * n + st - 1] = d.day_number;
+    }
+}
+
+// MultiObjectiveVRPConfig<float>::load_from_file(path); returns the number
+// of customers, and the vehicle count and capacity it parsed.
+int ref_movrp_info(const char* path, int* vehicles, int* capacity)
+{
+    auto cfg = MultiObjectiveVRPConfig<float>::load_from_file(path);
+    *vehicles = cfg->get_num_vehicles();
+    *capacity = cfg->get_vehicle_capacity();
+    return cfg->get_num_customers();
+}
+
+// Objectives [count][2] and the route (1-based) of each customer.
+void ref_movrp(const char* path, const float* keys, int count, float* obj, int32_t* route)
+{
+    auto cfg = MultiObjectiveVRPConfig<float>::load_from_file(path);
+    const int n = cfg->get_num_customers();
+    for (int r = 0; r < count; ++r) {
+        const float* k = keys + (size_t)r * n;
+        cfg->decode_solution(k, obj + 2 * (size_t)r);
+        auto routes = cfg->decode_routes(k);
+        for (size_t ri = 0; ri < routes.size(); ++ri)
+            for (int c : routes[ri].customers) route[(size_t)r * n + c - 1] = (int32_t)ri + 1;
+    }
 }
 
 }  // extern "C"

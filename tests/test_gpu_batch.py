@@ -33,7 +33,14 @@ MEMBERS = [
     ("tspj_ls", 45, 64, 12, 6, 0.75, 2, 3, 2),
     ("bitsp", 30, 32, 0, 0, 0.7, 1, 0, 0),
     ("bitsp", 80, 64, 0, 0, 0.7, 1, 0, 0),
+    ("cont_rastrigin", 30, 64, 12, 6, 0.7, 2, 2, 2),
+    ("cont_rosenbrock", 200, 64, 12, 6, 0.7, 1, 0, 0),
+    ("jobshop", 80, 64, 12, 6, 0.7, 1, 0, 0),
+    ("vrp_days", 50, 64, 12, 6, 0.7, 1, 0, 0),
+    ("zdt1", 30, 32, 0, 0, 0.5, 1, 0, 0),
+    ("movrp", 31, 32, 0, 0, 0.5, 1, 0, 0),
 ]
+NSGA_KINDS = ("bitsp", "zdt1", "movrp")
 
 
 def build(engine):
@@ -107,7 +114,7 @@ def test_batch_autotune_keeps_results(engine):
     for b in after:
         assert b["block"] in (64, 128, 256, 512, 1024) and b["block"] * 16 >= b["n_max"]
         assert b["chunk"] >= 1
-    assert sum(b["members"] for b in after) == sum(1 for m in MEMBERS if not m[0].startswith("bitsp"))
+    assert sum(b["members"] for b in after) == sum(1 for m in MEMBERS if m[0] not in NSGA_KINDS)
     batch.evolve(G, "launch")
     torch.cuda.synchronize()
     assert_same(batch, solo, G, "after autotune")

@@ -30,6 +30,11 @@ REQUESTS = [
     ("bitsp", 120, 128, 0, 0, 0.7, 1, 0, 0, 3),
     ("knap_greedy", 700, 128, 20, 12, 0.7, 1, 0, 0, 5),
     ("tsp", 5000, 32, 5, 5, 0.7, 1, 0, 0, 2),
+    ("jobshop", 120, 64, 12, 6, 0.7, 1, 0, 0, 6),
+    ("cont_sphere", 40, 64, 12, 6, 0.7, 1, 0, 0, 5),
+    ("vrp_days", 60, 64, 12, 6, 0.7, 1, 0, 0, 4),
+    ("zdt1", 30, 32, 0, 0, 0.5, 1, 0, 0, 7),
+    ("movrp", 31, 32, 0, 0, 0.5, 1, 0, 0, 3),
 ]
 
 
