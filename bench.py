@@ -280,6 +280,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rates", type=str, default=",".join(str(r) for r in SERVE_RATES))
     ap.add_argument("--serve-budget-gib", type=float, default=8.0)
+    ap.add_argument("--step-block", type=int, default=0,
+                    help="step workgroup size (64..1024); 0 = autotune it (under a profiler, pin it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the island model runs all islands on one device); "
@@ -303,7 +305,11 @@ def main():
     # warm-up: plain steps, workgroup-size tuning, then one run of the
     # K-step graph (captures it)
     pop.evolve(W)
-    block = pop.autotune(20)
+    if args.step_block:
+        pop.set_block(args.step_block)
+        block = args.step_block
+    else:
+        block = pop.autotune(20)
     pop.evolve(K, graph=True)
     sync()
 
@@ -342,7 +348,8 @@ def main():
                    "launch": "hipGraph replay of the K-generation loop", "step_block": block},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                     "kernel": f"brkga::k_step_one<0, {block}> (crossover / mutants / elite copy + decode + fitness)",
+                     "kernel": f"brkga::k_step_one<0, {block}, {4 if L <= 4 * block else 16}> "
+                               "(crossover / mutants / elite copy + decode + fitness)",
                      "kernel_us": round(step_us, 3), "partition_us": round(close_us, 3),
                      "bytes_per_generation": B,
                      "generation_frac": round(B * gens_per_s / (HBM_PEAK_GBS * 1e9), 4)},

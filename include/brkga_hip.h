@@ -273,14 +273,20 @@ brkga_status brkga_population_device_buffers(brkga_pop_t pop, float** d_keys, fl
  * depend on the size.  *block (may be NULL) receives the choice. */
 brkga_status brkga_population_autotune(brkga_pop_t pop, int32_t reps, int32_t* block, void* stream);
 
+/* Sets the workgroup size of the population's generation kernel (64, 128,
+ * 256, 512 or 1024, at least n / 16) without timing anything; the
+ * counterpart of BRKGAConfig::threads_per_block (core/config.hpp:30, 80-82).
+ * Results never depend on it. */
+brkga_status brkga_population_set_block(brkga_pop_t pop, int32_t block);
+
 /* Mean device time of the generation kernel (BRKGA step / NSGA-II
- * offspring) and of the closing kernels (partition / NSGA-II selection +
- * gather), in microseconds, from HIP events on `stream`.  BRKGA: `reps`
- * back-to-back step launches from the current generation (each rewrites
- * the same next-generation rows, so the population does not advance), then
- * `reps` partition launches of the current generation.  NSGA-II: `reps`
- * real generations with an event pair around each kernel group (the
- * population advances).  Replaces the reference's PerformanceStats timers
+ * offspring) and of the kernels that close a generation (partition and any
+ * local search / NSGA-II selection + gather), in microseconds, over `reps`
+ * real generations enqueued on `stream` as brkga_evolve does.  BRKGA without
+ * local search: each kernel's own start / stop time (events attached to its
+ * dispatch); otherwise HIP events recorded between the kernels.  One
+ * synchronisation at the end; the population advances by `reps`
+ * generations.  Replaces the reference's PerformanceStats timers
  * (core/solver.hpp:143-157, 406-438), which time synchronous launches with
  * std::chrono. */
 brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* step_us, double* close_us,

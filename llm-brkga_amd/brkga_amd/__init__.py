@@ -159,6 +159,7 @@ _SIGNATURES = {
     "brkga_population_profile": ([_P, _I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _P],
                                  ctypes.c_int),
     "brkga_population_autotune": ([_P, _I32, _PI, _P], ctypes.c_int),
+    "brkga_population_set_block": ([_P, _I32], ctypes.c_int),
     "brkga_batch_create": ([_P, _I32, ctypes.POINTER(_P), ctypes.POINTER(CParams), ctypes.POINTER(_P)],
                            ctypes.c_int),
     "brkga_batch_destroy": ([_P], ctypes.c_int),
@@ -413,6 +414,10 @@ class Population:
         _check(lib().brkga_get_best(self.handle, _fptr(fit), _fptr(keys), _iptr(dec), stream or None))
         return (fit if M > 1 else float(fit[0])), keys, dec
 
+    def set_block(self, block: int):
+        """Workgroup size of the generation kernel (results never depend on it)."""
+        _check(lib().brkga_population_set_block(self.handle, int(block)))
+
     def autotune(self, reps: int = 20, stream: int = 0) -> int:
         """Fastest workgroup size of the generation kernel (results unchanged)."""
         b = ctypes.c_int32()
@@ -421,8 +426,8 @@ class Population:
 
     def profile(self, reps: int, stream: int = 0):
         """Mean (generation kernel, closing kernels) device time in
-        microseconds over `reps` back-to-back launches (BRKGA: the population
-        does not advance; NSGA-II: it advances `reps` generations)."""
+        microseconds over `reps` real generations (the population advances
+        `reps` generations)."""
         a, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().brkga_population_profile(self.handle, reps, ctypes.byref(a), ctypes.byref(b),
                                               stream or None))

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "capi_internal.hpp"
 #include "device_common.hpp"
@@ -1015,6 +1016,22 @@ brkga_status brkga_population_autotune(brkga_pop_t pop, int32_t reps, int32_t* b
     return BRKGA_OK;
 }
 
+brkga_status brkga_population_set_block(brkga_pop_t pop, int32_t block)
+{
+    if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
+    const int n = pop->host.prob.n;
+    if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024)
+        return set_error(BRKGA_ERR_INVALID_ARG, "block must be 64, 128, 256, 512 or 1024");
+    if ((long long)block * kSortItems < n)
+        return set_error(BRKGA_ERR_INVALID_ARG, "block too small: a workgroup decodes at most 16 keys per thread");
+    if (block != pop->block) {
+        pop->block = block;
+        for (auto& kv : pop->graphs) (void)hipGraphExecDestroy(kv.second);
+        pop->graphs.clear();
+    }
+    return BRKGA_OK;
+}
+
 brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* step_us, double* close_us,
                                       void* stream)
 {
@@ -1022,52 +1039,63 @@ brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* ste
     DeviceGuard g(pop->problem->engine->device);
     hipStream_t s = as_stream(stream);
     const InstDesc& d = pop->host;
-    hipEvent_t ev[3];
-    for (int i = 0; i < 3; ++i) BRKGA_HIP_TRY(hipEventCreate(&ev[i]));
-    brkga_status st = BRKGA_OK;
+    // reps real generations, enqueued as brkga_evolve does.  BRKGA: the step
+    // and partition kernels take start / stop events from their own
+    // dispatch (no marker packets between kernels).  NSGA-II: marker events
+    // around the offspring kernel and around selection + gather.  One
+    // synchronisation at the end; the population advances reps generations.
+    const bool timed = !pop->nsga2 && d.ls_interval <= 0;
+    std::vector<hipEvent_t> ev(4 * (size_t)reps, nullptr);
     hipError_t e = hipSuccess;
-    float a = 0.f, b = 0.f;
-    if (!pop->nsga2) {
-        // the step reads generation t and rewrites the same rows of t + 1 on
-        // every launch; the partition re-sorts generation t in place
+    for (auto& x : ev)
+        if (e == hipSuccess) e = hipEventCreate(&x);
+    for (int i = 0; i < reps && e == hipSuccess; ++i) {
+        hipEvent_t* q = &ev[4 * (size_t)i];
         const int cur = pop->host_t & 1;
-        e = launch_step_one(d, pop->block, cur, 0, s);  // warm-up
-        if (e == hipSuccess) e = hipEventRecord(ev[0], s);
-        for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch_step_one(d, pop->block, cur, 0, s);
-        if (e == hipSuccess) e = hipEventRecord(ev[1], s);
-        for (int i = 0; i < reps && e == hipSuccess; ++i) e = launch_partition_one(d, cur, 0, s);
-        if (e == hipSuccess) e = hipEventRecord(ev[2], s);
-    } else {
-        // selection advances the counter: one event pair per real generation
-        double sa = 0.0, sb = 0.0;
-        for (int i = 0; i < reps && e == hipSuccess; ++i) {
-            e = hipEventRecord(ev[0], s);
-            if (e == hipSuccess)
-                e = launch_nsga2_offspring(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, 0, s);
-            if (e == hipSuccess) e = hipEventRecord(ev[1], s);
-            if (e == hipSuccess) e = launch_nsga2_select(pop->d_desc, 1, 0, d.pop, s);
-            if (e == hipSuccess) e = launch_nsga2_gather(pop->d_desc, 1, d.pop, s);
-            if (e == hipSuccess) e = hipEventRecord(ev[2], s);
-            if (e == hipSuccess) e = hipEventSynchronize(ev[2]);
-            if (e == hipSuccess) e = hipEventElapsedTime(&a, ev[0], ev[1]);
-            if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
-            sa += a;
-            sb += b;
-            ++pop->host_t;
+        if (timed) {
+            e = launch_step_one(d, pop->block, cur, 0, s, q[0], q[1]);
+            if (e == hipSuccess) e = launch_partition_one(d, cur ^ 1, 1, s, q[2], q[3]);
+        } else {
+            e = hipEventRecord(q[0], s);
+            if (e == hipSuccess) {
+                if (pop->nsga2)
+                    e = launch_nsga2_offspring(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, 0, s);
+                else
+                    e = launch_step_one(d, pop->block, cur, 0, s);
+            }
+            if (e == hipSuccess) e = hipEventRecord(q[1], s);
+            if (e == hipSuccess) e = hipEventRecord(q[2], s);
+            if (e == hipSuccess) {
+                if (pop->nsga2) {
+                    e = launch_nsga2_select(pop->d_desc, 1, 0, d.pop, s);
+                    if (e == hipSuccess) e = launch_nsga2_gather(pop->d_desc, 1, d.pop, s);
+                } else {
+                    e = launch_partition_one(d, cur ^ 1, 1, s);
+                    if (e == hipSuccess)
+                        e = launch_tspj_local_search(pop->d_desc, 1, d.islands * d.ls_k, pop->block,
+                                                     tspj_ls_lds_bytes(d.prob.n, d.nb), s);
+                    if (e == hipSuccess) e = launch_partition_one(d, -1, 2, s);
+                }
+            }
+            if (e == hipSuccess) e = hipEventRecord(q[3], s);
         }
-        a = (float)sa;
-        b = (float)sb;
+        ++pop->host_t;
     }
-    if (e == hipSuccess && !pop->nsga2) {
-        e = hipEventSynchronize(ev[2]);
-        if (e == hipSuccess) e = hipEventElapsedTime(&a, ev[0], ev[1]);
-        if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
+    double sa = 0.0, sb = 0.0;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    for (int i = 0; i < reps && e == hipSuccess; ++i) {
+        const hipEvent_t* q = &ev[4 * (size_t)i];
+        float a = 0.f, b = 0.f;
+        e = hipEventElapsedTime(&a, q[0], q[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&b, q[2], q[3]);
+        sa += a;
+        sb += b;
     }
-    if (e != hipSuccess) st = hip_error(e, "profiled launches");
-    for (int i = 0; i < 3; ++i) (void)hipEventDestroy(ev[i]);
-    if (st != BRKGA_OK) return st;
-    if (step_us) *step_us = 1000.0 * a / reps;
-    if (close_us) *close_us = 1000.0 * b / reps;
+    for (auto& x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (e != hipSuccess) return hip_error(e, "profiled generations");
+    if (step_us) *step_us = 1000.0 * sa / reps;
+    if (close_us) *close_us = 1000.0 * sb / reps;
     return BRKGA_OK;
 }
 

@@ -52,6 +52,7 @@ struct brkga_engine_s {
     struct Shape {
         int block, chunk;
         bool xys;
+        double child_us;  // measured step time per child slot (0: not measured)
     };
     std::map<std::pair<int, int>, Shape> tuned;
     std::mutex tuned_mu;

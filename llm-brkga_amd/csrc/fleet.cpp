@@ -127,6 +127,7 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
                     b.block = it->second.block;
                     b.chunk = it->second.chunk;
                     b.xys = it->second.xys;
+                    b.child_us = it->second.child_us;
                 } else {
                     b.block = default_block(key);
                     b.chunk = default_chunk(key);
@@ -136,6 +137,7 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
             }
             FleetBucket& b = buckets_.back();
             ++b.count;
+            b.children += d.pop;
             b.work += (double)d.pop * (n + kChildFixedWork);
             b.n_min = std::min(b.n_min, n);
             b.n_max = std::max(b.n_max, n);
@@ -224,21 +226,26 @@ brkga_status Fleet::build_items(hipStream_t s)
 }
 
 // Largest-work-first onto the least-loaded lane; with NSGA-II members the
-// last lane is theirs.
+// last lane is theirs.  The work of a bucket is its measured time (child
+// slots x tuned time per child) when every bucket has been tuned, else
+// population x (n + kChildFixedWork).
 void Fleet::assign_lanes()
 {
     const int brkga_lanes = groups_.empty() ? kLanes : kLanes - 1;
     nsga_lane_ = groups_.empty() ? 0 : (buckets_.empty() ? 0 : kLanes - 1);
+    bool measured = !buckets_.empty();
+    for (const auto& b : buckets_) measured = measured && b.child_us > 0.0;
     std::vector<int> idx(buckets_.size());
     std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return buckets_[a].work > buckets_[b].work; });
+    auto cost = [&](int i) { return measured ? buckets_[i].children * buckets_[i].child_us : buckets_[i].work; };
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost(a) > cost(b); });
     double load[kLanes] = {};
     for (int i : idx) {
         int best = 0;
         for (int k = 1; k < brkga_lanes; ++k)
             if (load[k] < load[best]) best = k;
         buckets_[i].lane = best;
-        load[best] += buckets_[i].work;
+        load[best] += cost(i);
     }
 }
 
@@ -279,7 +286,11 @@ brkga_status Fleet::tick(hipStream_t s)
             forked = true;
             if (e == hipSuccess) e = hipStreamWaitEvent(lanes_[k], fork_, 0);
         }
-    for (size_t i = 0; i < buckets_.size() && e == hipSuccess; ++i) {
+    // Buckets are stored largest n first; each lane launches its smallest
+    // (latency-bound) buckets first, so the kernels still running at the end
+    // of the tick are the ones with the most parallel work.
+    for (size_t r = buckets_.size(); r-- > 0 && e == hipSuccess;) {
+        const size_t i = r;
         const FleetBucket& b = buckets_[i];
         e = launch_step_queue(b.kind, b.block, b.xys, b.n_max, d_descs_, d_items_ + b.item_begin, b.item_count,
                               d_queues_ + kQueueStride * i, b.grid, b.lds, lane(b.lane));
@@ -425,7 +436,7 @@ brkga_status Fleet::tune(int reps, bool only_new, hipStream_t s)
                     if (!have || ms < best_ms) {
                         have = true;
                         best_ms = ms;
-                        best = Shape{blk, ch, xys};
+                        best = Shape{blk, ch, xys, 1000.0 * ms / reps / std::max(1, buckets_[bi].children)};
                     }
                 }
                 if (st != BRKGA_OK) break;
@@ -435,12 +446,14 @@ brkga_status Fleet::tune(int reps, bool only_new, hipStream_t s)
         buckets_[bi].chunk = best.chunk;
         buckets_[bi].xys = best.xys;
         buckets_[bi].lds = bucket_lds(descs_, buckets_[bi], best.xys);
+        buckets_[bi].child_us = best.child_us;
         std::lock_guard<std::mutex> lock(eng_->tuned_mu);
         eng_->tuned[{buckets_[bi].kind, buckets_[bi].key}] = best;
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (st != BRKGA_OK) return st;
+    assign_lanes();
     drop_graphs();
     return build_items(s);
 }

@@ -30,7 +30,9 @@ struct FleetBucket {
     int block, chunk;  // launch shape; results do not depend on it
     bool xys;          // coordinates staged in LDS (coordinate kind only)
     int item_begin, item_count, grid;
-    double work;       // sum of population x n over the members
+    double work;       // lane-balancing cost (see assign_lanes)
+    int children;      // child slots of the members (sum of populations)
+    double child_us;   // measured step time per child slot, 0 if not tuned
     int lane;          // stream lane of the bucket's launch (0 = caller's stream)
 };
 

@@ -20,6 +20,8 @@
 //
 // Key rows are [pop][ld] FP32 with ld a multiple of 32, so every row starts
 // 128-byte aligned and is streamed as float4.
+#include <hip/hip_ext.h>
+
 #include "launch.hpp"
 #include "step_device.hpp"
 
@@ -361,29 +363,42 @@ hipError_t eval_kind(int block, const ProblemDesc& p, const float* keys, long lo
     }
 }
 
+template <typename K, typename... Args>
+void launch_maybe_timed(K kernel, dim3 grid, dim3 block, int lds, hipStream_t s, hipEvent_t start, hipEvent_t stop,
+                        Args... args)
+{
+    if (start || stop)
+        hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)lds, s, start, stop, 0u, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+}
+
 template <int KIND, int BLOCK>
-hipError_t step_single(const InstDesc& d, int lds, int cur, int init, hipStream_t s)
+hipError_t step_single(const InstDesc& d, int lds, int cur, int init, hipStream_t s, hipEvent_t start,
+                       hipEvent_t stop)
 {
     if (kind_sorts(KIND) && d.prob.n <= kCompactItems * BLOCK) {
         ensure_lds(k_step_one<KIND, BLOCK, kCompactItems>, lds);
-        hipLaunchKernelGGL((k_step_one<KIND, BLOCK, kCompactItems>), dim3(d.pop), dim3(BLOCK), lds, s, d, cur,
-                           init);
+        launch_maybe_timed(k_step_one<KIND, BLOCK, kCompactItems>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop, d,
+                           cur, init);
     } else {
         ensure_lds(k_step_one<KIND, BLOCK, kSortItems>, lds);
-        hipLaunchKernelGGL((k_step_one<KIND, BLOCK, kSortItems>), dim3(d.pop), dim3(BLOCK), lds, s, d, cur, init);
+        launch_maybe_timed(k_step_one<KIND, BLOCK, kSortItems>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop, d, cur,
+                           init);
     }
     return hipGetLastError();
 }
 
 template <int KIND>
-hipError_t step_kind(int block, const InstDesc& d, int lds, int cur, int init, hipStream_t s)
+hipError_t step_kind(int block, const InstDesc& d, int lds, int cur, int init, hipStream_t s, hipEvent_t start,
+                     hipEvent_t stop)
 {
     switch (block) {
-    case 64: return step_single<KIND, 64>(d, lds, cur, init, s);
-    case 128: return step_single<KIND, 128>(d, lds, cur, init, s);
-    case 256: return step_single<KIND, 256>(d, lds, cur, init, s);
-    case 512: return step_single<KIND, 512>(d, lds, cur, init, s);
-    case 1024: return step_single<KIND, 1024>(d, lds, cur, init, s);
+    case 64: return step_single<KIND, 64>(d, lds, cur, init, s, start, stop);
+    case 128: return step_single<KIND, 128>(d, lds, cur, init, s, start, stop);
+    case 256: return step_single<KIND, 256>(d, lds, cur, init, s, start, stop);
+    case 512: return step_single<KIND, 512>(d, lds, cur, init, s, start, stop);
+    case 1024: return step_single<KIND, 1024>(d, lds, cur, init, s, start, stop);
     default: return hipErrorInvalidValue;
     }
 }
@@ -418,28 +433,30 @@ int step_lds_bytes(const InstDesc& d)
     return step_layout(d.prob.kind, d.prob.n, d.L, d.nb, d.Pi, false, d.prob.aux).bytes;
 }
 
-hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s)
+hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s, hipEvent_t start,
+                           hipEvent_t stop)
 {
     const int lds = step_lds_bytes(d);
     switch (d.prob.kind) {
-    case KIND_TSP_COORDS: return step_kind<KIND_TSP_COORDS>(block, d, lds, cur, init, s);
-    case KIND_TSP_MATRIX: return step_kind<KIND_TSP_MATRIX>(block, d, lds, cur, init, s);
-    case KIND_KNAP_THRESHOLD: return step_kind<KIND_KNAP_THRESHOLD>(block, d, lds, cur, init, s);
-    case KIND_KNAP_GREEDY: return step_kind<KIND_KNAP_GREEDY>(block, d, lds, cur, init, s);
-    case KIND_TSPJ: return step_kind<KIND_TSPJ>(block, d, lds, cur, init, s);
-    case KIND_CONTINUOUS: return step_kind<KIND_CONTINUOUS>(block, d, lds, cur, init, s);
-    case KIND_JOBSHOP: return step_kind<KIND_JOBSHOP>(block, d, lds, cur, init, s);
-    case KIND_VRP_DAYS: return step_kind<KIND_VRP_DAYS>(block, d, lds, cur, init, s);
+    case KIND_TSP_COORDS: return step_kind<KIND_TSP_COORDS>(block, d, lds, cur, init, s, start, stop);
+    case KIND_TSP_MATRIX: return step_kind<KIND_TSP_MATRIX>(block, d, lds, cur, init, s, start, stop);
+    case KIND_KNAP_THRESHOLD: return step_kind<KIND_KNAP_THRESHOLD>(block, d, lds, cur, init, s, start, stop);
+    case KIND_KNAP_GREEDY: return step_kind<KIND_KNAP_GREEDY>(block, d, lds, cur, init, s, start, stop);
+    case KIND_TSPJ: return step_kind<KIND_TSPJ>(block, d, lds, cur, init, s, start, stop);
+    case KIND_CONTINUOUS: return step_kind<KIND_CONTINUOUS>(block, d, lds, cur, init, s, start, stop);
+    case KIND_JOBSHOP: return step_kind<KIND_JOBSHOP>(block, d, lds, cur, init, s, start, stop);
+    case KIND_VRP_DAYS: return step_kind<KIND_VRP_DAYS>(block, d, lds, cur, init, s, start, stop);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_partition_one(const InstDesc& d, int buf, int mode, hipStream_t s)
+hipError_t launch_partition_one(const InstDesc& d, int buf, int mode, hipStream_t s, hipEvent_t start,
+                                hipEvent_t stop)
 {
     const int lds = partition_lds_bytes(d.Pi);
     ensure_lds(k_partition_one, lds);
-    hipLaunchKernelGGL(k_partition_one, dim3(d.islands * partition_parts(d.Pi)), dim3(kPartBlock), lds, s, d, buf,
-                       mode);
+    launch_maybe_timed(k_partition_one, dim3(d.islands * partition_parts(d.Pi)), dim3(kPartBlock), lds, s, start,
+                       stop, d, buf, mode);
     return hipGetLastError();
 }
 

@@ -15,14 +15,18 @@ hipError_t launch_evaluate(const ProblemDesc& p, const float* keys, long long ld
 // One BRKGA generation step of one population, descriptor passed by value.
 // cur = buffer parity of the parent generation; init=1 builds generation 0
 // (Philox keys + evaluation) in buffer 0.
-hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s);
+// start / stop (optional): events that take the kernel's own start and end
+// times (hipExtLaunchKernel), for profiling without marker packets.
+hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s,
+                           hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 
 // Stable per-island sort of the fitness in buffer `buf` into the rank order.
 // mode 0: sort only; 1: closes a generation step (bumps the step counter and
 // raises the local-search flag when the new generation is due); 2: sorts the
 // current generation again after a local search, only when the flag is
 // raised, and lowers it (buf is then taken from the step counter).
-hipError_t launch_partition_one(const InstDesc& d, int buf, int mode, hipStream_t s);
+hipError_t launch_partition_one(const InstDesc& d, int buf, int mode, hipStream_t s, hipEvent_t start = nullptr,
+                                hipEvent_t stop = nullptr);
 // buf < 0: buffer from each population's step counter.
 hipError_t launch_partition_many(const InstDesc* d_descs, int num_inst, int blocks, int max_Pi, int buf, int mode,
                                  hipStream_t s);

@@ -9,11 +9,19 @@
 #include <set>
 #include <vector>
 
+#include "brkga/configs/continuous_opt_gpu_config.hpp"
+#include "brkga/configs/job_scheduling_gpu_config.hpp"
 #include "brkga/configs/knapsack_config.hpp"
+#include "brkga/configs/multi_objective_vrp_config.hpp"
 #include "brkga/configs/multi_tsp_config.hpp"
 #include "brkga/configs/tsp_config.hpp"
 #include "brkga/configs/tspj_config.hpp"
+#include "brkga/configs/vrp_blazer_config.hpp"
+#include "brkga/configs/zdt1_config.hpp"
 #include "brkga/core/solver.hpp"
+
+#include <cstdlib>
+#include <fstream>
 
 static int failures = 0;
 #define EXPECT(cond, ...)                        \
@@ -208,6 +216,168 @@ static void test_multi_tsp()
     }
 }
 
+static void test_continuous()
+{
+    for (int f = 0; f < 3; ++f) {
+        auto cfg = f == 0 ? ContinuousOptGPUConfig<float>::create_sphere(20)
+                   : f == 1 ? ContinuousOptGPUConfig<float>::create_rastrigin(20)
+                            : ContinuousOptGPUConfig<float>::create_rosenbrock(20);
+        cfg->population_size = 128;
+        cfg->elite_size = 20;
+        cfg->mutant_size = 12;
+        cfg->max_generations = 30;
+        ContinuousOptGPUConfig<float>* raw = cfg.get();
+        const float want = abi_best(raw, 77, 30);
+        Solver<float> solver(std::move(cfg), false);
+        solver.set_seed(77);
+        solver.set_islands(1, 0, 0);
+        solver.run();
+        const auto best = solver.get_best_individual();
+        EXPECT(best.fitness == want, "continuous %d: facade best %g vs C ABI %g", f, best.fitness, want);
+        EXPECT(raw->fitness_function(best) == best.fitness, "continuous %d: callback disagrees", f);
+        const auto x = raw->decode_to_solution(best)[0];
+        const auto& g = best.get_chromosome();
+        for (std::size_t i = 0; i < x.size(); ++i) {
+            const float w = raw->get_upper_bound() - raw->get_lower_bound();
+            const float t = g[i] * w;
+            EXPECT(x[i] == t + raw->get_lower_bound(), "continuous decode x[%zu]", i);
+        }
+    }
+}
+
+static void test_job_scheduling()
+{
+    auto cfg = JobSchedulingGPUConfig<float>::create_test_instance(50, 5);
+    cfg->population_size = 128;
+    cfg->elite_size = 20;
+    cfg->mutant_size = 12;
+    cfg->max_generations = 40;
+    JobSchedulingGPUConfig<float>* raw = cfg.get();
+    const float want = abi_best(raw, 5, 40);
+    Solver<float> solver(std::move(cfg), false);
+    solver.set_seed(5);
+    solver.set_islands(1, 0, 0);
+    solver.run();
+    const auto best = solver.get_best_individual();
+    EXPECT(best.fitness == want, "job scheduling: facade best %g vs C ABI %g", best.fitness, want);
+    const auto m = raw->decode_to_solution(best)[0];
+    std::vector<float> load(5, 0.0f);
+    for (std::size_t i = 0; i < m.size(); ++i) {
+        EXPECT(m[i] >= 0 && m[i] < 5, "machine out of range");
+        load[(int)m[i]] += raw->get_job_times()[i];
+    }
+    EXPECT(*std::max_element(load.begin(), load.end()) == best.fitness, "makespan is not the largest load");
+}
+
+static void test_zdt1()
+{
+    auto cfg = std::make_unique<ZDT1Config<float>>(30);
+    cfg->max_generations = 40;
+    ZDT1Config<float>* raw = cfg.get();
+    Solver<float> solver(std::move(cfg), false);
+    solver.set_seed(9);
+    solver.run();
+    const auto front = solver.get_pareto_front();
+    EXPECT(!front.empty(), "ZDT1: empty front");
+    for (const auto& ind : front) {
+        EXPECT(raw->objective_f1(ind) == ind.objectives[0] && raw->objective_f2(ind) == ind.objectives[1],
+               "ZDT1 objective callbacks disagree");
+        EXPECT(ind.objectives[1] >= 1.0f - std::sqrt(ind.objectives[0]) - 1e-5f, "ZDT1 below the true front");
+        for (const auto& other : front) EXPECT(!other.dominates(ind), "ZDT1 front member dominated");
+    }
+}
+
+static void test_vrp_blazer()
+{
+    unsigned s = 41;
+    const int dim = 41;
+    std::vector<std::pair<float, float>> p;
+    for (int i = 0; i < dim; ++i) p.push_back({(float)(lcg(s) % 100), (float)(lcg(s) % 100)});
+    std::vector<std::vector<float>> tm(dim, std::vector<float>(dim));
+    for (int i = 0; i < dim; ++i)
+        for (int j = 0; j < dim; ++j)
+            tm[i][j] = std::sqrt((p[i].first - p[j].first) * (p[i].first - p[j].first) +
+                                 (p[i].second - p[j].second) * (p[i].second - p[j].second));
+    auto cfg = std::make_unique<VrpBlazerConfig<float>>();
+    cfg->max_stops_per_day = 6;
+    cfg->max_drive_hours_per_day = 4.0;
+    cfg->update_derived_params();
+    cfg->set_time_matrix(tm);
+    cfg->population_size = 128;
+    cfg->elite_size = 25;
+    cfg->mutant_size = 12;
+    cfg->max_generations = 40;
+    VrpBlazerConfig<float>* raw = cfg.get();
+    const float want = abi_best(raw, 13, 40);
+    Solver<float> solver(std::move(cfg), false);
+    solver.set_seed(13);
+    solver.set_islands(1, 0, 0);
+    solver.run();
+    const auto best = solver.get_best_individual();
+    EXPECT(best.fitness == want, "VRP days: facade best %g vs C ABI %g", best.fitness, want);
+    EXPECT(raw->calculate_fitness(best) == best.fitness, "VRP days callback disagrees");
+    const auto sol = raw->decode_solution(best.get_chromosome().data());
+    std::set<int> seen;
+    for (const auto& d : sol.days) {
+        EXPECT((int)d.stop_indices.size() <= 6 && !d.stop_indices.empty(), "day size");
+        EXPECT(d.total_drive_minutes <= 240.0f + 1e-3f, "day over the drive limit");
+        for (int st : d.stop_indices) seen.insert(st);
+    }
+    EXPECT((int)seen.size() + (int)sol.unvisited_stops.size() / 2 == dim - 1, "stops lost");
+    const float approx = sol.total_drive_minutes + 1000.0f * sol.unvisited_stops.size();
+    EXPECT(std::fabs(approx - best.fitness) <= 1e-5f * best.fitness, "fitness vs listed days: %g vs %g", approx,
+           best.fitness);
+    raw->export_solution_json(best, "/tmp/facade_vrp_days.json", 1.0);
+    std::ifstream in("/tmp/facade_vrp_days.json");
+    EXPECT(in.good(), "VRP JSON export missing");
+}
+
+static void test_movrp()
+{
+    unsigned s = 61;
+    const int dim = 32;
+    const char* path = "/tmp/facade_movrp.vrp";
+    {
+        std::ofstream f(path);
+        f << "NAME : A-n32-k5\nTYPE : CVRP\nDIMENSION : " << dim << "\nCAPACITY : 100\nNODE_COORD_SECTION\n";
+        for (int i = 0; i < dim; ++i) f << (i + 1) << " " << (lcg(s) % 100) << " " << (lcg(s) % 100) << "\n";
+        f << "DEMAND_SECTION\n";
+        for (int i = 0; i < dim; ++i) f << (i + 1) << " " << (i == 0 ? 0 : 1 + (int)(lcg(s) % 29)) << "\n";
+        f << "DEPOT_SECTION\n1\n-1\nEOF\n";
+    }
+    auto cfg = MultiObjectiveVRPConfig<float>::load_from_file(path);
+    EXPECT(cfg->get_num_customers() == 31 && cfg->get_num_vehicles() == 5 && cfg->get_vehicle_capacity() == 100,
+           "MO-VRP loader");
+    cfg->population_size = 64;
+    cfg->max_generations = 20;
+    MultiObjectiveVRPConfig<float>* raw = cfg.get();
+    Solver<float> solver(std::move(cfg), false);
+    solver.set_seed(3);
+    solver.run();
+    const auto front = solver.get_pareto_front();
+    EXPECT(!front.empty(), "MO-VRP: empty front");
+    for (const auto& ind : front) {
+        float obj[2];
+        raw->decode_solution(ind.get_chromosome().data(), obj);
+        EXPECT(obj[0] == ind.objectives[0] && obj[1] == ind.objectives[1], "MO-VRP decode_solution disagrees");
+        const auto routes = raw->decode_routes(ind.get_chromosome().data());
+        std::set<int> seen;
+        float longest = 0.0f;
+        for (const auto& r : routes) {
+            EXPECT(r.load <= 100, "route over capacity");
+            for (int c : r.customers) seen.insert(c);
+            longest = std::max(longest, r.distance);
+        }
+        EXPECT((int)seen.size() == 31, "customers lost");
+        const float pen = routes.size() > 5 ? (routes.size() - 5) * 500.0f : 0.0f;
+        EXPECT(std::fabs(longest + pen - ind.objectives[1]) <= 1e-5f * ind.objectives[1], "longest route");
+        for (const auto& other : front) EXPECT(!other.dominates(ind), "MO-VRP front member dominated");
+    }
+    raw->export_solution(front[0], "/tmp/facade_movrp.txt");
+    std::ifstream in("/tmp/facade_movrp.txt");
+    EXPECT(in.good(), "MO-VRP export missing");
+}
+
 int main()
 {
     int count = 0;
@@ -220,6 +390,11 @@ int main()
     test_knapsack(true);
     test_tspj();
     test_multi_tsp();
+    test_continuous();
+    test_job_scheduling();
+    test_zdt1();
+    test_vrp_blazer();
+    test_movrp();
     if (failures) {
         std::printf("%d failure(s)\n", failures);
         return 1;

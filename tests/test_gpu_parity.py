@@ -273,3 +273,34 @@ def test_nsga2_trajectory_parity(engine, n, N, G):
         pop.evolve(1)
         ost.step()
         compare_nsga(pop, ost, f"step {t}")
+
+
+def test_set_block_and_profile(engine):
+    """Any admissible workgroup size gives the same trajectory; profile()
+    runs real generations and returns positive kernel times."""
+    print("set_block / profile", flush=True)
+    gp, _ = make_problem(engine, "tsp", 300, inst=5)
+    prm = brkga_amd.Params(128, 20, 12, 0.7, 2, 3, 2, 99)
+    states = []
+    for blk in (64, 256, 1024):
+        pop = brkga_amd.Population(gp, prm)
+        pop.set_block(blk)
+        pop.initialize()
+        pop.evolve(6)
+        states.append(pop.read())
+        if blk == 256:
+            g0 = pop.generation
+            step_us, close_us = pop.profile(5)
+            assert step_us > 0 and close_us > 0
+            assert pop.generation == g0 + 5
+        pop.close()
+    for key in states[0]:
+        assert np.array_equal(states[0][key], states[1][key]) and np.array_equal(states[0][key], states[2][key])
+    pop = brkga_amd.Population(gp, prm)
+    for bad in (32, 100, 2048):
+        with pytest.raises(brkga_amd.BrkgaError):
+            pop.set_block(bad)
+    big, _ = make_problem(engine, "tsp", 2000, inst=6)
+    with pytest.raises(brkga_amd.BrkgaError):
+        brkga_amd.Population(big, prm).set_block(64)
+    pop.close()
