@@ -190,43 +190,86 @@ __host__ __device__ inline int job_machine(float gene, int M)
 
 // ------------------------------------------------------------ wave / block --
 
-__device__ __forceinline__ long long wave_sum_i64(long long v)
+// Wave-wide scans and reductions on DPP row shifts and row broadcasts (no
+// LDS round trips).  Within each 16-lane row a Hillis-Steele scan with
+// row_shr 1, 2, 4, 8; then row_bcast:15 adds row 0's (row 2's) last lane to
+// row 1 (row 3) and row_bcast:31 adds lane 31 to rows 2 and 3.  A lane whose
+// source is outside its row, or in a row the mask leaves out, reads `old`.
+// Every lane of the wave must be active.
+enum : int {
+    kDppShr1 = 0x111,
+    kDppShr2 = 0x112,
+    kDppShr4 = 0x114,
+    kDppShr8 = 0x118,
+    kDppBcast15 = 0x142,
+    kDppBcast31 = 0x143
+};
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ unsigned dpp_u32(unsigned old, unsigned v)
 {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    return (unsigned)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROW_MASK, 0xf, false);
 }
 
-__device__ __forceinline__ long long wave_max_i64(long long v)
+// 64-bit lane value from the lane CTRL names (both halves from one lane).
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long old, unsigned long long v)
 {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        long long u = __shfl_xor(v, o, kWave);
-        v = u > v ? u : v;
-    }
-    return v;
+    const unsigned lo = dpp_u32<CTRL, ROW_MASK>((unsigned)old, (unsigned)v);
+    const unsigned hi = dpp_u32<CTRL, ROW_MASK>((unsigned)(old >> 32), (unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ long long wave_read_last(long long v)
+{
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, kWave - 1);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)v >> 32), kWave - 1);
+    return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
 __device__ __forceinline__ int wave_inclusive_scan_i32(int v)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        int u = __shfl_up(v, o, kWave);
-        if (lane >= o) v += u;
-    }
-    return v;
+    unsigned x = (unsigned)v;
+    x += dpp_u32<kDppShr1>(0u, x);
+    x += dpp_u32<kDppShr2>(0u, x);
+    x += dpp_u32<kDppShr4>(0u, x);
+    x += dpp_u32<kDppShr8>(0u, x);
+    x += dpp_u32<kDppBcast15, 0xa>(0u, x);
+    x += dpp_u32<kDppBcast31, 0xc>(0u, x);
+    return (int)x;
 }
 
 __device__ __forceinline__ long long wave_inclusive_scan_i64(long long v)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        long long u = __shfl_up(v, o, kWave);
-        if (lane >= o) v += u;
-    }
-    return v;
+    unsigned long long x = (unsigned long long)v;
+    x += dpp_u64<kDppShr1>(0ull, x);
+    x += dpp_u64<kDppShr2>(0ull, x);
+    x += dpp_u64<kDppShr4>(0ull, x);
+    x += dpp_u64<kDppShr8>(0ull, x);
+    x += dpp_u64<kDppBcast15, 0xa>(0ull, x);
+    x += dpp_u64<kDppBcast31, 0xc>(0ull, x);
+    return (long long)x;
+}
+
+// Sum over the wave, in every lane.
+__device__ __forceinline__ long long wave_sum_i64(long long v)
+{
+    return wave_read_last(wave_inclusive_scan_i64(v));
+}
+
+// Maximum over the wave, in every lane (a lane with no source keeps its own
+// value, the identity of max).
+__device__ __forceinline__ long long wave_max_i64(long long v)
+{
+    long long x = v;
+    auto step = [&](unsigned long long u) { x = (long long)u > x ? (long long)u : x; };
+    step(dpp_u64<kDppShr1>((unsigned long long)x, (unsigned long long)x));
+    step(dpp_u64<kDppShr2>((unsigned long long)x, (unsigned long long)x));
+    step(dpp_u64<kDppShr4>((unsigned long long)x, (unsigned long long)x));
+    step(dpp_u64<kDppShr8>((unsigned long long)x, (unsigned long long)x));
+    step(dpp_u64<kDppBcast15, 0xa>((unsigned long long)x, (unsigned long long)x));
+    step(dpp_u64<kDppBcast31, 0xc>((unsigned long long)x, (unsigned long long)x));
+    return wave_read_last(x);
 }
 
 // Block sum; every thread gets the total. `red` holds BLOCK/64 int64.
@@ -270,7 +313,7 @@ __device__ __forceinline__ int block_exclusive_scan_i32(int v, int* red, int* to
     const int inc = wave_inclusive_scan_i32(v);
     const int wid = threadIdx.x / kWave;
     if (W == 1) {
-        *total = __shfl(inc, kWave - 1, kWave);
+        *total = __builtin_amdgcn_readlane(inc, kWave - 1);
         return inc - v;
     }
     if ((threadIdx.x & (kWave - 1)) == kWave - 1) red[wid] = inc;
