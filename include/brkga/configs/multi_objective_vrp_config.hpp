@@ -15,8 +15,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <filesystem>
 #include <fstream>
 #include <iomanip>
+#include <iostream>
 #include <memory>
 #include <numeric>
 #include <sstream>
@@ -209,6 +211,63 @@ public:
             << "\nAverage route: " << std::fixed << std::setprecision(2) << (total / routes.size())
             << "\nBalance ratio (max/avg): " << std::fixed << std::setprecision(2)
             << (longest / (total / routes.size())) << "\n";
+    }
+
+    // configs/multi_objective_vrp_config.hpp:177-192
+    void print_instance_info() const
+    {
+        std::cout << "\n=== VRP Instance Information ===\nInstance: " << instance_name_
+                  << "\nCustomers: " << num_customers_ << "\nVehicles: " << num_vehicles_
+                  << "\nVehicle capacity: " << vehicle_capacity_ << "\nChromosome size: " << num_customers_
+                  << "\nOptimization: Multi-Objective (Distance vs. Max Route)\n"
+                  << "  Objective 1: Total distance (minimize)\n"
+                  << "  Objective 2: Longest single route (minimize - minimax)\n\nTrade-off explanation:\n"
+                  << "  - Minimize total: May create very long routes for some vehicles\n"
+                  << "  - Minimize longest: Ensures no vehicle is overworked (fairness)\n"
+                  << "================================" << std::endl;
+    }
+
+    // configs/multi_objective_vrp_config.hpp:369-386: one solution file per
+    // front member plus the route data file, in `directory`
+    void export_all_pareto_solutions(const std::vector<Individual<Float>>& pareto_front, const std::string& directory)
+    {
+        std::filesystem::create_directories(directory);
+        std::cout << "\nExporting " << pareto_front.size() << " Pareto solutions..." << std::endl;
+        for (std::size_t i = 0; i < pareto_front.size(); ++i) {
+            std::stringstream name;
+            name << directory << "/solution_" << std::setw(3) << std::setfill('0') << i << ".txt";
+            export_solution(pareto_front[i], name.str());
+        }
+        export_pareto_routes_data(pareto_front, directory + "/routes_data.txt");
+        std::cout << "All solutions exported to: " << directory << "/" << std::endl;
+    }
+
+    // configs/multi_objective_vrp_config.hpp:389-446
+    void export_pareto_routes_data(const std::vector<Individual<Float>>& pareto_front, const std::string& filename)
+    {
+        std::ofstream out(filename);
+        if (!out.is_open()) throw std::runtime_error("Cannot open output file: " + filename);
+        out << "# VRP Routes Data for Visualization\n# Instance: " << instance_name_ << "\n# Customers: "
+            << num_customers_ << "\n# Vehicles: " << num_vehicles_ << "\n#\n# Format:\n"
+            << "# SOLUTION <idx> <obj1> <obj2>\n# ROUTE <route_idx> <distance> <load>\n"
+            << "# CUSTOMER <customer_id> <x> <y> <demand>\n#\n\n";
+        const Customer& depot = customers_[0];
+        for (std::size_t si = 0; si < pareto_front.size(); ++si) {
+            const auto& ind = pareto_front[si];
+            const auto routes = decode_routes(ind.get_chromosome().data());
+            out << "SOLUTION " << si << " " << std::fixed << std::setprecision(2) << ind.objectives[0] << " "
+                << ind.objectives[1] << "\n";
+            for (std::size_t r = 0; r < routes.size(); ++r) {
+                out << "ROUTE " << r << " " << std::fixed << std::setprecision(2) << routes[r].distance << " "
+                    << routes[r].load << "\n";
+                out << "CUSTOMER 0 " << depot.x << " " << depot.y << " " << depot.demand << "\n";
+                for (int c : routes[r].customers)
+                    out << "CUSTOMER " << c << " " << customers_[c].x << " " << customers_[c].y << " "
+                        << customers_[c].demand << "\n";
+                out << "CUSTOMER 0 " << depot.x << " " << depot.y << " " << depot.demand << "\nEND_ROUTE\n";
+            }
+            out << "END_SOLUTION\n\n";
+        }
     }
 
     int native_kind() const override { return BRKGA_MOVRP; }

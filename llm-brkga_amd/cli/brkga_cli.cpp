@@ -15,6 +15,7 @@
 #include <string>
 
 #include "brkga/configs/knapsack_config.hpp"
+#include "brkga/configs/multi_objective_vrp_config.hpp"
 #include "brkga/configs/multi_tsp_config.hpp"
 #include "brkga/configs/tsp_config.hpp"
 #include "brkga/configs/tspj_config.hpp"
@@ -44,9 +45,11 @@ void print_usage(const char* prog)
               << "    knapsack_config   - 0/1 Knapsack Problem (text or CSV)\n"
               << "\n  Multi-Objective (NSGA-II):\n"
               << "    multi_tsp_config  - Multi-objective TSP (distance + time)\n"
+              << "    multi_objective_vrp_config - Multi-objective VRP (distance + max route)\n"
               << "\nExamples:\n"
               << "  " << prog << " --data data/berlin52.tsp --config tsp_config\n"
               << "  " << prog << " --data data/berlin52.tsp --config multi_tsp_config --pop 512\n"
+              << "  " << prog << " --data data/A-n32-k5.vrp --config multi_objective_vrp_config\n"
               << "  " << prog << " --data data/knapsack.txt --config knapsack_config --gen 500\n";
 }
 
@@ -82,6 +85,12 @@ std::unique_ptr<BRKGAConfig<float>> load(const Options& o)
         auto c = MultiTSPConfig<float>::load_from_file(o.data);
         c->print_instance_info();
         MultiTSPConfig<float>::configure_for_size(c.get(), c->get_num_cities());
+        return c;
+    }
+    if (o.config == "multi_objective_vrp_config") {
+        auto c = MultiObjectiveVRPConfig<float>::load_from_file(o.data);
+        c->print_instance_info();
+        MultiObjectiveVRPConfig<float>::configure_for_size(c.get(), c->get_num_customers());
         return c;
     }
     throw std::runtime_error("Unknown config type: " + o.config);
@@ -134,6 +143,13 @@ int run(const Options& o)
         solver.export_pareto_front(front_file);
         std::cout << "\n✓ Pareto front exported to: " << front_file << std::endl;
         const auto pareto = solver.get_pareto_front();
+        if (o.config == "multi_objective_vrp_config" && !pareto.empty()) {
+            auto* vrp = dynamic_cast<MultiObjectiveVRPConfig<float>*>(solver.get_config());
+            if (vrp) {
+                vrp->export_all_pareto_solutions(pareto, "pareto_solutions");
+                std::cout << "\n✓ All " << pareto.size() << " solutions exported to: pareto_solutions/" << std::endl;
+            }
+        }
         std::cout << "\n=== Pareto Front Analysis ===\nSolutions in Pareto front: " << pareto.size() << std::endl;
         if (!pareto.empty()) {
             std::cout << "\nFirst 5 solutions:\n  Obj1      Obj2\n  ----------------------" << std::endl;

@@ -85,11 +85,13 @@ __device__ __forceinline__ float fx_to_float(long long acc, float inv_scale)
 // and below y - u'/2 iff d <= -y*u' (midpoint squares are never floats, so
 // there are no ties).  V_SQRT_F32 is accurate to one ulp, so one correction
 // suffices; inputs whose ulp would be subnormal take the FP64 midpoint test.
+// Zeros, negatives, infinities and NaN: the hardware result is already the
+// IEEE one (the correction leaves infinities and NaN unchanged; zeros are
+// selected back, since a zero y has no ulp to test against).
 __device__ __forceinline__ float sqrt_rn(float x)
 {
     float y = __builtin_amdgcn_sqrtf(x);
-    if (!(x > 0.0f) || !(x < 3.0e38f)) return __builtin_sqrtf(x);
-    if (x < 1.0e-30f) {
+    if (__builtin_expect(x > 0.0f && x < 1.0e-30f, 0)) {
         const double xd = (double)x;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
@@ -109,9 +111,8 @@ __device__ __forceinline__ float sqrt_rn(float x)
     const float u = __uint_as_float((b & 0x7F800000u) - (23u << 23));  // ulp(y)
     const float ud = (b & 0x007FFFFFu) ? u : __uint_as_float((b & 0x7F800000u) - (24u << 23));
     const float d = __builtin_fmaf(-y, y, x);
-    if (d > y * u) y = __uint_as_float(b + 1u);
-    else if (d <= -(y * ud)) y = __uint_as_float(b - 1u);
-    return y;
+    const float yc = d > y * u ? __uint_as_float(b + 1u) : (d <= -(y * ud) ? __uint_as_float(b - 1u) : y);
+    return x == 0.0f ? x : yc;
 }
 
 // Euclidean edge in FP32 with separately rounded products and sum

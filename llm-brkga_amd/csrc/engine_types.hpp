@@ -154,8 +154,8 @@ __host__ __device__ inline DecodeLayout decode_layout(int kind, int n, int L, in
 }
 
 // Keys one decode thread holds: a workgroup of B threads decodes n <= 16 B.
-// Kernels also come in a compact form for n <= 4 B, which holds fewer
-// registers.
+// The step kernels also come in forms for n <= 4 B and n <= 8 B, which hold
+// fewer registers.
 constexpr int kSortItems = 16;
 constexpr int kCompactItems = 4;
 

@@ -381,6 +381,10 @@ hipError_t step_single(const InstDesc& d, int lds, int cur, int init, hipStream_
         ensure_lds(k_step_one<KIND, BLOCK, kCompactItems>, lds);
         launch_maybe_timed(k_step_one<KIND, BLOCK, kCompactItems>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop, d,
                            cur, init);
+    } else if (kind_sorts(KIND) && d.prob.n <= 2 * kCompactItems * BLOCK) {
+        ensure_lds(k_step_one<KIND, BLOCK, 2 * kCompactItems>, lds);
+        launch_maybe_timed(k_step_one<KIND, BLOCK, 2 * kCompactItems>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop,
+                           d, cur, init);
     } else {
         ensure_lds(k_step_one<KIND, BLOCK, kSortItems>, lds);
         launch_maybe_timed(k_step_one<KIND, BLOCK, kSortItems>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop, d, cur,

@@ -37,7 +37,8 @@ def test_usage_and_argument_errors(cli):
 
 
 CASES = [("tsp_config", "tiny8.tsp"), ("knapsack_config", "knap_small.txt"), ("knapsack_config", "knap_cap.csv"),
-         ("tspj_config", "mini_TT.csv"), ("multi_tsp_config", "tiny8.tsp")]
+         ("tspj_config", "mini_TT.csv"), ("multi_tsp_config", "tiny8.tsp"),
+         ("multi_objective_vrp_config", "tiny-n12-k3.vrp")]
 
 
 @pytest.mark.gpu
@@ -50,10 +51,14 @@ def test_cli_solves(cli, tmp_path, config, data):
     out = r.stdout
     assert "Optimization completed successfully" in out
     assert int(re.search(r"Generations:\s*(\d+)", out).group(1)) == 20
-    if config == "multi_tsp_config":
+    if config in ("multi_tsp_config", "multi_objective_vrp_config"):
         assert int(re.search(r"Pareto front size:\s*(\d+)", out).group(1)) >= 1
-        front = (tmp_path / "multi_tsp_config_pareto_front.txt").read_text()
+        front = (tmp_path / f"{config}_pareto_front.txt").read_text()
         assert "# Pareto Front" in front
+        if config == "multi_objective_vrp_config":
+            routes = (tmp_path / "pareto_solutions" / "routes_data.txt").read_text()
+            assert "SOLUTION 0" in routes and "END_ROUTE" in routes
+            assert (tmp_path / "pareto_solutions" / "solution_000.txt").exists()
     else:
         assert re.search(r"Best fitness:\s*([-\d.]+)", out)
         assert (tmp_path / "solutions" / f"{config}_solution.txt").exists()
