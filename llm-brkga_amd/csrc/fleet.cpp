@@ -2,7 +2,9 @@
 #include "fleet.hpp"
 
 #include <algorithm>
+#include <map>
 #include <numeric>
+#include <tuple>
 
 #include "launch.hpp"
 
@@ -176,12 +178,13 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
         if ((size_t)b.lds > eng_->lds_limit)
             return set_error(BRKGA_ERR_INVALID_ARG, "fleet member too large for the LDS kernels");
     }
+    form_launches();
     assign_lanes();
     drop_graphs();
     brkga_status st = reserve(reinterpret_cast<void**>(&d_descs_), &descs_cap_, sizeof(InstDesc) * descs_.size(),
                               "fleet descriptors");
     if (st != BRKGA_OK) return st;
-    st = reserve(reinterpret_cast<void**>(&d_queues_), &queues_cap_, sizeof(int) * kQueueStride * buckets_.size(),
+    st = reserve(reinterpret_cast<void**>(&d_queues_), &queues_cap_, sizeof(int) * kQueueStride * 2 * buckets_.size(),
                  "fleet queues");
     if (st != BRKGA_OK) return st;
     if (!descs_.empty())
@@ -204,18 +207,31 @@ brkga_status Fleet::build_items(hipStream_t s)
     BRKGA_HIP_TRY(hipStreamSynchronize(s));
     std::vector<int4>& items = items_;
     items.clear();
-    for (auto& b : buckets_) {
-        b.item_begin = (int)items.size();
-        for (int m = b.first; m < b.first + b.count; ++m) {
-            const InstDesc& d = descs_[m];
-            for (int isl = 0; isl < d.islands; ++isl)
-                for (int s0 = 0; s0 < d.Pi; s0 += b.chunk)
-                    items.push_back(make_int4(m, isl * d.Pi + s0, isl * d.Pi + std::min(s0 + b.chunk, d.Pi), 0));
+    for (auto& L : launches_) {
+        L.item_begin = (int)items.size();
+        L.n_max = 0;
+        L.lds = 0;
+        for (int bi : L.buckets) {
+            FleetBucket& b = buckets_[bi];
+            b.item_begin = (int)items.size();
+            for (int m = b.first; m < b.first + b.count; ++m) {
+                const InstDesc& d = descs_[m];
+                for (int isl = 0; isl < d.islands; ++isl)
+                    for (int s0 = 0; s0 < d.Pi; s0 += b.chunk)
+                        items.push_back(
+                            make_int4(m, isl * d.Pi + s0, isl * d.Pi + std::min(s0 + b.chunk, d.Pi), 0));
+            }
+            b.item_count = (int)items.size() - b.item_begin;
+            const int per_cu = step_queue_blocks_per_cu(b.kind, b.block, b.xys, b.n_max, b.lds);
+            if (per_cu <= 0) return set_error(BRKGA_ERR_DEVICE, "queue step kernel does not fit on a CU");
+            b.grid = std::min(b.item_count, per_cu * eng_->cu_count);
+            L.n_max = std::max(L.n_max, b.n_max);
+            L.lds = std::max(L.lds, b.lds);
         }
-        b.item_count = (int)items.size() - b.item_begin;
-        const int per_cu = step_queue_blocks_per_cu(b.kind, b.block, b.xys, b.n_max, b.lds);
+        L.item_count = (int)items.size() - L.item_begin;
+        const int per_cu = step_queue_blocks_per_cu(L.kind, L.block, L.xys, L.n_max, L.lds);
         if (per_cu <= 0) return set_error(BRKGA_ERR_DEVICE, "queue step kernel does not fit on a CU");
-        b.grid = std::min(b.item_count, per_cu * eng_->cu_count);
+        L.grid = std::min(L.item_count, per_cu * eng_->cu_count);
     }
     brkga_status st =
         reserve(reinterpret_cast<void**>(&d_items_), &items_cap_, sizeof(int4) * items.size(), "fleet work items");
@@ -225,27 +241,64 @@ brkga_status Fleet::build_items(hipStream_t s)
     return BRKGA_OK;
 }
 
-// Largest-work-first onto the least-loaded lane; with NSGA-II members the
-// last lane is theirs.  The work of a bucket is its measured time (child
-// slots x tuned time per child) when every bucket has been tuned, else
-// population x (n + kChildFixedWork).
+// Items per thread of the queue-step kernel a bucket's shape selects (the
+// same rule as the kernel choice in kernels_queue.hip).
+static int queue_items(int kind, int block, int n_max)
+{
+    if (!kind_sorts(kind)) return kSortItems;
+    if (n_max <= kCompactItems * block) return kCompactItems;
+    if (n_max <= 2 * kCompactItems * block) return 2 * kCompactItems;
+    return kSortItems;
+}
+
+// Buckets whose shapes select the same kernel share a launch; buckets stay
+// in their order (largest n first) inside it.
+void Fleet::form_launches()
+{
+    launches_.clear();
+    std::map<std::tuple<int, int, bool, int>, int> index;
+    for (int i = 0; i < (int)buckets_.size(); ++i) {
+        const FleetBucket& b = buckets_[i];
+        const auto key = std::make_tuple(b.kind, b.block, b.xys, queue_items(b.kind, b.block, b.n_max));
+        auto it = index.find(key);
+        if (it == index.end()) {
+            FleetLaunch L{};
+            L.kind = b.kind;
+            L.block = b.block;
+            L.xys = b.xys;
+            launches_.push_back(L);
+            it = index.emplace(key, (int)launches_.size() - 1).first;
+        }
+        launches_[it->second].buckets.push_back(i);
+    }
+}
+
+// Largest-work-first onto the least-loaded lane, one launch at a time; with
+// NSGA-II members the last lane is theirs.  The work of a bucket is its
+// measured time (child slots x tuned time per child) when every bucket has
+// been tuned, else population x (n + kChildFixedWork).
 void Fleet::assign_lanes()
 {
     const int brkga_lanes = groups_.empty() ? kLanes : kLanes - 1;
     nsga_lane_ = groups_.empty() ? 0 : (buckets_.empty() ? 0 : kLanes - 1);
     bool measured = !buckets_.empty();
     for (const auto& b : buckets_) measured = measured && b.child_us > 0.0;
-    std::vector<int> idx(buckets_.size());
+    for (auto& L : launches_) {
+        L.cost = 0.0;
+        for (int bi : L.buckets)
+            L.cost += measured ? buckets_[bi].children * buckets_[bi].child_us : buckets_[bi].work;
+    }
+    std::vector<int> idx(launches_.size());
     std::iota(idx.begin(), idx.end(), 0);
-    auto cost = [&](int i) { return measured ? buckets_[i].children * buckets_[i].child_us : buckets_[i].work; };
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return launches_[a].cost > launches_[b].cost; });
     double load[kLanes] = {};
     for (int i : idx) {
         int best = 0;
         for (int k = 1; k < brkga_lanes; ++k)
             if (load[k] < load[best]) best = k;
-        buckets_[i].lane = best;
-        load[best] += cost(i);
+        launches_[i].lane = best;
+        for (int bi : launches_[i].buckets) buckets_[bi].lane = best;
+        load[best] += launches_[i].cost;
     }
 }
 
@@ -275,7 +328,7 @@ brkga_status Fleet::tick(hipStream_t s)
     brkga_status st = ensure_lanes();
     if (st != BRKGA_OK) return st;
     bool used[kLanes] = {};
-    for (const auto& b : buckets_) used[b.lane] = true;
+    for (const auto& L : launches_) used[L.lane] = true;
     if (!groups_.empty()) used[nsga_lane_] = true;
     auto lane = [&](int k) { return k == 0 ? s : lanes_[k]; };
     hipError_t e = hipSuccess;
@@ -286,14 +339,18 @@ brkga_status Fleet::tick(hipStream_t s)
             forked = true;
             if (e == hipSuccess) e = hipStreamWaitEvent(lanes_[k], fork_, 0);
         }
-    // Buckets are stored largest n first; each lane launches its smallest
-    // (latency-bound) buckets first, so the kernels still running at the end
-    // of the tick are the ones with the most parallel work.
-    for (size_t r = buckets_.size(); r-- > 0 && e == hipSuccess;) {
-        const size_t i = r;
-        const FleetBucket& b = buckets_[i];
-        e = launch_step_queue(b.kind, b.block, b.xys, b.n_max, d_descs_, d_items_ + b.item_begin, b.item_count,
-                              d_queues_ + kQueueStride * i, b.grid, b.lds, lane(b.lane));
+    // Each lane launches its smallest-n (latency-bound) launches first, so
+    // the kernels still running at the end of the tick are the ones with the
+    // most parallel work.
+    std::vector<int> order(launches_.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return launches_[a].n_max < launches_[b].n_max; });
+    for (size_t r = 0; r < order.size() && e == hipSuccess; ++r) {
+        const int j = order[r];
+        const FleetLaunch& L = launches_[j];
+        e = launch_step_queue(L.kind, L.block, L.xys, L.n_max, d_descs_, d_items_ + L.item_begin, L.item_count,
+                              d_queues_ + kQueueStride * (buckets_.size() + j), L.grid, L.lds, lane(L.lane));
     }
     for (size_t i = 0; i < groups_.size() && e == hipSuccess; ++i) {
         const FleetGroup& gr = groups_[i];
@@ -453,6 +510,7 @@ brkga_status Fleet::tune(int reps, bool only_new, hipStream_t s)
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (st != BRKGA_OK) return st;
+    form_launches();
     assign_lanes();
     drop_graphs();
     return build_items(s);

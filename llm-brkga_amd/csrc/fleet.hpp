@@ -3,9 +3,10 @@
 // served forms.
 //
 // BRKGA members are grouped into buckets by (problem kind, n rounded up to
-// a power of two); each bucket runs one persistent queue-step launch per
-// tick with its own workgroup size and work-item size (slots per item),
-// which the autotuner picks by timing.  One partition launch covers every
+// a power of two); each bucket has its own workgroup size and work-item
+// size (slots per item), which the autotuner picks by timing.  Buckets that
+// end up with the same queue-step kernel share one persistent launch per
+// tick (their work items side by side, largest n first).  One partition launch covers every
 // BRKGA member; NSGA-II members run the offspring / selection / gather
 // launches per problem kind.  Members may sit at different generations:
 // each kernel reads the member's step counter (-1 = build generation 0) and
@@ -34,6 +35,18 @@ struct FleetBucket {
     int children;      // child slots of the members (sum of populations)
     double child_us;   // measured step time per child slot, 0 if not tuned
     int lane;          // stream lane of the bucket's launch (0 = caller's stream)
+};
+
+// One queue-step launch per tick: the buckets whose shapes select the same
+// kernel instantiation, their work items contiguous.
+struct FleetLaunch {
+    int kind, block;
+    bool xys;
+    int n_max, lds;
+    int item_begin, item_count, grid;
+    double cost;  // sum of the buckets' lane-balancing costs
+    int lane;
+    std::vector<int> buckets;  // indices into the bucket list, largest n first
 };
 
 struct FleetGroup {  // NSGA-II members of one problem kind
@@ -77,6 +90,7 @@ public:
 
 private:
     brkga_status build_items(hipStream_t s);
+    void form_launches();
     brkga_status tune(int reps, bool only_new, hipStream_t s);
     brkga_status reserve(void** p, size_t* cap, size_t bytes, const char* what);
     brkga_status ensure_lanes();
@@ -88,6 +102,7 @@ private:
     std::vector<InstDesc> descs_;
     std::vector<int4> items_;  // host copy; kept until the next upload has been ordered after this one
     std::vector<FleetBucket> buckets_;
+    std::vector<FleetLaunch> launches_;
     std::vector<FleetGroup> groups_;
     int n_brkga_ = 0, total_islands_ = 0, max_Pi_ = 0;
     using Shape = brkga_engine_s::Shape;
@@ -101,7 +116,7 @@ private:
     int ls_items_ = 0, ls_block_ = 0, ls_lds_ = 0;
     int4* d_items_ = nullptr;
     size_t items_cap_ = 0;
-    int* d_queues_ = nullptr;  // two counters per bucket, 128 bytes apart
+    int* d_queues_ = nullptr;  // two counters per bucket (tuning launches), then per launch; 128 bytes apart
     size_t queues_cap_ = 0;
     hipStream_t capture_ = nullptr;
     std::map<int, hipGraphExec_t> graphs_;
