@@ -241,16 +241,6 @@ brkga_status Fleet::build_items(hipStream_t s)
     return BRKGA_OK;
 }
 
-// Items per thread of the queue-step kernel a bucket's shape selects (the
-// same rule as the kernel choice in kernels_queue.hip).
-static int queue_items(int kind, int block, int n_max)
-{
-    if (!kind_sorts(kind)) return kSortItems;
-    if (n_max <= kCompactItems * block) return kCompactItems;
-    if (n_max <= 2 * kCompactItems * block) return 2 * kCompactItems;
-    return kSortItems;
-}
-
 // Buckets whose shapes select the same kernel share a launch; buckets stay
 // in their order (largest n first) inside it.
 void Fleet::form_launches()
@@ -259,7 +249,7 @@ void Fleet::form_launches()
     std::map<std::tuple<int, int, bool, int>, int> index;
     for (int i = 0; i < (int)buckets_.size(); ++i) {
         const FleetBucket& b = buckets_[i];
-        const auto key = std::make_tuple(b.kind, b.block, b.xys, queue_items(b.kind, b.block, b.n_max));
+        const auto key = std::make_tuple(b.kind, b.block, b.xys, queue_items_for(b.kind, b.block, b.n_max));
         auto it = index.find(key);
         if (it == index.end()) {
             FleetLaunch L{};

@@ -1,105 +1,18 @@
-// kernels_brkga.hip - BRKGA generation step for gfx950.
+// kernels_brkga.hip - per-island ranking and TSPJ local search for gfx950.
 //
-//   k_evaluate       decode + fitness of caller-owned key rows: the
-//                    evaluate_population_gpu hook (core/config.hpp:50-53).
-//   k_step_one       one workgroup per child slot: elite / migrant row copy,
-//                    mutant keys from Philox, or biased uniform crossover,
-//                    then decode + fitness of the new row straight from LDS.
-//                    Replaces brkga_generation_kernel (core/cuda_kernels.cuh:88-138)
-//                    fused with the config fitness kernels, and the host-staged
-//                    ring migration (core/solver.hpp:517-558).
 //   k_partition_*    stable sort of each island's fitness -> rank order;
 //                    replaces thrust::sequence + sort_by_key (core/solver.hpp:416-419).
+//   k_tspj_local_search  2-opt moves on each island's top elites
+//                    (configs/tspj_config.hpp:77-135).
 //
-// The *_one kernels take the population descriptor by value (kernel
-// arguments); k_partition_many reads one descriptor per instance from
-// device memory (batched and served populations).  `cur` is the buffer
-// parity of the parent generation, known to the launcher; the generation
-// counter in device memory only feeds the Philox counter and the migration
-// schedule, so a captured graph can be replayed.
-//
-// Key rows are [pop][ld] FP32 with ld a multiple of 32, so every row starts
-// 128-byte aligned and is streamed as float4.
-#include <hip/hip_ext.h>
-
+// k_partition_one takes the population descriptor by value;
+// k_partition_many reads one descriptor per instance from device memory
+// (batched and served populations).
 #include "launch.hpp"
+#include "launch_util.hpp"
 #include "step_device.hpp"
 
 namespace brkga {
-
-template <int KIND, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_evaluate(ProblemDesc p, const float* __restrict__ keys, long long ld,
-                                                    int count, float* __restrict__ fit, int* __restrict__ decoded)
-{
-    extern __shared__ __align__(16) char lds[];
-    const int row = blockIdx.x;
-    if (row >= count) return;
-    const int L = p.chrom_len;
-    const int nb = sort_buckets(p.n);
-    const DecodeLayout lay = decode_layout(KIND, p.n, L, nb, p.aux);
-    float* k = reinterpret_cast<float*>(lds + lay.keys_off);
-    const float* src = keys + (size_t)row * ld;
-    for (int j = threadIdx.x; j < L; j += BLOCK) k[j] = src[j];
-    __syncthreads();
-    decode_block<KIND, BLOCK>(p, lds, lay, nb, fit ? fit + (size_t)row * p.num_obj : nullptr,
-                              decoded ? decoded + (size_t)row * L : nullptr);
-}
-
-// One block of a generation step: slot gs of the population described by d.
-// n <= ITEMS * BLOCK.
-template <int KIND, int BLOCK, int ITEMS>
-__device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, int init, char* lds)
-{
-    const int Pi = d.Pi, Ei = d.Ei, I = d.islands;
-    const int island = gs / Pi;
-    const int s = gs - island * Pi;
-    const int nxt = init ? 0 : cur ^ 1;
-    const int M = d.prob.num_obj;
-    const int ld = d.ld, L = d.L, n = d.prob.n;
-    const int nq = (L + 3) >> 2;
-    float4* out = reinterpret_cast<float4*>(d.keys[nxt] + (size_t)gs * ld);
-    float* fit_out = d.fit[nxt] + (size_t)gs * M;
-    const int* perm_cur = d.perm[cur] + (size_t)island * Pi;
-    const int k = d.mig_k;
-    const bool maybe_migrant = !init && I > 1 && k > 0 && d.mig_interval > 0 && s >= Pi - k;
-
-    // elite slot that no migration can claim: copy row + fitness, no decode
-    if (!init && !maybe_migrant && s < Ei) {
-        const int src = island * Pi + perm_cur[s];
-        const float4* in = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)src * ld);
-        for (int c = threadIdx.x; c < nq; c += BLOCK) out[c] = in[c];
-        if (threadIdx.x < M) fit_out[threadIdx.x] = d.fit[cur][(size_t)src * M + threadIdx.x];
-        return;
-    }
-
-    const int t = init ? 0 : *d.gen;
-    const int nb = d.nb;
-    const StepLayout sl = step_layout(KIND, n, L, nb, Pi, false, d.prob.aux);
-    float4* kl = reinterpret_cast<float4*>(lds + sl.dec.keys_off);
-    int* order = reinterpret_cast<int*>(lds + sl.isl_off);
-    const bool staged = Pi <= kMaxStagedOrder;
-    SortScratch ss;
-    ss.cnt = reinterpret_cast<int*>(lds + sl.dec.cnt_off);
-    ss.red = reinterpret_cast<long long*>(lds + sl.dec.red_off);
-
-    // stage the island rank order, clear the argsort counters
-    if (!init && staged)
-        for (int i = threadIdx.x; i < Pi; i += BLOCK) order[i] = perm_cur[i];
-    argsort_clear<BLOCK>(ss, nb);
-    __syncthreads();
-
-    const bool child = build_child<BLOCK>(d, island, s, cur, init, t, staged ? order : perm_cur, out, kl, fit_out);
-    if (!child) return;
-    __syncthreads();
-    decode_block<KIND, BLOCK, true, false, ITEMS>(d.prob, lds, sl.dec, nb, fit_out, nullptr);
-}
-
-template <int KIND, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_step_one(InstDesc d, int cur, int init)
-{
-    extern __shared__ __align__(16) char lds[];
-    step_body<KIND, BLOCK, ITEMS>(d, blockIdx.x, cur, init, lds);
-}
 
 constexpr int kPartBlock = 1024;
 
@@ -328,131 +241,6 @@ int partition_lds_bytes(int Pi)
 }
 
 // ------------------------------------------------------------- launchers --
-
-namespace {
-
-template <typename F>
-void ensure_lds(F func, int bytes)
-{
-    if (bytes > 65536)
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(func), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  bytes);
-}
-
-template <int KIND, int BLOCK>
-hipError_t eval_one(const ProblemDesc& p, const float* keys, long long ld, int count, float* fit, int* decoded,
-                    int lds, hipStream_t s)
-{
-    ensure_lds(k_evaluate<KIND, BLOCK>, lds);
-    hipLaunchKernelGGL((k_evaluate<KIND, BLOCK>), dim3(count), dim3(BLOCK), lds, s, p, keys, ld, count, fit,
-                       decoded);
-    return hipGetLastError();
-}
-
-template <int KIND>
-hipError_t eval_kind(int block, const ProblemDesc& p, const float* keys, long long ld, int count, float* fit,
-                     int* decoded, int lds, hipStream_t s)
-{
-    switch (block) {
-    case 64: return eval_one<KIND, 64>(p, keys, ld, count, fit, decoded, lds, s);
-    case 128: return eval_one<KIND, 128>(p, keys, ld, count, fit, decoded, lds, s);
-    case 256: return eval_one<KIND, 256>(p, keys, ld, count, fit, decoded, lds, s);
-    case 512: return eval_one<KIND, 512>(p, keys, ld, count, fit, decoded, lds, s);
-    case 1024: return eval_one<KIND, 1024>(p, keys, ld, count, fit, decoded, lds, s);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-template <typename K, typename... Args>
-void launch_maybe_timed(K kernel, dim3 grid, dim3 block, int lds, hipStream_t s, hipEvent_t start, hipEvent_t stop,
-                        Args... args)
-{
-    if (start || stop)
-        hipExtLaunchKernelGGL(kernel, grid, block, (uint32_t)lds, s, start, stop, 0u, args...);
-    else
-        hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
-}
-
-template <int KIND, int BLOCK>
-hipError_t step_single(const InstDesc& d, int lds, int cur, int init, hipStream_t s, hipEvent_t start,
-                       hipEvent_t stop)
-{
-    if (kind_sorts(KIND) && d.prob.n <= kCompactItems * BLOCK) {
-        ensure_lds(k_step_one<KIND, BLOCK, kCompactItems>, lds);
-        launch_maybe_timed(k_step_one<KIND, BLOCK, kCompactItems>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop, d,
-                           cur, init);
-    } else if (kind_sorts(KIND) && d.prob.n <= 2 * kCompactItems * BLOCK) {
-        ensure_lds(k_step_one<KIND, BLOCK, 2 * kCompactItems>, lds);
-        launch_maybe_timed(k_step_one<KIND, BLOCK, 2 * kCompactItems>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop,
-                           d, cur, init);
-    } else {
-        ensure_lds(k_step_one<KIND, BLOCK, kSortItems>, lds);
-        launch_maybe_timed(k_step_one<KIND, BLOCK, kSortItems>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop, d, cur,
-                           init);
-    }
-    return hipGetLastError();
-}
-
-template <int KIND>
-hipError_t step_kind(int block, const InstDesc& d, int lds, int cur, int init, hipStream_t s, hipEvent_t start,
-                     hipEvent_t stop)
-{
-    switch (block) {
-    case 64: return step_single<KIND, 64>(d, lds, cur, init, s, start, stop);
-    case 128: return step_single<KIND, 128>(d, lds, cur, init, s, start, stop);
-    case 256: return step_single<KIND, 256>(d, lds, cur, init, s, start, stop);
-    case 512: return step_single<KIND, 512>(d, lds, cur, init, s, start, stop);
-    case 1024: return step_single<KIND, 1024>(d, lds, cur, init, s, start, stop);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-}  // namespace
-
-hipError_t launch_evaluate(const ProblemDesc& p, const float* keys, long long ld, int count, float* fit,
-                           int* decoded, int block, hipStream_t s)
-{
-    if (count <= 0) return hipSuccess;
-    const int lds = decode_layout(p.kind, p.n, p.chrom_len, sort_buckets(p.n), p.aux).bytes;
-    switch (p.kind) {
-    case KIND_TSP_COORDS: return eval_kind<KIND_TSP_COORDS>(block, p, keys, ld, count, fit, decoded, lds, s);
-    case KIND_TSP_MATRIX: return eval_kind<KIND_TSP_MATRIX>(block, p, keys, ld, count, fit, decoded, lds, s);
-    case KIND_KNAP_THRESHOLD:
-        return eval_kind<KIND_KNAP_THRESHOLD>(block, p, keys, ld, count, fit, decoded, lds, s);
-    case KIND_KNAP_GREEDY: return eval_kind<KIND_KNAP_GREEDY>(block, p, keys, ld, count, fit, decoded, lds, s);
-    case KIND_TSPJ: return eval_kind<KIND_TSPJ>(block, p, keys, ld, count, fit, decoded, lds, s);
-    case KIND_BITSP_COORDS: return eval_kind<KIND_BITSP_COORDS>(block, p, keys, ld, count, fit, decoded, lds, s);
-    case KIND_BITSP_MATRIX: return eval_kind<KIND_BITSP_MATRIX>(block, p, keys, ld, count, fit, decoded, lds, s);
-    case KIND_CONTINUOUS: return eval_kind<KIND_CONTINUOUS>(block, p, keys, ld, count, fit, decoded, lds, s);
-    case KIND_JOBSHOP: return eval_kind<KIND_JOBSHOP>(block, p, keys, ld, count, fit, decoded, lds, s);
-    case KIND_ZDT1: return eval_kind<KIND_ZDT1>(block, p, keys, ld, count, fit, decoded, lds, s);
-    case KIND_VRP_DAYS: return eval_kind<KIND_VRP_DAYS>(block, p, keys, ld, count, fit, decoded, lds, s);
-    case KIND_MOVRP: return eval_kind<KIND_MOVRP>(block, p, keys, ld, count, fit, decoded, lds, s);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-int step_lds_bytes(const InstDesc& d)
-{
-    return step_layout(d.prob.kind, d.prob.n, d.L, d.nb, d.Pi, false, d.prob.aux).bytes;
-}
-
-hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s, hipEvent_t start,
-                           hipEvent_t stop)
-{
-    const int lds = step_lds_bytes(d);
-    switch (d.prob.kind) {
-    case KIND_TSP_COORDS: return step_kind<KIND_TSP_COORDS>(block, d, lds, cur, init, s, start, stop);
-    case KIND_TSP_MATRIX: return step_kind<KIND_TSP_MATRIX>(block, d, lds, cur, init, s, start, stop);
-    case KIND_KNAP_THRESHOLD: return step_kind<KIND_KNAP_THRESHOLD>(block, d, lds, cur, init, s, start, stop);
-    case KIND_KNAP_GREEDY: return step_kind<KIND_KNAP_GREEDY>(block, d, lds, cur, init, s, start, stop);
-    case KIND_TSPJ: return step_kind<KIND_TSPJ>(block, d, lds, cur, init, s, start, stop);
-    case KIND_CONTINUOUS: return step_kind<KIND_CONTINUOUS>(block, d, lds, cur, init, s, start, stop);
-    case KIND_JOBSHOP: return step_kind<KIND_JOBSHOP>(block, d, lds, cur, init, s, start, stop);
-    case KIND_VRP_DAYS: return step_kind<KIND_VRP_DAYS>(block, d, lds, cur, init, s, start, stop);
-    default: return hipErrorInvalidValue;
-    }
-}
 
 hipError_t launch_partition_one(const InstDesc& d, int buf, int mode, hipStream_t s, hipEvent_t start,
                                 hipEvent_t stop)

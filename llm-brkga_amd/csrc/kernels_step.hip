@@ -1,0 +1,34 @@
+// kernels_step.hip - entry of the one-population generation step: picks the
+// unit that holds the step kernel of the problem kind (k_step_one, defined
+// in step_kernel.hpp and instantiated per group of kinds in
+// kernels_step_{tsp,knapsack,plugins}.hip).  Replaces the fitness launch
+// and the brkga_generation_kernel launch of the GPU-resident generation
+// (core/solver.hpp:323, 346).
+#include "step_kernel.hpp"
+
+namespace brkga {
+
+int step_lds_bytes(const InstDesc& d)
+{
+    return step_layout(d.prob.kind, d.prob.n, d.L, d.nb, d.Pi, false, d.prob.aux).bytes;
+}
+
+hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s, hipEvent_t start,
+                           hipEvent_t stop)
+{
+    const int lds = step_lds_bytes(d);
+    const int kind = d.prob.kind;
+    switch (kind) {
+    case KIND_TSP_COORDS:
+    case KIND_TSP_MATRIX: return launch_step_tsp_kinds(kind, block, d, lds, cur, init, s, start, stop);
+    case KIND_KNAP_THRESHOLD:
+    case KIND_KNAP_GREEDY:
+    case KIND_TSPJ: return launch_step_knapsack_tspj(kind, block, d, lds, cur, init, s, start, stop);
+    case KIND_CONTINUOUS:
+    case KIND_JOBSHOP:
+    case KIND_VRP_DAYS: return launch_step_plugins(kind, block, d, lds, cur, init, s, start, stop);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace brkga

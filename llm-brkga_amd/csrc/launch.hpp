@@ -44,6 +44,16 @@ int tspj_ls_lds_bytes(int n, int nb);
 // launch; the kernel re-arms them.
 hipError_t launch_step_queue(int kind, int block, bool xys, int n_max, const InstDesc* d_descs, const int4* d_items,
                              int num_items, int* d_queue, int grid, int lds, hipStream_t s);
+// Keys per thread of the queue-step kernel for a bucket of problem sizes up
+// to n_max on `block` threads: the fewest of 1, 2, 4, 8, 16 that holds n_max
+// (at least 4 above 256 threads; 16 for kinds that do not sort).
+inline int queue_items_for(int kind, int block, int n_max)
+{
+    if (!kind_sorts(kind)) return kSortItems;
+    for (int items = block > 256 ? kCompactItems : 1; items < kSortItems; items <<= 1)
+        if (n_max <= items * block) return items;
+    return kSortItems;
+}
 // Resident workgroups of the queue step per CU (0: no such kernel).
 int step_queue_blocks_per_cu(int kind, int block, bool xys, int n_max, int lds);
 // Step counter := gen and arrival counter := 0 for `num` instances.

@@ -1,0 +1,125 @@
+// step_kernel.hpp - one BRKGA generation step of one population, one
+// workgroup per child slot: elite / migrant row copy, mutant keys from
+// Philox, or biased uniform crossover, then decode + fitness of the new row
+// straight from LDS.  Replaces brkga_generation_kernel
+// (core/cuda_kernels.cuh:88-138) fused with the config fitness kernels, and
+// the host-staged ring migration (core/solver.hpp:517-558).
+//
+// The population descriptor is a kernel argument.  `cur` is the buffer
+// parity of the parent generation, known to the launcher; the generation
+// counter in device memory only feeds the Philox counter and the migration
+// schedule, so a captured graph can be replayed.  Key rows are [pop][ld]
+// FP32 with ld a multiple of 32, so every row starts 128-byte aligned and is
+// streamed as float4.
+#pragma once
+
+#include "launch.hpp"
+#include "launch_util.hpp"
+#include "step_device.hpp"
+
+namespace brkga {
+
+// One block of a generation step: slot gs of the population described by d.
+// n <= ITEMS * BLOCK.
+template <int KIND, int BLOCK, int ITEMS>
+__device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, int init, char* lds)
+{
+    const int Pi = d.Pi, Ei = d.Ei, I = d.islands;
+    const int island = gs / Pi;
+    const int s = gs - island * Pi;
+    const int nxt = init ? 0 : cur ^ 1;
+    const int M = d.prob.num_obj;
+    const int ld = d.ld, L = d.L, n = d.prob.n;
+    const int nq = (L + 3) >> 2;
+    float4* out = reinterpret_cast<float4*>(d.keys[nxt] + (size_t)gs * ld);
+    float* fit_out = d.fit[nxt] + (size_t)gs * M;
+    const int* perm_cur = d.perm[cur] + (size_t)island * Pi;
+    const int k = d.mig_k;
+    const bool maybe_migrant = !init && I > 1 && k > 0 && d.mig_interval > 0 && s >= Pi - k;
+
+    // elite slot that no migration can claim: copy row + fitness, no decode
+    if (!init && !maybe_migrant && s < Ei) {
+        const int src = island * Pi + perm_cur[s];
+        const float4* in = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)src * ld);
+        for (int c = threadIdx.x; c < nq; c += BLOCK) out[c] = in[c];
+        if (threadIdx.x < M) fit_out[threadIdx.x] = d.fit[cur][(size_t)src * M + threadIdx.x];
+        return;
+    }
+
+    const int t = init ? 0 : *d.gen;
+    const int nb = d.nb;
+    const StepLayout sl = step_layout(KIND, n, L, nb, Pi, false, d.prob.aux);
+    float4* kl = reinterpret_cast<float4*>(lds + sl.dec.keys_off);
+    int* order = reinterpret_cast<int*>(lds + sl.isl_off);
+    const bool staged = Pi <= kMaxStagedOrder;
+    SortScratch ss;
+    ss.cnt = reinterpret_cast<int*>(lds + sl.dec.cnt_off);
+    ss.red = reinterpret_cast<long long*>(lds + sl.dec.red_off);
+
+    // stage the island rank order, clear the argsort counters
+    if (!init && staged)
+        for (int i = threadIdx.x; i < Pi; i += BLOCK) order[i] = perm_cur[i];
+    argsort_clear<BLOCK>(ss, nb);
+    __syncthreads();
+
+    const bool child = build_child<BLOCK>(d, island, s, cur, init, t, staged ? order : perm_cur, out, kl, fit_out);
+    if (!child) return;
+    __syncthreads();
+    decode_block<KIND, BLOCK, true, false, ITEMS>(d.prob, lds, sl.dec, nb, fit_out, nullptr);
+}
+
+template <int KIND, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_step_one(InstDesc d, int cur, int init)
+{
+    extern __shared__ __align__(16) char lds[];
+    step_body<KIND, BLOCK, ITEMS>(d, blockIdx.x, cur, init, lds);
+}
+
+template <int KIND, int BLOCK>
+inline hipError_t step_single(const InstDesc& d, int lds, int cur, int init, hipStream_t s, hipEvent_t start,
+                              hipEvent_t stop)
+{
+    if constexpr (kind_sorts(KIND)) {
+        if (d.prob.n <= kCompactItems * BLOCK) {
+            ensure_lds(k_step_one<KIND, BLOCK, kCompactItems>, lds);
+            launch_maybe_timed(k_step_one<KIND, BLOCK, kCompactItems>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop,
+                               d, cur, init);
+            return hipGetLastError();
+        }
+        if (d.prob.n <= 2 * kCompactItems * BLOCK) {
+            ensure_lds(k_step_one<KIND, BLOCK, 2 * kCompactItems>, lds);
+            launch_maybe_timed(k_step_one<KIND, BLOCK, 2 * kCompactItems>, dim3(d.pop), dim3(BLOCK), lds, s, start,
+                               stop, d, cur, init);
+            return hipGetLastError();
+        }
+    }
+    ensure_lds(k_step_one<KIND, BLOCK, kSortItems>, lds);
+    launch_maybe_timed(k_step_one<KIND, BLOCK, kSortItems>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop, d, cur,
+                       init);
+    return hipGetLastError();
+}
+
+template <int KIND>
+inline hipError_t step_kind(int block, const InstDesc& d, int lds, int cur, int init, hipStream_t s,
+                            hipEvent_t start, hipEvent_t stop)
+{
+    switch (block) {
+    case 64: return step_single<KIND, 64>(d, lds, cur, init, s, start, stop);
+    case 128: return step_single<KIND, 128>(d, lds, cur, init, s, start, stop);
+    case 256: return step_single<KIND, 256>(d, lds, cur, init, s, start, stop);
+    case 512: return step_single<KIND, 512>(d, lds, cur, init, s, start, stop);
+    case 1024: return step_single<KIND, 1024>(d, lds, cur, init, s, start, stop);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Step launchers per group of kinds, defined in the kernels_step_*.hip
+// units so that the instantiations compile in parallel.
+hipError_t launch_step_tsp_kinds(int kind, int block, const InstDesc& d, int lds, int cur, int init, hipStream_t s,
+                                 hipEvent_t start, hipEvent_t stop);
+hipError_t launch_step_knapsack_tspj(int kind, int block, const InstDesc& d, int lds, int cur, int init,
+                                     hipStream_t s, hipEvent_t start, hipEvent_t stop);
+hipError_t launch_step_plugins(int kind, int block, const InstDesc& d, int lds, int cur, int init, hipStream_t s,
+                               hipEvent_t start, hipEvent_t stop);
+
+}  // namespace brkga
