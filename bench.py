@@ -267,6 +267,11 @@ def served_leg(brkga_amd, workloads, eng, count, rates, budget):
             "hbm_budget_bytes": budget, "rates": rows, "results_identical": bool(all_same)}
 
 
+def step_items(n, block):
+    """Keys per thread of the step kernel instantiation that runs n keys."""
+    return 4 if n <= 4 * block else 8 if n <= 8 * block else 16
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -348,8 +353,8 @@ def main():
                    "launch": "hipGraph replay of the K-generation loop", "step_block": block},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                     "kernel": f"brkga::k_step_one<0, {block}, {4 if L <= 4 * block else 16}> "
-                               "(crossover / mutants / elite copy + decode + fitness)",
+                     "kernel": f"brkga::k_step_one<0, {block}, {step_items(L, block)}> "
+                               "(island rank + crossover / mutants / elite copy + decode + fitness)",
                      "kernel_us": round(step_us, 3), "partition_us": round(close_us, 3),
                      "bytes_per_generation": B,
                      "generation_frac": round(B * gens_per_s / (HBM_PEAK_GBS * 1e9), 4)},

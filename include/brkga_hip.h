@@ -284,7 +284,9 @@ brkga_status brkga_population_set_block(brkga_pop_t pop, int32_t block);
  * local search / NSGA-II selection + gather), in microseconds, over `reps`
  * real generations enqueued on `stream` as brkga_evolve does.  BRKGA without
  * local search: each kernel's own start / stop time (events attached to its
- * dispatch); otherwise HIP events recorded between the kernels.  One
+ * dispatch); otherwise HIP events recorded between the kernels.  When the
+ * island fits the step's workgroup, the step ranks its parents itself and
+ * *close_us is 0 (one partition, untimed, ends the run).  One
  * synchronisation at the end; the population advances by `reps`
  * generations.  Replaces the reference's PerformanceStats timers
  * (core/solver.hpp:143-157, 406-438), which time synchronous launches with

@@ -427,7 +427,8 @@ class Population:
     def profile(self, reps: int, stream: int = 0):
         """Mean (generation kernel, closing kernels) device time in
         microseconds over `reps` real generations (the population advances
-        `reps` generations)."""
+        `reps` generations).  The closing time is 0 when the step ranks its
+        parent island itself (islands no larger than the workgroup)."""
         a, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().brkga_population_profile(self.handle, reps, ctypes.byref(a), ctypes.byref(b),
                                               stream or None))

@@ -398,6 +398,15 @@ hipStream_t as_stream(void* s)
     return reinterpret_cast<hipStream_t>(s);
 }
 
+// Whether the population's steps rank their parent islands themselves: the
+// island fits the workgroup, and no local search needs each generation's
+// rank order.  A run of such steps then needs no partition per generation,
+// only one at its end (pop_close).
+bool pop_ranks(const brkga_pop_s* pop)
+{
+    return !pop->nsga2 && step_can_rank(pop->host, pop->block, pop->problem->engine->lds_limit);
+}
+
 // Enqueues generation 0 (init) or one step whose parent generation has
 // buffer parity `cur`.
 brkga_status pop_step(brkga_pop_t pop, int init, int cur, hipStream_t s)
@@ -409,8 +418,9 @@ brkga_status pop_step(brkga_pop_t pop, int init, int cur, hipStream_t s)
         if (e == hipSuccess) e = launch_nsga2_select(pop->d_desc, 1, init, d.pop, s);
         if (e == hipSuccess && !init) e = launch_nsga2_gather(pop->d_desc, 1, d.pop, s);
     } else {
-        e = launch_step_one(d, pop->block, cur, init, s);
-        if (e == hipSuccess) e = launch_partition_one(d, init ? 0 : cur ^ 1, init ? 0 : 1, s);
+        const int rank = !init && pop_ranks(pop);
+        e = launch_step_one(d, pop->block, cur, init, rank, s);
+        if (e == hipSuccess && !rank) e = launch_partition_one(d, init ? 0 : cur ^ 1, init ? 0 : 1, s);
         // TSPJ local search: the kernels check on the device whether the
         // generation just built is due, so graphs replay at any generation
         if (e == hipSuccess && !init && d.ls_interval > 0) {
@@ -420,6 +430,17 @@ brkga_status pop_step(brkga_pop_t pop, int init, int cur, hipStream_t s)
         }
     }
     if (e != hipSuccess) return hip_error(e, "kernel launch");
+    return BRKGA_OK;
+}
+
+// Ends a run of generations whose last one has buffer parity `buf`: steps
+// that ranked their parents leave the rank order of the newest generation
+// and the canonical step counter to one partition launch.
+brkga_status pop_close(brkga_pop_t pop, int buf, hipStream_t s)
+{
+    if (!pop_ranks(pop)) return BRKGA_OK;
+    hipError_t e = launch_partition_one(pop->host, buf, 3, s);
+    if (e != hipSuccess) return hip_error(e, "partition launch");
     return BRKGA_OK;
 }
 
@@ -819,7 +840,7 @@ brkga_status brkga_evolve(brkga_pop_t pop, int32_t generations, void* stream)
         if (st != BRKGA_OK) return st;
         ++pop->host_t;
     }
-    return BRKGA_OK;
+    return generations > 0 ? pop_close(pop, pop->host_t & 1, s) : BRKGA_OK;
 }
 
 brkga_status brkga_evolve_graph(brkga_pop_t pop, int32_t generations, void* stream)
@@ -839,6 +860,7 @@ brkga_status brkga_evolve_graph(brkga_pop_t pop, int32_t generations, void* stre
         brkga_status st = BRKGA_OK;
         for (int i = 0; i < generations && st == BRKGA_OK; ++i)
             st = pop_step(pop, 0, (pop->host_t + i) & 1, pop->capture_stream);
+        if (st == BRKGA_OK) st = pop_close(pop, (pop->host_t + generations) & 1, pop->capture_stream);
         hipError_t e = hipStreamEndCapture(pop->capture_stream, &graph);
         if (st != BRKGA_OK) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -984,10 +1006,22 @@ brkga_status brkga_population_autotune(brkga_pop_t pop, int32_t reps, int32_t* b
     float best_ms = 0.0f;
     bool have = false;
     // the step (BRKGA) / offspring (NSGA-II) kernel rewrites the same rows
-    // from unchanged inputs on every launch: timing it changes nothing
+    // from unchanged inputs on every launch: timing it changes nothing.  A
+    // BRKGA generation is timed as it would run at that workgroup size: the
+    // ranking step alone, or the step and a partition of the rows it wrote
+    // (mode 0: the counter stays).  A ranking step writes the next counter
+    // slot; the closing partition below sets both slots back.
+    const size_t lds_limit = pop->problem->engine->lds_limit;
+    bool ranked = false;
     auto launch = [&](int blk) {
-        return pop->nsga2 ? launch_nsga2_offspring(d.prob.kind, blk, pop->d_desc, 1, d.pop, pop->step_lds, 0, s)
-                          : launch_step_one(d, blk, cur, 0, s);
+        if (pop->nsga2)
+            return launch_nsga2_offspring(d.prob.kind, blk, pop->d_desc, 1, d.pop, pop->step_lds, 0, s);
+        if (step_can_rank(d, blk, lds_limit)) {
+            ranked = true;
+            return launch_step_one(d, blk, cur, 0, 1, s);
+        }
+        hipError_t r = launch_step_one(d, blk, cur, 0, 0, s);
+        return r == hipSuccess ? launch_partition_one(d, cur ^ 1, 0, s) : r;
     };
     for (int blk = 64; blk <= 1024 && e == hipSuccess; blk <<= 1) {
         if (blk * kSortItems < n) continue;
@@ -1004,6 +1038,7 @@ brkga_status brkga_population_autotune(brkga_pop_t pop, int32_t reps, int32_t* b
             best = blk;
         }
     }
+    if (e == hipSuccess && ranked) e = launch_partition_one(d, cur, 3, s);
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (e != hipSuccess) return hip_error(e, "autotune launches");
@@ -1041,10 +1076,13 @@ brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* ste
     const InstDesc& d = pop->host;
     // reps real generations, enqueued as brkga_evolve does.  BRKGA: the step
     // and partition kernels take start / stop events from their own
-    // dispatch (no marker packets between kernels).  NSGA-II: marker events
-    // around the offspring kernel and around selection + gather.  One
-    // synchronisation at the end; the population advances reps generations.
+    // dispatch (no marker packets between kernels); steps that rank their
+    // parents have no partition per generation (close_us = 0), one closes
+    // the run.  NSGA-II: marker events around the offspring kernel and
+    // around selection + gather.  One synchronisation at the end; the
+    // population advances reps generations.
     const bool timed = !pop->nsga2 && d.ls_interval <= 0;
+    const bool ranks = pop_ranks(pop);
     std::vector<hipEvent_t> ev(4 * (size_t)reps, nullptr);
     hipError_t e = hipSuccess;
     for (auto& x : ev)
@@ -1053,15 +1091,15 @@ brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* ste
         hipEvent_t* q = &ev[4 * (size_t)i];
         const int cur = pop->host_t & 1;
         if (timed) {
-            e = launch_step_one(d, pop->block, cur, 0, s, q[0], q[1]);
-            if (e == hipSuccess) e = launch_partition_one(d, cur ^ 1, 1, s, q[2], q[3]);
+            e = launch_step_one(d, pop->block, cur, 0, ranks, s, q[0], q[1]);
+            if (e == hipSuccess && !ranks) e = launch_partition_one(d, cur ^ 1, 1, s, q[2], q[3]);
         } else {
             e = hipEventRecord(q[0], s);
             if (e == hipSuccess) {
                 if (pop->nsga2)
                     e = launch_nsga2_offspring(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, 0, s);
                 else
-                    e = launch_step_one(d, pop->block, cur, 0, s);
+                    e = launch_step_one(d, pop->block, cur, 0, 0, s);
             }
             if (e == hipSuccess) e = hipEventRecord(q[1], s);
             if (e == hipSuccess) e = hipEventRecord(q[2], s);
@@ -1081,13 +1119,21 @@ brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* ste
         }
         ++pop->host_t;
     }
+    if (e == hipSuccess && ranks) {
+        const brkga_status st = pop_close(pop, pop->host_t & 1, s);
+        if (st != BRKGA_OK) {
+            for (auto& x : ev)
+                if (x) (void)hipEventDestroy(x);
+            return st;
+        }
+    }
     double sa = 0.0, sb = 0.0;
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     for (int i = 0; i < reps && e == hipSuccess; ++i) {
         const hipEvent_t* q = &ev[4 * (size_t)i];
         float a = 0.f, b = 0.f;
         e = hipEventElapsedTime(&a, q[0], q[1]);
-        if (e == hipSuccess) e = hipEventElapsedTime(&b, q[2], q[3]);
+        if (e == hipSuccess && !(timed && ranks)) e = hipEventElapsedTime(&b, q[2], q[3]);
         sa += a;
         sb += b;
     }

@@ -84,7 +84,7 @@ struct InstDesc {
     int* perm[2];     // [islands][Pi]: local slot of each rank
     int* rank[2];     // NSGA-II front index per slot
     float* crowd[2];  // NSGA-II crowding distance per slot
-    int* gen;         // completed generation steps
+    int* gen;         // completed generation steps; gen[3] is the odd-parity slot of the ranking steps (gen_slot)
     int* done;        // arrival counter of the per-island partition blocks
     int* ls_flag;     // 1 while a local search of the current generation is due
     int L, ld;
@@ -161,6 +161,24 @@ constexpr int kCompactItems = 4;
 
 // Rank orders longer than this are read from global memory, not staged.
 constexpr int kMaxStagedOrder = 2048;
+
+// Step counter a ranking step reads when its parent buffer has parity
+// `parity`: gen[0] (even) or gen[3] (odd).  The step writes t + 1 to the
+// other slot, so no workgroup of a launch reads what that launch writes.
+__host__ __device__ inline int* gen_slot(const InstDesc& d, int parity)
+{
+    return parity ? d.gen + 3 : d.gen;
+}
+
+// LDS a step that ranks its parent island itself adds to the step layout:
+// two exchange buffers of pow2(Pi) 64-bit keys for the network stages that
+// cross waves (none when the island fits one wave).
+__host__ __device__ inline int island_rank_bytes(int Pi)
+{
+    int pn = 2;
+    while (pn < Pi) pn <<= 1;
+    return pn > 64 ? 16 * pn : 0;
+}
 
 __host__ __device__ inline StepLayout step_layout(int kind, int n, int L, int nb, int Pi, bool stage_xy, int aux)
 {

@@ -30,9 +30,10 @@ __host__ __device__ inline int partition_parts(int Pi)
 // `parts` ranks the keys of its slice, G = kPartBlock / slice threads per
 // key counting over interleaved shares and combining with wave shuffles.
 // Larger islands use a bitonic network (parts = 1).  mode 1: the last
-// workgroup of the population bumps the generation counter to t + 1 and
-// raises the local-search flag when generation t + 1 is due; mode 2: the
-// last workgroup lowers the flag.
+// workgroup of the population bumps the generation counter (both parity
+// slots) to t + 1 and raises the local-search flag when generation t + 1 is
+// due; mode 2: the last workgroup lowers the flag; mode 3: the last
+// workgroup sets both counter slots to t.
 __device__ __forceinline__ void partition_body(const InstDesc& d, int island, int part, int parts, int buf,
                                                int mode, int t, char* lds)
 {
@@ -102,7 +103,11 @@ __device__ __forceinline__ void partition_body(const InstDesc& d, int island, in
                 if (mode == 1) {
                     const int g = t + 1;
                     *d.ls_flag = d.ls_interval > 0 && g > 0 && g % d.ls_interval == 0 ? 1 : 0;
-                    *d.gen = g;
+                    d.gen[0] = g;
+                    d.gen[3] = g;
+                } else if (mode == 3) {
+                    d.gen[0] = t;
+                    d.gen[3] = t;
                 } else {
                     *d.ls_flag = 0;
                 }
@@ -122,6 +127,8 @@ __global__ __launch_bounds__(kPartBlock) void k_partition_one(InstDesc d, int bu
     } else if (mode == 2) {
         if (*d.ls_flag == 0) return;
         buf = *d.gen & 1;
+    } else if (mode == 3) {
+        t = *gen_slot(d, buf);
     }
     partition_body(d, blockIdx.x / parts, blockIdx.x % parts, parts, buf, mode, t, lds);
 }

@@ -5,12 +5,12 @@
 
 namespace brkga {
 
-hipError_t launch_step_tsp_kinds(int kind, int block, const InstDesc& d, int lds, int cur, int init, hipStream_t s,
-                                 hipEvent_t start, hipEvent_t stop)
+hipError_t launch_step_tsp_kinds(int kind, int block, const InstDesc& d, int lds, int cur, int init, int rank,
+                                 hipStream_t s, hipEvent_t start, hipEvent_t stop)
 {
     switch (kind) {
-    case KIND_TSP_COORDS: return step_kind<KIND_TSP_COORDS>(block, d, lds, cur, init, s, start, stop);
-    case KIND_TSP_MATRIX: return step_kind<KIND_TSP_MATRIX>(block, d, lds, cur, init, s, start, stop);
+    case KIND_TSP_COORDS: return step_kind<KIND_TSP_COORDS>(block, d, lds, cur, init, rank, s, start, stop);
+    case KIND_TSP_MATRIX: return step_kind<KIND_TSP_MATRIX>(block, d, lds, cur, init, rank, s, start, stop);
     default: return hipErrorInvalidValue;
     }
 }

@@ -14,17 +14,25 @@ hipError_t launch_evaluate(const ProblemDesc& p, const float* keys, long long ld
 
 // One BRKGA generation step of one population, descriptor passed by value.
 // cur = buffer parity of the parent generation; init=1 builds generation 0
-// (Philox keys + evaluation) in buffer 0.
+// (Philox keys + evaluation) in buffer 0.  rank=1 (step_can_rank): each
+// workgroup ranks its parent island itself and the step advances the
+// parity-slot step counter (gen_slot), so no partition follows; a run of
+// such steps ends with a partition in mode 3.
 // start / stop (optional): events that take the kernel's own start and end
 // times (hipExtLaunchKernel), for profiling without marker packets.
-hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, hipStream_t s,
+hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, int rank, hipStream_t s,
                            hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+// Whether launch_step_one can rank in the step at this workgroup size with
+// `lds_limit` bytes of LDS per workgroup.
+bool step_can_rank(const InstDesc& d, int block, size_t lds_limit);
 
 // Stable per-island sort of the fitness in buffer `buf` into the rank order.
 // mode 0: sort only; 1: closes a generation step (bumps the step counter and
 // raises the local-search flag when the new generation is due); 2: sorts the
 // current generation again after a local search, only when the flag is
-// raised, and lowers it (buf is then taken from the step counter).
+// raised, and lowers it (buf is then taken from the step counter); 3: closes
+// a run of ranking steps whose last generation is in buffer `buf` (both
+// step counter slots := that generation's counter).
 hipError_t launch_partition_one(const InstDesc& d, int buf, int mode, hipStream_t s, hipEvent_t start = nullptr,
                                 hipEvent_t stop = nullptr);
 // buf < 0: buffer from each population's step counter.

@@ -65,7 +65,8 @@ __global__ __launch_bounds__(BLOCK) void k_step_queue(const InstDesc* __restrict
             __syncthreads();  // staging done; the previous child's decode is done with LDS
             argsort_clear<BLOCK>(ss, nb);
             const bool child =
-                build_child<BLOCK>(d, island, s, cur, init, t, staged ? order : perm_cur, out, kl, fit_out);
+                build_child<BLOCK>(d, island, s, cur, init, t, staged ? order : perm_cur, nullptr, out, kl,
+                                   fit_out);
             if (!child) continue;
             __syncthreads();
             decode_block<KIND, BLOCK, true, kXY, ITEMS>(d.prob, lds, sl.dec, nb, fit_out, nullptr, kXY ? xy : nullptr,

@@ -38,6 +38,103 @@ __device__ __forceinline__ uint32_t fitness_order_key(float f, int maximize)
     return maximize ? ~u : u;
 }
 
+// The 64-bit value of lane (lane ^ J) of the wave, J < 64: DPP quad
+// permutes for 1 and 2, a row rotation by 8 for 8, ds_swizzle's xor mode
+// (within 32 lanes) for 4 and 16, ds_bpermute from byte address `bp32`
+// (4 (lane ^ 32)) for 32.  Every lane must be active.
+template <int J>
+__device__ __forceinline__ unsigned xor_lane_u32(unsigned v, int bp32)
+{
+    if constexpr (J == 1) return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xf, 0xf, true);  // [1,0,3,2]
+    else if constexpr (J == 2) return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xf, 0xf, true);  // [2,3,0,1]
+    else if constexpr (J == 8) return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xf, 0xf, true);  // row_ror:8
+    else if constexpr (J == 4 || J == 16) return (unsigned)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (J << 10));
+    else return (unsigned)__builtin_amdgcn_ds_bpermute(bp32, (int)v);
+}
+
+// Compare-exchange steps J, J/2, .., 1 of merge level K of an ascending
+// bitonic network over PN keys, one per thread (threads >= PN idle).
+// bit[i] = bit i of the thread index; the lower thread of a pair keeps the
+// minimum in ascending blocks (bit K clear) and the maximum otherwise.
+// Steps with J >= kWave exchange through LDS, alternating between two
+// buffers of PN keys so that one barrier per step suffices.
+template <int PN, int K, int J>
+__device__ __forceinline__ void bitonic_steps(unsigned long long& v, unsigned long long* xch, const bool* bit,
+                                              int bp32, int& half)
+{
+    constexpr int lj = __builtin_ctz(J), lk = __builtin_ctz(K);
+    unsigned long long o;
+    if constexpr (J >= kWave) {
+        const int t = threadIdx.x;
+        unsigned long long* b = xch + half * PN;
+        half ^= 1;
+        if (t < PN) b[t] = v;
+        __syncthreads();
+        o = t < PN ? b[t ^ J] : v;
+    } else {
+        const unsigned lo = xor_lane_u32<J>((unsigned)v, bp32);
+        const unsigned hi = xor_lane_u32<J>((unsigned)(v >> 32), bp32);
+        o = ((unsigned long long)hi << 32) | lo;
+    }
+    const bool take_min = K < PN ? bit[lj] == bit[lk] : !bit[lj];
+    if ((o < v) == take_min) v = o;
+    if constexpr (J > 1) bitonic_steps<PN, K, J / 2>(v, xch, bit, bp32, half);
+}
+
+template <int PN, int K>
+__device__ __forceinline__ void bitonic_levels(unsigned long long& v, unsigned long long* xch, const bool* bit,
+                                               int bp32, int& half)
+{
+    bitonic_steps<PN, K, K / 2>(v, xch, bit, bp32, half);
+    if constexpr (K < PN) bitonic_levels<PN, 2 * K>(v, xch, bit, bp32, half);
+}
+
+// Ascending bitonic sort of one 64-bit key per thread over threads [0, PN).
+template <int PN>
+__device__ __forceinline__ unsigned long long block_bitonic_sort(unsigned long long v, unsigned long long* xch)
+{
+    const int t = threadIdx.x;
+    bool bit[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) bit[i] = (t >> i) & 1;
+    const int lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    const int bp32 = (lane ^ 32) << 2;
+    int half = 0;
+    bitonic_levels<PN, 2>(v, xch, bit, bp32, half);
+    return v;
+}
+
+// Rank order of island `island`'s fitness in buffer `buf` into order[0, Pi)
+// (LDS), computed by the whole workgroup: the keys (fitness order key,
+// slot) of the partition kernels, which are unique, so the order is the
+// same stable order.  (Converged populations hold many equal fitness
+// values, so the slot has to travel with the key.)  pow2(Pi) <= BLOCK; the
+// network spans max(pow2(Pi), 64) threads; xch holds island_rank_bytes(Pi)
+// bytes.  Ends with a barrier.
+template <int BLOCK>
+__device__ __forceinline__ void island_rank_order(const InstDesc& d, int island, int buf, int* order, void* xch)
+{
+    const int Pi = d.Pi, M = d.prob.num_obj;
+    const int t = threadIdx.x;
+    const float* fit = d.fit[buf] + (size_t)island * Pi * M;
+    unsigned long long v = ~0ull;
+    if (t < Pi) v = ((unsigned long long)fitness_order_key(fit[(size_t)t * M], d.prob.maximize) << 32) | (unsigned)t;
+    unsigned long long* x = static_cast<unsigned long long*>(xch);
+    if (Pi <= 64) v = block_bitonic_sort<64>(v, x);
+    else if constexpr (BLOCK >= 128) {
+        if (Pi <= 128) v = block_bitonic_sort<128>(v, x);
+        else if constexpr (BLOCK >= 256) {
+            if (Pi <= 256) v = block_bitonic_sort<256>(v, x);
+            else if constexpr (BLOCK >= 512) {
+                if (Pi <= 512) v = block_bitonic_sort<512>(v, x);
+                else if constexpr (BLOCK >= 1024) v = block_bitonic_sort<1024>(v, x);
+            }
+        }
+    }
+    if (t < Pi) order[t] = (int)(unsigned)v;
+    __syncthreads();
+}
+
 __device__ __forceinline__ float4 unit4(const U4& r)
 {
     return make_float4(unit_from_u32(r.x), unit_from_u32(r.y), unit_from_u32(r.z), unit_from_u32(r.w));
@@ -47,10 +144,13 @@ __device__ __forceinline__ float4 unit4(const U4& r)
 // `cur` with step index t (generation 0 when init): the row goes to `out`
 // (global) and `kl` (LDS).  Elite and migrant slots are copied with their
 // fitness instead; then it returns false and nothing is left to decode.
-// `order` is the island's rank order (LDS-staged or global).
+// `order` is the island's rank order (LDS-staged or global); `mig_order`
+// the rank order of the preceding island on the ring, or null to read it
+// from the partition's output.
 template <int BLOCK>
 __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s, int cur, int init, int t,
-                                            const int* order, float4* out, float4* kl, float* fit_out)
+                                            const int* order, const int* mig_order, float4* out, float4* kl,
+                                            float* fit_out)
 {
     const int Pi = d.Pi, Ei = d.Ei, Mi = d.Mi, I = d.islands;
     const int M = d.prob.num_obj, ld = d.ld;
@@ -69,7 +169,7 @@ __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s
     int src = -1;
     if (migrate) {
         const int si = (island + I - 1) % I;
-        src = si * Pi + d.perm[cur][si * Pi + (s - (Pi - k))];
+        src = si * Pi + (mig_order ? mig_order : d.perm[cur] + si * Pi)[s - (Pi - k)];
     } else if (s < Ei) {
         src = island * Pi + order[s];
     }

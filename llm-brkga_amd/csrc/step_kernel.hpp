@@ -20,9 +20,14 @@
 namespace brkga {
 
 // One block of a generation step: slot gs of the population described by d.
-// n <= ITEMS * BLOCK.
+// n <= ITEMS * BLOCK.  rank = 0: the parents' rank order comes from the
+// partition kernel (perm[cur]) and the partition advances the step counter.
+// rank = 1 (pow2(Pi) <= BLOCK, the LDS of island_rank_bytes(Pi) after the
+// step layout): the block ranks the parent island itself (the preceding
+// island for a migrant slot) and reads the step counter from its parity
+// slot; block 0 writes t + 1 to the other slot.
 template <int KIND, int BLOCK, int ITEMS>
-__device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, int init, char* lds)
+__device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, int init, int rank, char* lds)
 {
     const int Pi = d.Pi, Ei = d.Ei, I = d.islands;
     const int island = gs / Pi;
@@ -38,7 +43,7 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
     const bool maybe_migrant = !init && I > 1 && k > 0 && d.mig_interval > 0 && s >= Pi - k;
 
     // elite slot that no migration can claim: copy row + fitness, no decode
-    if (!init && !maybe_migrant && s < Ei) {
+    if (!init && !rank && !maybe_migrant && s < Ei) {
         const int src = island * Pi + perm_cur[s];
         const float4* in = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)src * ld);
         for (int c = threadIdx.x; c < nq; c += BLOCK) out[c] = in[c];
@@ -46,80 +51,98 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
         return;
     }
 
-    const int t = init ? 0 : *d.gen;
     const int nb = d.nb;
     const StepLayout sl = step_layout(KIND, n, L, nb, Pi, false, d.prob.aux);
     float4* kl = reinterpret_cast<float4*>(lds + sl.dec.keys_off);
     int* order = reinterpret_cast<int*>(lds + sl.isl_off);
-    const bool staged = Pi <= kMaxStagedOrder;
     SortScratch ss;
     ss.cnt = reinterpret_cast<int*>(lds + sl.dec.cnt_off);
     ss.red = reinterpret_cast<long long*>(lds + sl.dec.red_off);
-
-    // stage the island rank order, clear the argsort counters
-    if (!init && staged)
-        for (int i = threadIdx.x; i < Pi; i += BLOCK) order[i] = perm_cur[i];
-    argsort_clear<BLOCK>(ss, nb);
+    int t = 0;
+    const int* own_order = order;
+    const int* mig_order = nullptr;
+    if (init) {
+        argsort_clear<BLOCK>(ss, nb);
+    } else if (rank) {
+        t = *gen_slot(d, cur);
+        if (gs == 0 && threadIdx.x == 0) *gen_slot(d, cur ^ 1) = t + 1;
+        const bool migrate = maybe_migrant && t > 0 && (t % d.mig_interval) == 0;
+        if (migrate) {
+            island_rank_order<BLOCK>(d, (island + I - 1) % I, cur, order, lds + sl.bytes);
+            mig_order = order;
+        } else if (s < Ei || s < Pi - d.Mi) {
+            island_rank_order<BLOCK>(d, island, cur, order, lds + sl.bytes);
+        }
+        argsort_clear<BLOCK>(ss, nb);
+    } else {
+        t = *d.gen;
+        // stage the island rank order, clear the argsort counters
+        if (Pi <= kMaxStagedOrder)
+            for (int i = threadIdx.x; i < Pi; i += BLOCK) order[i] = perm_cur[i];
+        else
+            own_order = perm_cur;
+        argsort_clear<BLOCK>(ss, nb);
+    }
     __syncthreads();
 
-    const bool child = build_child<BLOCK>(d, island, s, cur, init, t, staged ? order : perm_cur, out, kl, fit_out);
+    const bool child = build_child<BLOCK>(d, island, s, cur, init, t, own_order, mig_order, out, kl, fit_out);
     if (!child) return;
     __syncthreads();
     decode_block<KIND, BLOCK, true, false, ITEMS>(d.prob, lds, sl.dec, nb, fit_out, nullptr);
 }
 
 template <int KIND, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_step_one(InstDesc d, int cur, int init)
+__global__ __launch_bounds__(BLOCK) void k_step_one(InstDesc d, int cur, int init, int rank)
 {
     extern __shared__ __align__(16) char lds[];
-    step_body<KIND, BLOCK, ITEMS>(d, blockIdx.x, cur, init, lds);
+    step_body<KIND, BLOCK, ITEMS>(d, blockIdx.x, cur, init, rank, lds);
 }
 
 template <int KIND, int BLOCK>
-inline hipError_t step_single(const InstDesc& d, int lds, int cur, int init, hipStream_t s, hipEvent_t start,
-                              hipEvent_t stop)
+inline hipError_t step_single(const InstDesc& d, int lds, int cur, int init, int rank, hipStream_t s,
+                              hipEvent_t start, hipEvent_t stop)
 {
     if constexpr (kind_sorts(KIND)) {
         if (d.prob.n <= kCompactItems * BLOCK) {
             ensure_lds(k_step_one<KIND, BLOCK, kCompactItems>, lds);
             launch_maybe_timed(k_step_one<KIND, BLOCK, kCompactItems>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop,
-                               d, cur, init);
+                               d, cur, init, rank);
             return hipGetLastError();
         }
         if (d.prob.n <= 2 * kCompactItems * BLOCK) {
             ensure_lds(k_step_one<KIND, BLOCK, 2 * kCompactItems>, lds);
             launch_maybe_timed(k_step_one<KIND, BLOCK, 2 * kCompactItems>, dim3(d.pop), dim3(BLOCK), lds, s, start,
-                               stop, d, cur, init);
+                               stop, d, cur, init, rank);
             return hipGetLastError();
         }
     }
     ensure_lds(k_step_one<KIND, BLOCK, kSortItems>, lds);
     launch_maybe_timed(k_step_one<KIND, BLOCK, kSortItems>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop, d, cur,
-                       init);
+                       init, rank);
     return hipGetLastError();
 }
 
 template <int KIND>
-inline hipError_t step_kind(int block, const InstDesc& d, int lds, int cur, int init, hipStream_t s,
+inline hipError_t step_kind(int block, const InstDesc& d, int lds, int cur, int init, int rank, hipStream_t s,
                             hipEvent_t start, hipEvent_t stop)
 {
     switch (block) {
-    case 64: return step_single<KIND, 64>(d, lds, cur, init, s, start, stop);
-    case 128: return step_single<KIND, 128>(d, lds, cur, init, s, start, stop);
-    case 256: return step_single<KIND, 256>(d, lds, cur, init, s, start, stop);
-    case 512: return step_single<KIND, 512>(d, lds, cur, init, s, start, stop);
-    case 1024: return step_single<KIND, 1024>(d, lds, cur, init, s, start, stop);
+    case 64: return step_single<KIND, 64>(d, lds, cur, init, rank, s, start, stop);
+    case 128: return step_single<KIND, 128>(d, lds, cur, init, rank, s, start, stop);
+    case 256: return step_single<KIND, 256>(d, lds, cur, init, rank, s, start, stop);
+    case 512: return step_single<KIND, 512>(d, lds, cur, init, rank, s, start, stop);
+    case 1024: return step_single<KIND, 1024>(d, lds, cur, init, rank, s, start, stop);
     default: return hipErrorInvalidValue;
     }
 }
 
 // Step launchers per group of kinds, defined in the kernels_step_*.hip
 // units so that the instantiations compile in parallel.
-hipError_t launch_step_tsp_kinds(int kind, int block, const InstDesc& d, int lds, int cur, int init, hipStream_t s,
-                                 hipEvent_t start, hipEvent_t stop);
-hipError_t launch_step_knapsack_tspj(int kind, int block, const InstDesc& d, int lds, int cur, int init,
+hipError_t launch_step_tsp_kinds(int kind, int block, const InstDesc& d, int lds, int cur, int init, int rank,
+                                 hipStream_t s, hipEvent_t start, hipEvent_t stop);
+hipError_t launch_step_knapsack_tspj(int kind, int block, const InstDesc& d, int lds, int cur, int init, int rank,
                                      hipStream_t s, hipEvent_t start, hipEvent_t stop);
-hipError_t launch_step_plugins(int kind, int block, const InstDesc& d, int lds, int cur, int init, hipStream_t s,
-                               hipEvent_t start, hipEvent_t stop);
+hipError_t launch_step_plugins(int kind, int block, const InstDesc& d, int lds, int cur, int init, int rank,
+                               hipStream_t s, hipEvent_t start, hipEvent_t stop);
 
 }  // namespace brkga
