@@ -38,55 +38,69 @@ __device__ __forceinline__ uint32_t fitness_order_key(float f, int maximize)
     return maximize ? ~u : u;
 }
 
-// The 64-bit value of lane (lane ^ J) of the wave, J < 64: DPP quad
-// permutes for 1 and 2, a row rotation by 8 for 8, ds_swizzle's xor mode
-// (within 32 lanes) for 4 and 16, ds_bpermute from byte address `bp32`
-// (4 (lane ^ 32)) for 32.  Every lane must be active.
-template <int J>
-__device__ __forceinline__ unsigned xor_lane_u32(unsigned v, int bp32)
+// The 32-bit value of lane (lane ^ X) of the wave, 0 < X < 64: DPP quad
+// permutes for 1, 2 and 3, row rotation by 8 for 8, the row half mirror for
+// 7, the row mirror for 15, ds_swizzle's xor mode (within 32 lanes) for 4,
+// 16 and 31, ds_bpermute from byte address `bp` = 4 (lane ^ X) otherwise.
+// Every lane must be active.
+template <int X>
+__device__ __forceinline__ unsigned xor_lane_u32(unsigned v, int bp)
 {
-    if constexpr (J == 1) return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xf, 0xf, true);  // [1,0,3,2]
-    else if constexpr (J == 2) return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xf, 0xf, true);  // [2,3,0,1]
-    else if constexpr (J == 8) return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xf, 0xf, true);  // row_ror:8
-    else if constexpr (J == 4 || J == 16) return (unsigned)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (J << 10));
-    else return (unsigned)__builtin_amdgcn_ds_bpermute(bp32, (int)v);
+    if constexpr (X == 1) return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xf, 0xf, true);  // [1,0,3,2]
+    else if constexpr (X == 2) return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xf, 0xf, true);  // [2,3,0,1]
+    else if constexpr (X == 3) return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x1B, 0xf, 0xf, true);  // [3,2,1,0]
+    else if constexpr (X == 7) return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xf, 0xf, true);  // half mirror
+    else if constexpr (X == 8) return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xf, 0xf, true);  // row_ror:8
+    else if constexpr (X == 15) return (unsigned)__builtin_amdgcn_mov_dpp((int)v, 0x140, 0xf, 0xf, true);  // mirror
+    else if constexpr (X == 4 || X == 16 || X == 31)
+        return (unsigned)__builtin_amdgcn_ds_swizzle((int)v, 0x1F | (X << 10));
+    else return (unsigned)__builtin_amdgcn_ds_bpermute(bp, (int)v);
 }
 
-// Compare-exchange steps J, J/2, .., 1 of merge level K of an ascending
-// bitonic network over PN keys, one per thread (threads >= PN idle).
-// bit[i] = bit i of the thread index; the lower thread of a pair keeps the
-// minimum in ascending blocks (bit K clear) and the maximum otherwise.
-// Steps with J >= kWave exchange through LDS, alternating between two
-// buffers of PN keys so that one barrier per step suffices.
-template <int PN, int K, int J>
-__device__ __forceinline__ void bitonic_steps(unsigned long long& v, unsigned long long* xch, const bool* bit,
-                                              int bp32, int& half)
+// Lane (lane ^ X) of a 64-bit value, or through LDS (two buffers of PN
+// keys, one barrier per exchange) when X reaches across waves.
+template <int PN, int X>
+__device__ __forceinline__ unsigned long long xor_exchange(unsigned long long v, unsigned long long* xch, int bp,
+                                                           int& half)
 {
-    constexpr int lj = __builtin_ctz(J), lk = __builtin_ctz(K);
-    unsigned long long o;
-    if constexpr (J >= kWave) {
+    if constexpr (X >= kWave) {
         const int t = threadIdx.x;
         unsigned long long* b = xch + half * PN;
         half ^= 1;
         if (t < PN) b[t] = v;
         __syncthreads();
-        o = t < PN ? b[t ^ J] : v;
+        return t < PN ? b[t ^ X] : v;
     } else {
-        const unsigned lo = xor_lane_u32<J>((unsigned)v, bp32);
-        const unsigned hi = xor_lane_u32<J>((unsigned)(v >> 32), bp32);
-        o = ((unsigned long long)hi << 32) | lo;
+        const unsigned lo = xor_lane_u32<X>((unsigned)v, bp);
+        const unsigned hi = xor_lane_u32<X>((unsigned)(v >> 32), bp);
+        return ((unsigned long long)hi << 32) | lo;
     }
-    const bool take_min = K < PN ? bit[lj] == bit[lk] : !bit[lj];
-    if ((o < v) == take_min) v = o;
-    if constexpr (J > 1) bitonic_steps<PN, K, J / 2>(v, xch, bit, bp32, half);
 }
 
-template <int PN, int K>
-__device__ __forceinline__ void bitonic_levels(unsigned long long& v, unsigned long long* xch, const bool* bit,
-                                               int bp32, int& half)
+// Half-cleaner steps J, J/2, .., 1 of an ascending bitonic network in flip
+// form (every compare-exchange ascending): the thread with bit J clear
+// keeps the minimum.  bit[i] = bit i of the thread index; bp32 / bp63 =
+// ds_bpermute addresses of lanes lane ^ 32 and lane ^ 63.
+template <int PN, int J>
+__device__ __forceinline__ void bitonic_half_cleaners(unsigned long long& v, unsigned long long* xch, const bool* bit,
+                                                      int bp32, int& half)
 {
-    bitonic_steps<PN, K, K / 2>(v, xch, bit, bp32, half);
-    if constexpr (K < PN) bitonic_levels<PN, 2 * K>(v, xch, bit, bp32, half);
+    const unsigned long long o = xor_exchange<PN, J>(v, xch, bp32, half);
+    if ((o < v) != bit[__builtin_ctz(J)]) v = o;
+    if constexpr (J > 1) bitonic_half_cleaners<PN, J / 2>(v, xch, bit, bp32, half);
+}
+
+// Merge level K (blocks of K keys): the flip step pairs t with t ^ (K - 1),
+// the lower thread (bit K/2 clear) keeping the minimum, then the half
+// cleaners from K/4 down.
+template <int PN, int K>
+__device__ __forceinline__ void bitonic_flip_levels(unsigned long long& v, unsigned long long* xch, const bool* bit,
+                                                    int bp32, int bp63, int& half)
+{
+    const unsigned long long o = xor_exchange<PN, K - 1>(v, xch, K - 1 == 63 ? bp63 : bp32, half);
+    if ((o < v) != bit[__builtin_ctz(K) - 1]) v = o;
+    if constexpr (K >= 4) bitonic_half_cleaners<PN, K / 4>(v, xch, bit, bp32, half);
+    if constexpr (K < PN) bitonic_flip_levels<PN, 2 * K>(v, xch, bit, bp32, bp63, half);
 }
 
 // Ascending bitonic sort of one 64-bit key per thread over threads [0, PN).
@@ -98,9 +112,9 @@ __device__ __forceinline__ unsigned long long block_bitonic_sort(unsigned long l
 #pragma unroll
     for (int i = 0; i < 12; ++i) bit[i] = (t >> i) & 1;
     const int lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-    const int bp32 = (lane ^ 32) << 2;
+    const int bp32 = (lane ^ 32) << 2, bp63 = (lane ^ 63) << 2;
     int half = 0;
-    bitonic_levels<PN, 2>(v, xch, bit, bp32, half);
+    bitonic_flip_levels<PN, 2>(v, xch, bit, bp32, bp63, half);
     return v;
 }
 
