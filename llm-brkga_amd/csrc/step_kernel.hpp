@@ -64,15 +64,20 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
     if (init) {
         argsort_clear<BLOCK>(ss, nb);
     } else if (rank) {
+        // only a slot that migration may claim needs the counter to choose
+        // the island it ranks: every other block issues the island's
+        // fitness loads together with the counter's
         t = *gen_slot(d, cur);
-        if (gs == 0 && threadIdx.x == 0) *gen_slot(d, cur ^ 1) = t + 1;
-        const bool migrate = maybe_migrant && t > 0 && (t % d.mig_interval) == 0;
-        if (migrate) {
+        if (!maybe_migrant) {
+            if (s < Ei || s < Pi - d.Mi)
+                island_rank_order<BLOCK>(d, island, cur, order, lds + sl.bytes);
+        } else if (t > 0 && (t % d.mig_interval) == 0) {
             island_rank_order<BLOCK>(d, (island + I - 1) % I, cur, order, lds + sl.bytes);
             mig_order = order;
         } else if (s < Ei || s < Pi - d.Mi) {
             island_rank_order<BLOCK>(d, island, cur, order, lds + sl.bytes);
         }
+        if (gs == 0 && threadIdx.x == 0) *gen_slot(d, cur ^ 1) = t + 1;
         argsort_clear<BLOCK>(ss, nb);
     } else {
         t = *d.gen;
