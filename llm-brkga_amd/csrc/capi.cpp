@@ -341,6 +341,7 @@ brkga_status create_population(brkga_problem_s* problem, const brkga_params* par
             e = hipMemsetAsync(d.keys[b], 0, rows * d.ld * sizeof(float), s);
         if (e == hipSuccess) e = hipMemsetAsync(d.gen, 0, 4 * sizeof(int), s);
         if (e == hipSuccess && gen0 < 0) e = hipMemsetAsync(d.gen, 0xFF, sizeof(int), s);
+        if (e == hipSuccess && gen0 < 0) e = hipMemsetAsync(d.gen + 3, 0xFF, sizeof(int), s);
         if (e == hipSuccess) e = hipMemcpyAsync(pop->d_desc, &pop->host, sizeof(InstDesc), hipMemcpyHostToDevice, s);
         if (e != hipSuccess) st = hip_error(e, "population upload");
     }

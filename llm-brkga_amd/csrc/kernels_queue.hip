@@ -13,7 +13,8 @@ __global__ __launch_bounds__(256) void k_set_counters(const InstDesc* __restrict
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= num) return;
-    *descs[i].gen = gen;
+    descs[i].gen[0] = gen;
+    descs[i].gen[3] = gen;  // the odd-parity slot of steps that rank their island
     *descs[i].done = 0;
 }
 

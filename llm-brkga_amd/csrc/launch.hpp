@@ -64,7 +64,8 @@ inline int queue_items_for(int kind, int block, int n_max)
 }
 // Resident workgroups of the queue step per CU (0: no such kernel).
 int step_queue_blocks_per_cu(int kind, int block, bool xys, int n_max, int lds);
-// Step counter := gen and arrival counter := 0 for `num` instances.
+// Step counter (both parity slots) := gen and arrival counter := 0 for `num`
+// instances.
 hipError_t launch_set_counters(const InstDesc* d_descs, int num, int gen, hipStream_t s);
 
 int step_lds_bytes(const InstDesc& d);
