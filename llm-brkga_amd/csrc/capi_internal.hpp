@@ -53,6 +53,7 @@ struct brkga_engine_s {
         int block, chunk;
         bool xys;
         double child_us;  // measured step time per child slot (0: not measured)
+        int children;     // child slots the measurement ran over
     };
     std::map<std::pair<int, int>, Shape> tuned;
     std::mutex tuned_mu;

@@ -135,6 +135,7 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
                     b.chunk = it->second.chunk;
                     b.xys = it->second.xys;
                     b.child_us = it->second.child_us;
+                    b.tuned_children = it->second.children;
                 } else {
                     b.block = default_block(key);
                     b.chunk = default_chunk(key);
@@ -276,12 +277,26 @@ void Fleet::assign_lanes()
 {
     const int brkga_lanes = groups_.empty() ? kLanes : kLanes - 1;
     nsga_lane_ = groups_.empty() ? 0 : (buckets_.empty() ? 0 : kLanes - 1);
-    bool measured = !buckets_.empty();
-    for (const auto& b : buckets_) measured = measured && b.child_us > 0.0;
+    // A bucket's measured time per child is trusted when the measurement ran
+    // over at least half its current children: measured on a few members
+    // (a size class the server first saw with one request) a launch is
+    // latency-bound and its time per child is many times the rate of a full
+    // bucket.  Other buckets are costed by the work model, scaled to
+    // microseconds by the trusted buckets' measured rate.
+    auto trusted = [&](const FleetBucket& b) { return b.child_us > 0.0 && 2 * b.tuned_children >= b.children; };
+    double us = 0.0, work = 0.0;
+    for (const auto& b : buckets_)
+        if (trusted(b)) {
+            us += b.children * b.child_us;
+            work += b.work;
+        }
+    const double scale = work > 0.0 ? us / work : 1.0;
     for (auto& L : launches_) {
         L.cost = 0.0;
-        for (int bi : L.buckets)
-            L.cost += measured ? buckets_[bi].children * buckets_[bi].child_us : buckets_[bi].work;
+        for (int bi : L.buckets) {
+            const FleetBucket& b = buckets_[bi];
+            L.cost += trusted(b) ? b.children * b.child_us : b.work * scale;
+        }
     }
     std::vector<int> idx(launches_.size());
     std::iota(idx.begin(), idx.end(), 0);
@@ -493,7 +508,8 @@ brkga_status Fleet::tune(int reps, bool only_new, hipStream_t s)
                     if (!have || ms < best_ms) {
                         have = true;
                         best_ms = ms;
-                        best = Shape{blk, ch, xys, 1000.0 * ms / reps / std::max(1, buckets_[bi].children)};
+                        best = Shape{blk, ch, xys, 1000.0 * ms / reps / std::max(1, buckets_[bi].children),
+                                     buckets_[bi].children};
                     }
                 }
                 if (st != BRKGA_OK) break;
@@ -504,6 +520,7 @@ brkga_status Fleet::tune(int reps, bool only_new, hipStream_t s)
         buckets_[bi].xys = best.xys;
         buckets_[bi].lds = bucket_lds(descs_, buckets_[bi], best.xys);
         buckets_[bi].child_us = best.child_us;
+        buckets_[bi].tuned_children = best.children;
         std::lock_guard<std::mutex> lock(eng_->tuned_mu);
         eng_->tuned[{buckets_[bi].kind, buckets_[bi].key}] = best;
     }

@@ -34,6 +34,7 @@ struct FleetBucket {
     double work;       // lane-balancing cost (see assign_lanes)
     int children;      // child slots of the members (sum of populations)
     double child_us;   // measured step time per child slot, 0 if not tuned
+    int tuned_children;  // child slots that measurement ran over
     int lane;          // stream lane of the bucket's launch (0 = caller's stream)
 };
 
