@@ -233,8 +233,12 @@ brkga_status brkga_population_destroy(brkga_pop_t pop);
  * (Population::initialize, core/population.hpp:107-124). */
 brkga_status brkga_population_initialize(brkga_pop_t pop, void* stream);
 
-/* `generations` generation steps, one launch sequence per step
- * (Solver::evolve_generation, core/solver.hpp:213-231 / 378-476). */
+/* `generations` generation steps (Solver::evolve_generation,
+ * core/solver.hpp:213-231 / 378-476).  BRKGA islands no wider than the step
+ * kernel's workgroup, without local search: one step launch per generation
+ * (each workgroup ranks its parent island), then one partition that leaves
+ * the newest generation's rank order for readers.  Otherwise each step is
+ * followed by its partition (and any local search). */
 brkga_status brkga_evolve(brkga_pop_t pop, int32_t generations, void* stream);
 
 /* Same steps replayed from a captured HIP graph of `generations` steps
