@@ -96,11 +96,21 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
     decode_block<KIND, BLOCK, true, false, ITEMS>(d.prob, lds, sl.dec, nb, fit_out, nullptr);
 }
 
+// Child slot of workgroup b.  Workgroups go to the 8 XCDs round-robin
+// (XCD b % 8), so XCD x gets the contiguous slots [x P/8, (x+1) P/8): an
+// island's children, whose rows and fitness the island's next generation
+// reads, stay in the L2 of one or two XCDs.  Identity when 8 does not
+// divide P.  A bijection on [0, P) either way.
+__device__ __forceinline__ int xcd_slot(int b, int P)
+{
+    return (P & 7) ? b : (b & 7) * (P >> 3) + (b >> 3);
+}
+
 template <int KIND, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_step_one(InstDesc d, int cur, int init, int rank)
 {
     extern __shared__ __align__(16) char lds[];
-    step_body<KIND, BLOCK, ITEMS>(d, blockIdx.x, cur, init, rank, lds);
+    step_body<KIND, BLOCK, ITEMS>(d, xcd_slot(blockIdx.x, d.pop), cur, init, rank, lds);
 }
 
 template <int KIND, int BLOCK>
