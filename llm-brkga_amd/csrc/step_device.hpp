@@ -124,7 +124,7 @@ __device__ __forceinline__ unsigned long long block_bitonic_sort(unsigned long l
 // same stable order.  (Converged populations hold many equal fitness
 // values, so the slot has to travel with the key.)  pow2(Pi) <= BLOCK; the
 // network spans max(pow2(Pi), 64) threads; xch holds island_rank_bytes(Pi)
-// bytes.  Ends with a barrier.
+// bytes.  order is complete after the caller's next barrier.
 template <int BLOCK>
 __device__ __forceinline__ void island_rank_order(const InstDesc& d, int island, int buf, int* order, void* xch)
 {
@@ -146,7 +146,6 @@ __device__ __forceinline__ void island_rank_order(const InstDesc& d, int island,
         }
     }
     if (t < Pi) order[t] = (int)(unsigned)v;
-    __syncthreads();
 }
 
 __device__ __forceinline__ float4 unit4(const U4& r)

@@ -66,8 +66,10 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
     } else if (rank) {
         // only a slot that migration may claim needs the counter to choose
         // the island it ranks: every other block issues the island's
-        // fitness loads together with the counter's
+        // fitness loads together with the counter's.  The argsort counters
+        // (their own LDS) are cleared first; one barrier below ends both.
         t = *gen_slot(d, cur);
+        argsort_clear<BLOCK>(ss, nb);
         if (!maybe_migrant) {
             if (s < Ei || s < Pi - d.Mi)
                 island_rank_order<BLOCK>(d, island, cur, order, lds + sl.bytes);
@@ -78,7 +80,6 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
             island_rank_order<BLOCK>(d, island, cur, order, lds + sl.bytes);
         }
         if (gs == 0 && threadIdx.x == 0) *gen_slot(d, cur ^ 1) = t + 1;
-        argsort_clear<BLOCK>(ss, nb);
     } else {
         t = *d.gen;
         // stage the island rank order, clear the argsort counters
