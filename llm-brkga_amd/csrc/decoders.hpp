@@ -56,9 +56,13 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
             block_rank<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, pos, nullptr, load);
         else
             block_rank<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, pos);
-        float2* txy = reinterpret_cast<float2*>(lds + lay.items_off);
+        // Tour positions go to the argsort counters, dead once the keys are
+        // placed, when they hold n coordinate pairs (nb >= 2n); then no
+        // barrier is needed against the ranking's reads of the items.
+        const bool in_cnt = nb + 4 >= 2 * n;
+        float2* txy = reinterpret_cast<float2*>(in_cnt ? reinterpret_cast<char*>(ss.cnt) : lds + lay.items_off);
         if constexpr (!kEarly) load();
-        __syncthreads();  // every ranking read of the items is done
+        if (!in_cnt) __syncthreads();  // every ranking read of the items is done
 #pragma unroll
         for (int i = 0; i < ITEMS; ++i) {
             const int e = tid + i * BLOCK;
