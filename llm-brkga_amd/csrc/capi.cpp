@@ -242,6 +242,11 @@ brkga_status alloc_inst_buffers(InstDesc* d, int nsga2, DeviceAllocator* a, std:
             d->crowd[b] = (float*)p;
         }
     }
+    if (nsga2 && rows <= (size_t)kDomRows) {
+        void* p;
+        if ((st = get(&p, rows * ((rows + 31) / 32) * sizeof(unsigned), "dominance bitsets")) != BRKGA_OK) return st;
+        d->dom = (unsigned*)p;
+    }
     void* c;
     if ((st = get(&c, 4 * sizeof(int), "counters")) != BRKGA_OK) return st;
     d->gen = (int*)c;

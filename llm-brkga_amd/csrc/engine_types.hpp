@@ -84,6 +84,7 @@ struct InstDesc {
     int* perm[2];     // [islands][Pi]: local slot of each rank
     int* rank[2];     // NSGA-II front index per slot
     float* crowd[2];  // NSGA-II crowding distance per slot
+    unsigned* dom;    // NSGA-II dominated-by bitsets [rows][rows / 32] (null: counted in the selection)
     int* gen;         // completed generation steps; gen[3] is the odd-parity slot of the ranking steps (gen_slot)
     int* done;        // arrival counter of the per-island partition blocks
     int* ls_flag;     // 1 while a local search of the current generation is due
@@ -99,6 +100,10 @@ struct InstDesc {
     int seg_begin;    // first block of this instance in a partition launch
     int gen_limit;    // steps are skipped once *gen >= gen_limit
 };
+
+// NSGA-II populations of up to this many rows (parents + children) get
+// dominated-by bitsets: one row per thread of the selection workgroup.
+constexpr int kDomRows = 1024;
 
 __host__ __device__ inline int round_up_int(int v, int m)
 {

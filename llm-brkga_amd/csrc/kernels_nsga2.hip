@@ -165,6 +165,43 @@ __host__ __device__ inline SelectLayout select_layout(int C)
 }
 
 constexpr int kSelectBlock = 1024;
+constexpr int kDomBlock = 256;
+static_assert(kDomRows <= kSelectBlock, "one dominance row per selection thread");
+
+// Dominated-by bitsets of the C rows the selection of every listed
+// population ranks (C = 2N, or N at generation 0): bit j of word w of row q
+// is set when row 32 w + j dominates row q.  One thread per (row, word);
+// grid (row-word chunks of kDomBlock, populations).  Populations without
+// bitsets (more than kDomRows rows) are skipped.
+__global__ __launch_bounds__(kDomBlock) void k_nsga2_dominance(const InstDesc* __restrict__ descs, int init)
+{
+    __shared__ float2 o[kDomRows];
+    const InstDesc& d = descs[blockIdx.y];
+    if (!d.dom) return;
+    const int N = d.pop;
+    const int t = init ? -1 : *d.gen;
+    if (t >= d.gen_limit) return;
+    const int first = t < 0;
+    const int cur = first ? 0 : (t & 1);
+    const int C = first ? N : 2 * N;
+    const int W = (C + 31) >> 5;
+    if ((int)blockIdx.x * kDomBlock >= C * W) return;
+    const float* fit = d.fit[cur];
+    for (int i = threadIdx.x; i < C; i += kDomBlock) o[i] = make_float2(fit[2 * i], fit[2 * i + 1]);
+    __syncthreads();
+    const int idx = blockIdx.x * kDomBlock + threadIdx.x;
+    if (idx >= C * W) return;
+    const int q = idx / W, w = idx - q * W;
+    const float2 b = o[q];
+    const int j0 = w << 5;
+    const int nj = C - j0 < 32 ? C - j0 : 32;
+    unsigned bits = 0;
+    for (int j = 0; j < nj; ++j) {
+        const float2 a = o[j0 + j];
+        bits |= (dominates2(a.x, a.y, b.x, b.y) ? 1u : 0u) << j;
+    }
+    d.dom[(size_t)q * W + w] = bits;
+}
 
 __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* __restrict__ descs, int init)
 {
@@ -200,6 +237,47 @@ __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* _
         rk[i] = -1;
     }
     __syncthreads();
+    int r = 0;
+    if (d.dom && C <= kDomRows) {
+        // Domination counts and front peeling (population.hpp:334-440) from
+        // the dominated-by bitsets: row q's count is the popcount of its
+        // bitset, and a front takes away popcount(bitset & front members).
+        // Thread q holds row q; wave k's ballot gives the front bits of rows
+        // [64k, 64k + 64).
+        __shared__ unsigned front[kSelectBlock / 32];
+        const int W = (C + 31) >> 5;
+        const int q = threadIdx.x;
+        unsigned dq[kDomRows / 32];
+        int c = 0;
+#pragma unroll
+        for (int w = 0; w < kDomRows / 32; ++w) {
+            dq[w] = (q < C && w < W) ? d.dom[(size_t)q * W + w] : 0u;
+            c += __popc(dq[w]);
+        }
+        if (q < C && c == 0) rk[q] = 0;
+        __syncthreads();
+        while (true) {
+            const unsigned long long bal = __ballot(q < C && rk[q] == r);
+            if ((q & (kWave - 1)) == 0) {
+                front[2 * (q / kWave)] = (unsigned)bal;
+                front[2 * (q / kWave) + 1] = (unsigned)(bal >> 32);
+            }
+            __syncthreads();
+            int m = 0;
+            for (int w = 0; w < W; ++w) m += __popc(front[w]);
+            if (m == 0) break;
+            if (q == 0) fs[r] = m;
+            if (q < C && rk[q] < 0) {
+                int dec = 0;
+#pragma unroll
+                for (int w = 0; w < kDomRows / 32; ++w) dec += __popc(dq[w] & front[w]);
+                c -= dec;
+                if (dec && c == 0) rk[q] = r + 1;
+            }
+            ++r;
+            __syncthreads();
+        }
+    } else {
     // domination counts (population.hpp:409-440)
     for (int i = threadIdx.x; i < C; i += B) {
         const float b0 = o0[i], b1 = o1[i];
@@ -210,7 +288,6 @@ __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* _
     }
     __syncthreads();
     // front peeling (population.hpp:334-369)
-    int r = 0;
     while (true) {
         if (threadIdx.x == 0) lst[0] = 0;
         __syncthreads();
@@ -235,6 +312,7 @@ __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* _
         }
         ++r;
         __syncthreads();
+    }
     }
     const int nfront = r;
     if (threadIdx.x == 0) {
@@ -425,7 +503,13 @@ hipError_t launch_nsga2_offspring(int kind, int block, const InstDesc* d_descs, 
 hipError_t launch_nsga2_select(const InstDesc* d_descs, int num_inst, int init, int max_pop, hipStream_t s)
 {
     if (num_inst <= 0) return hipSuccess;
-    const int lds = select_layout(init ? max_pop : 2 * max_pop).bytes;
+    const int C = init ? max_pop : 2 * max_pop;
+    const int Cd = C < kDomRows ? C : kDomRows;  // larger populations count in the selection
+    const int dom_blocks = (Cd * ((Cd + 31) / 32) + kDomBlock - 1) / kDomBlock;
+    hipLaunchKernelGGL(k_nsga2_dominance, dim3(dom_blocks, num_inst), dim3(kDomBlock), 0, s, d_descs, init);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int lds = select_layout(C).bytes;
     if (lds > 65536)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_nsga2_select),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds);
