@@ -260,7 +260,9 @@ def compare_nsga(pop, ost, ctx):
     assert np.array_equal(r["crowding"], ost.crowd), f"crowding differs {ctx}"
 
 
-@pytest.mark.parametrize("n,N,G", [(20, 32, 10), (120, 512, 4), (50, 64, 30)])
+# N <= 512: the selection reads dominated-by bitsets; N = 600 (1200 rows)
+# counts the dominance pairs inside the selection workgroup.
+@pytest.mark.parametrize("n,N,G", [(20, 32, 10), (120, 512, 4), (50, 64, 30), (40, 600, 3)])
 def test_nsga2_trajectory_parity(engine, n, N, G):
     print(f"nsga2 trajectory n={n} N={N}", flush=True)
     gp, op = make_problem(engine, "bitsp", n, inst=2)
