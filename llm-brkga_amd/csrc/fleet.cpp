@@ -20,6 +20,11 @@ constexpr int kBlocks[] = {64, 128, 256, 512, 1024};
 // thousand keys, so buckets of small instances take longer than their key
 // count says.
 constexpr double kChildFixedWork = 1024.0;
+
+// Lane calibration: the probe's length (about 60 us of dependent ALU steps)
+// and how many candidate streams are tried for the kLanes - 1 lanes.
+constexpr int kProbeIters = 15000;
+constexpr int kLaneCandidates = 8;
 constexpr int kChunks[] = {1, 2, 4, 8, 16};
 
 // The autotuner keeps the fastest of this many timed rounds per launch
@@ -185,9 +190,11 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
             return set_error(BRKGA_ERR_INVALID_ARG, "fleet member too large for the LDS kernels");
     }
     form_launches();
+    brkga_status st = ensure_lanes(s);
+    if (st != BRKGA_OK) return st;
     assign_lanes();
     drop_graphs();
-    brkga_status st = reserve(reinterpret_cast<void**>(&d_descs_), &descs_cap_, sizeof(InstDesc) * descs_.size(),
+    st = reserve(reinterpret_cast<void**>(&d_descs_), &descs_cap_, sizeof(InstDesc) * descs_.size(),
                               "fleet descriptors");
     if (st != BRKGA_OK) return st;
     st = reserve(reinterpret_cast<void**>(&d_queues_), &queues_cap_, sizeof(int) * kQueueStride * 2 * buckets_.size(),
@@ -275,8 +282,8 @@ void Fleet::form_launches()
 // been tuned, else population x (n + kChildFixedWork).
 void Fleet::assign_lanes()
 {
-    const int brkga_lanes = groups_.empty() ? kLanes : kLanes - 1;
-    nsga_lane_ = groups_.empty() ? 0 : (buckets_.empty() ? 0 : kLanes - 1);
+    const int brkga_lanes = groups_.empty() || nlanes_ < 2 ? nlanes_ : nlanes_ - 1;
+    nsga_lane_ = groups_.empty() || buckets_.empty() || nlanes_ < 2 ? 0 : nlanes_ - 1;
     // A bucket's measured time per child is trusted when the measurement ran
     // over at least half its current children: measured on a few members
     // (a size class the server first saw with one request) a launch is
@@ -312,14 +319,58 @@ void Fleet::assign_lanes()
     }
 }
 
-brkga_status Fleet::ensure_lanes()
+brkga_status Fleet::ensure_lanes(hipStream_t s)
 {
     if (fork_) return BRKGA_OK;
-    for (int k = 1; k < kLanes; ++k) {
-        BRKGA_HIP_TRY(hipStreamCreateWithFlags(&lanes_[k], hipStreamNonBlocking));
-        BRKGA_HIP_TRY(hipEventCreateWithFlags(&join_[k], hipEventDisableTiming));
-    }
     BRKGA_HIP_TRY(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+    for (int k = 1; k < kLanes; ++k) BRKGA_HIP_TRY(hipEventCreateWithFlags(&join_[k], hipEventDisableTiming));
+    float* probe = nullptr;
+    hipEvent_t e0 = nullptr, ea = nullptr, eb = nullptr;
+    std::vector<hipStream_t> rejected;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&probe), 2 * 64 * sizeof(float));
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&ea);
+    if (e == hipSuccess) e = hipEventCreate(&eb);
+    // start-to-end time of probes launched together on a and (unless null) b
+    auto timed = [&](hipStream_t a, hipStream_t b, float* ms) {
+        hipError_t r = hipEventRecord(e0, s);
+        if (r == hipSuccess && a != s) r = hipStreamWaitEvent(a, e0, 0);
+        if (r == hipSuccess && b) r = hipStreamWaitEvent(b, e0, 0);
+        if (r == hipSuccess) r = launch_lane_probe(kProbeIters, probe, a);
+        if (r == hipSuccess && b) r = launch_lane_probe(kProbeIters, probe + 64, b);
+        if (r == hipSuccess) r = hipEventRecord(ea, a);
+        if (r == hipSuccess && b) r = hipEventRecord(eb, b);
+        if (r == hipSuccess) r = hipEventSynchronize(ea);
+        if (r == hipSuccess && b) r = hipEventSynchronize(eb);
+        float ta = 0.0f, tb = 0.0f;
+        if (r == hipSuccess) r = hipEventElapsedTime(&ta, e0, ea);
+        if (r == hipSuccess && b) r = hipEventElapsedTime(&tb, e0, eb);
+        *ms = std::max(ta, tb);
+        return r;
+    };
+    float alone = 0.0f;
+    for (int w = 0; w < 2 && e == hipSuccess; ++w) e = timed(s, nullptr, &alone);  // the first one warms up
+    int found = 1;
+    for (int c = 0; c < kLaneCandidates && found < kLanes && e == hipSuccess; ++c) {
+        hipStream_t cand = nullptr;
+        e = hipStreamCreateWithFlags(&cand, hipStreamNonBlocking);
+        if (e != hipSuccess) break;
+        bool concurrent = true;
+        for (int k = 0; k < found && concurrent && e == hipSuccess; ++k) {
+            float ms = 0.0f;
+            e = timed(k == 0 ? s : lanes_[k], cand, &ms);
+            concurrent = ms < 1.5f * alone;
+        }
+        if (concurrent && e == hipSuccess) lanes_[found++] = cand;
+        else rejected.push_back(cand);
+    }
+    for (hipStream_t r : rejected) (void)hipStreamDestroy(r);
+    if (e0) (void)hipEventDestroy(e0);
+    if (ea) (void)hipEventDestroy(ea);
+    if (eb) (void)hipEventDestroy(eb);
+    if (probe) (void)hipFree(probe);
+    nlanes_ = found;
+    if (e != hipSuccess) return hip_error(e, "stream lane calibration");
     return BRKGA_OK;
 }
 
@@ -335,7 +386,7 @@ brkga_status Fleet::set_counters(int gen, hipStream_t s)
 brkga_status Fleet::tick(hipStream_t s)
 {
     DeviceGuard g(eng_->device);
-    brkga_status st = ensure_lanes();
+    brkga_status st = ensure_lanes(s);
     if (st != BRKGA_OK) return st;
     bool used[kLanes] = {};
     for (const auto& L : launches_) used[L.lane] = true;
@@ -399,7 +450,7 @@ brkga_status Fleet::tick_graph(int ticks, hipStream_t s)
     auto it = graphs_.find(ticks);
     if (it == graphs_.end()) {
         if (!capture_) BRKGA_HIP_TRY(hipStreamCreateWithFlags(&capture_, hipStreamNonBlocking));
-        brkga_status ls = ensure_lanes();  // no stream / event creation while capturing
+        brkga_status ls = ensure_lanes(s);  // no stream / event creation while capturing
         if (ls != BRKGA_OK) return ls;
         hipGraph_t graph = nullptr;
         BRKGA_HIP_TRY(hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal));

@@ -94,7 +94,9 @@ private:
     void form_launches();
     brkga_status tune(int reps, bool only_new, hipStream_t s);
     brkga_status reserve(void** p, size_t* cap, size_t bytes, const char* what);
-    brkga_status ensure_lanes();
+    // Creates the stream lanes once, keeping only streams that run
+    // concurrently with s and with each other (timed probes).
+    brkga_status ensure_lanes(hipStream_t s);
     void assign_lanes();
     void drop_graphs();
 
@@ -122,6 +124,7 @@ private:
     hipStream_t capture_ = nullptr;
     std::map<int, hipGraphExec_t> graphs_;
     hipStream_t lanes_[kLanes] = {};  // [0] unused: lane 0 is the caller's stream
+    int nlanes_ = 1;                  // lanes in use: the caller's stream and lanes_[1, nlanes_)
     hipEvent_t fork_ = nullptr;
     hipEvent_t join_[kLanes] = {};
     int nsga_lane_ = 0;

@@ -58,6 +58,22 @@ int step_queue_blocks_per_cu(int kind, int block, bool xys, int n_max, int lds)
     return nb;
 }
 
+// A fixed-length chain of dependent ALU operations in one wave (no memory
+// waits, so it always ends): timed on two streams at once it shows whether
+// their work runs concurrently.
+__global__ __launch_bounds__(64) void k_lane_probe(int iters, float* out)
+{
+    float x = (float)threadIdx.x;
+    for (int i = 0; i < iters; ++i) x = x * 0.999f + 1.0f;
+    out[threadIdx.x] = x;
+}
+
+hipError_t launch_lane_probe(int iters, float* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lane_probe, dim3(1), dim3(64), 0, s, iters, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_set_counters(const InstDesc* d_descs, int num, int gen, hipStream_t s)
 {
     if (num <= 0) return hipSuccess;

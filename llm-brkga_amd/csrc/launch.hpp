@@ -64,6 +64,9 @@ inline int queue_items_for(int kind, int block, int n_max)
 }
 // Resident workgroups of the queue step per CU (0: no such kernel).
 int step_queue_blocks_per_cu(int kind, int block, bool xys, int n_max, int lds);
+// One wave of `iters` dependent ALU steps writing out[0, 64): a probe of
+// whether two streams run concurrently.
+hipError_t launch_lane_probe(int iters, float* out, hipStream_t s);
 // Step counter (both parity slots) := gen and arrival counter := 0 for `num`
 // instances.
 hipError_t launch_set_counters(const InstDesc* d_descs, int num, int gen, hipStream_t s);
