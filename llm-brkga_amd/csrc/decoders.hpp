@@ -16,6 +16,77 @@
 
 namespace brkga {
 
+// The TSP length over coordinates once the keys are ranked: each key writes
+// its city's coordinates c[i] at its tour position pos[i] (element mapping
+// CONTIG as in block_rank_by), then the edges are summed from consecutive
+// positions.  Tour positions go to the argsort counters, dead once the keys
+// are placed, when they hold n coordinate pairs (nb >= 2n); then no barrier
+// is needed against the ranking's reads of the items.  EARLY: c[] was
+// loaded during the ranking; otherwise load() fills it here.
+template <int BLOCK, int ITEMS, bool CONTIG, bool EARLY, typename Load>
+__device__ __forceinline__ void tsp_coords_tail(const ProblemDesc& p, char* lds, const DecodeLayout& lay,
+                                                const SortScratch& ss, int nb, float2 (&c)[ITEMS],
+                                                const int (&pos)[ITEMS], float* fit_out, int* decoded, Load load)
+{
+    const int tid = threadIdx.x;
+    const int n = p.n;
+    const bool in_cnt = nb + 4 >= 2 * n;
+    float2* txy = reinterpret_cast<float2*>(in_cnt ? reinterpret_cast<char*>(ss.cnt) : lds + lay.items_off);
+    if constexpr (!EARLY) load();
+    if (!in_cnt) __syncthreads();  // every ranking read of the items is done
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
+        if (e < n) {
+            txy[pos[i]] = c[i];
+            if (decoded) decoded[pos[i]] = e;
+        }
+    }
+    __syncthreads();
+    FxAcc acc;
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+        const int q = tid + i * BLOCK;
+        if (q < n) {
+            const float2 a = txy[q];
+            const float2 b = txy[q + 1 < n ? q + 1 : 0];
+            fx_add(acc, edge_xy(a.x, a.y, b.x, b.y), p.scale0);
+        }
+    }
+    const long long tot = block_sum_fx<BLOCK>(fx_value(acc), ss);
+    if (tid == 0 && fit_out) fit_out[0] = fx_to_float(tot, p.inv0);
+}
+
+// TSP over coordinates with the child's keys in registers: thread t holds
+// keys 4t .. 4t+3 (the float4 column it built, so n <= 4 BLOCK) and ranks
+// them straight away, with no barrier between building the child and
+// ranking it.  The argsort counters were cleared before the caller's last
+// barrier.
+template <int BLOCK>
+__device__ __forceinline__ void decode_tsp_coords_reg(const ProblemDesc& p, char* lds, const DecodeLayout& lay,
+                                                      int nb, float* fit_out, float4 kv)
+{
+    const int tid = threadIdx.x;
+    const int n = p.n;
+    SortScratch ss;
+    ss.cnt = reinterpret_cast<int*>(lds + lay.cnt_off);
+    ss.items = reinterpret_cast<unsigned long long*>(lds + lay.items_off);
+    ss.red = reinterpret_cast<long long*>(lds + lay.red_off);
+    const float kk[4] = {kv.x, kv.y, kv.z, kv.w};
+    const float2* xy = reinterpret_cast<const float2*>(p.a);
+    float2 c[4];
+    int pos[4];
+    auto load = [&]() {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int e = 4 * tid + i;
+            if (e < n) c[i] = xy[e];
+        }
+    };
+    block_rank_by<BLOCK, true, 4, true>([&](int i, int) { return kk[i]; }, n, nb, ss, pos, nullptr, load);
+    tsp_coords_tail<BLOCK, 4, true, true>(p, lds, lay, ss, nb, c, pos, fit_out, nullptr, load);
+}
+
 // CLEARED: the argsort counters were cleared (and a barrier passed) by the
 // caller.  XY_LDS: coordinates were staged in LDS at xy0 / xy1 (coordinate
 // kinds only); otherwise they are read from p.a / p.b.  ITEMS: keys per
@@ -56,34 +127,7 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
             block_rank<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, pos, nullptr, load);
         else
             block_rank<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, pos);
-        // Tour positions go to the argsort counters, dead once the keys are
-        // placed, when they hold n coordinate pairs (nb >= 2n); then no
-        // barrier is needed against the ranking's reads of the items.
-        const bool in_cnt = nb + 4 >= 2 * n;
-        float2* txy = reinterpret_cast<float2*>(in_cnt ? reinterpret_cast<char*>(ss.cnt) : lds + lay.items_off);
-        if constexpr (!kEarly) load();
-        if (!in_cnt) __syncthreads();  // every ranking read of the items is done
-#pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-            const int e = tid + i * BLOCK;
-            if (e < n) {
-                txy[pos[i]] = c[i];
-                if (decoded) decoded[pos[i]] = e;
-            }
-        }
-        __syncthreads();
-        FxAcc acc;
-#pragma unroll
-        for (int i = 0; i < ITEMS; ++i) {
-            const int q = tid + i * BLOCK;
-            if (q < n) {
-                const float2 a = txy[q];
-                const float2 b = txy[q + 1 < n ? q + 1 : 0];
-                fx_add(acc, edge_xy(a.x, a.y, b.x, b.y), p.scale0);
-            }
-        }
-        const long long tot = block_sum_fx<BLOCK>(fx_value(acc), ss);
-        if (tid == 0 && fit_out) fit_out[0] = fx_to_float(tot, p.inv0);
+        tsp_coords_tail<BLOCK, ITEMS, false, kEarly>(p, lds, lay, ss, nb, c, pos, fit_out, decoded, load);
     } else if constexpr (KIND == KIND_TSP_MATRIX) {
         block_argsort<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, perm);
         // thread t sums the edges leaving positions [t*chunk, t*chunk+chunk)

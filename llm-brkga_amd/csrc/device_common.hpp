@@ -487,9 +487,20 @@ struct NoHook {
     __device__ void operator()() const {}
 };
 
-template <int BLOCK, bool CLEARED = false, int ITEMS = kSortItems, typename Hook = NoHook>
-__device__ __forceinline__ void block_rank(const float* keys, int n, int nb, const SortScratch& s,
-                                           int (&pos)[ITEMS], int* perm = nullptr, Hook pre_rank = Hook())
+// Element index of a thread's i-th key: tid + i * BLOCK (strided), or
+// tid * ITEMS + i (CONTIG: a thread's keys are consecutive).
+template <int BLOCK, int ITEMS, bool CONTIG>
+__device__ __forceinline__ int rank_elem(int tid, int i)
+{
+    return CONTIG ? tid * ITEMS + i : tid + i * BLOCK;
+}
+
+// block_rank with the keys supplied by key_at(i, e) (the thread's i-th key,
+// element e) and the element mapping CONTIG; positions and perm follow the
+// same mapping.
+template <int BLOCK, bool CLEARED, int ITEMS, bool CONTIG, typename KeyAt, typename Hook = NoHook>
+__device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const SortScratch& s,
+                                              int (&pos)[ITEMS], int* perm = nullptr, Hook pre_rank = Hook())
 {
     const int tid = threadIdx.x;
     if (!CLEARED) {
@@ -502,9 +513,9 @@ __device__ __forceinline__ void block_rank(const float* keys, int n, int nb, con
     uint32_t ok[ITEMS];
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-        const int e = tid + i * BLOCK;
+        const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
         if (e < n) {
-            const float k = keys[e];
+            const float k = key_at(i, e);
             ok[i] = key_order(k);
             rng[i] = key_bucket(k, nb);
             slot[i] = atomicAdd(&s.cnt[rng[i]], 1);
@@ -514,7 +525,7 @@ __device__ __forceinline__ void block_rank(const float* keys, int n, int nb, con
     scan_bucket_counts<BLOCK>(s.cnt, nb, n, reinterpret_cast<int*>(s.red));
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-        const int e = tid + i * BLOCK;
+        const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
         if (e < n) {
             const int st = s.cnt[rng[i]];
             const int en = s.cnt[rng[i] + 1];
@@ -529,7 +540,7 @@ __device__ __forceinline__ void block_rank(const float* keys, int n, int nb, con
     // finish in a loop.  A key compared with itself counts nothing.
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-        const int e = tid + i * BLOCK;
+        const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
         if (e < n) {
             const int st = rng[i] & 0xFFFF;
             const int sz = rng[i] >> 16;
@@ -548,6 +559,14 @@ __device__ __forceinline__ void block_rank(const float* keys, int n, int nb, con
             if (perm) perm[st + r] = e;
         }
     }
+}
+
+template <int BLOCK, bool CLEARED = false, int ITEMS = kSortItems, typename Hook = NoHook>
+__device__ __forceinline__ void block_rank(const float* keys, int n, int nb, const SortScratch& s,
+                                           int (&pos)[ITEMS], int* perm = nullptr, Hook pre_rank = Hook())
+{
+    block_rank_by<BLOCK, CLEARED, ITEMS, false>([&](int, int e) { return keys[e]; }, n, nb, s, pos, perm,
+                                                pre_rank);
 }
 
 // perm[p] = index of the element at position p of the stable ascending

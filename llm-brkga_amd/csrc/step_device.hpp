@@ -159,11 +159,12 @@ __device__ __forceinline__ float4 unit4(const U4& r)
 // fitness instead; then it returns false and nothing is left to decode.
 // `order` is the island's rank order (LDS-staged or global); `mig_order`
 // the rank order of the preceding island on the ring, or null to read it
-// from the partition's output.
+// from the partition's output.  mine (optional, nq <= BLOCK): the column
+// threadIdx.x this thread built, left unchanged for threads with none.
 template <int BLOCK>
 __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s, int cur, int init, int t,
                                             const int* order, const int* mig_order, float4* out, float4* kl,
-                                            float* fit_out)
+                                            float* fit_out, float4* mine = nullptr)
 {
     const int Pi = d.Pi, Ei = d.Ei, Mi = d.Mi, I = d.islands;
     const int M = d.prob.num_obj, ld = d.ld;
@@ -173,6 +174,7 @@ __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s
             const float4 x = unit4(rng_block(d.seed_lo, d.seed_hi, island, 0, s, TAG_INIT, c));
             out[c] = x;
             kl[c] = x;
+            if (mine) *mine = x;
         }
         return true;
     }
@@ -197,6 +199,7 @@ __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s
             const float4 x = unit4(rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_MUTANT, c));
             out[c] = x;
             kl[c] = x;
+            if (mine) *mine = x;
         }
         return true;
     }
@@ -216,6 +219,7 @@ __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s
         x.w = unit_from_u32(r.w) < rho ? a.w : b.w;
         out[c] = x;
         kl[c] = x;
+        if (mine) *mine = x;
     }
     return true;
 }

@@ -91,10 +91,18 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
     }
     __syncthreads();
 
-    const bool child = build_child<BLOCK>(d, island, s, cur, init, t, own_order, mig_order, out, kl, fit_out);
-    if (!child) return;
-    __syncthreads();
-    decode_block<KIND, BLOCK, true, false, ITEMS>(d.prob, lds, sl.dec, nb, fit_out, nullptr);
+    if constexpr (KIND == KIND_TSP_COORDS && ITEMS == kCompactItems) {
+        // n <= 4 BLOCK: each thread ranks the one column it built, from
+        // registers, without waiting for the others
+        float4 mine = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (!build_child<BLOCK>(d, island, s, cur, init, t, own_order, mig_order, out, kl, fit_out, &mine)) return;
+        decode_tsp_coords_reg<BLOCK>(d.prob, lds, sl.dec, nb, fit_out, mine);
+    } else {
+        const bool child = build_child<BLOCK>(d, island, s, cur, init, t, own_order, mig_order, out, kl, fit_out);
+        if (!child) return;
+        __syncthreads();
+        decode_block<KIND, BLOCK, true, false, ITEMS>(d.prob, lds, sl.dec, nb, fit_out, nullptr);
+    }
 }
 
 // Child slot of workgroup b.  Workgroups go to the 8 XCDs round-robin
