@@ -125,14 +125,20 @@ __device__ __forceinline__ unsigned long long block_bitonic_sort(unsigned long l
 // values, so the slot has to travel with the key.)  pow2(Pi) <= BLOCK; the
 // network spans max(pow2(Pi), 64) threads; xch holds island_rank_bytes(Pi)
 // bytes.  order is complete after the caller's next barrier.
-template <int BLOCK>
-__device__ __forceinline__ void island_rank_order(const InstDesc& d, int island, int buf, int* order, void* xch)
+// while_loading() runs between issuing the fitness loads and their first
+// use, so loads it issues overlap theirs.
+template <int BLOCK, typename Hook = NoHook>
+__device__ __forceinline__ void island_rank_order(const InstDesc& d, int island, int buf, int* order, void* xch,
+                                                  Hook while_loading = Hook())
 {
     const int Pi = d.Pi, M = d.prob.num_obj;
     const int t = threadIdx.x;
     const float* fit = d.fit[buf] + (size_t)island * Pi * M;
+    float f = 0.0f;
+    if (t < Pi) f = fit[(size_t)t * M];
+    while_loading();
     unsigned long long v = ~0ull;
-    if (t < Pi) v = ((unsigned long long)fitness_order_key(fit[(size_t)t * M], d.prob.maximize) << 32) | (unsigned)t;
+    if (t < Pi) v = ((unsigned long long)fitness_order_key(f, d.prob.maximize) << 32) | (unsigned)t;
     unsigned long long* x = static_cast<unsigned long long*>(xch);
     if (Pi <= 64) v = block_bitonic_sort<64>(v, x);
     else if constexpr (BLOCK >= 128) {

@@ -19,6 +19,16 @@
 
 namespace brkga {
 
+// A step counter slot read through the scalar cache: the slot a step reads
+// was written by an earlier launch (kernel start invalidates the cache) and
+// no workgroup of the same launch writes it.  A scalar load is waited for
+// on its own counter, so loads issued after it (the island's fitness, the
+// staged rank order) are in flight before the counter is needed.
+__device__ __forceinline__ int load_step_counter(const int* slot)
+{
+    return *(const __attribute__((address_space(4))) int*)slot;
+}
+
 // One block of a generation step: slot gs of the population described by d.
 // n <= ITEMS * BLOCK.  rank = 0: the parents' rank order comes from the
 // partition kernel (perm[cur]) and the partition advances the step counter.
@@ -64,24 +74,31 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
     if (init) {
         argsort_clear<BLOCK>(ss, nb);
     } else if (rank) {
-        // only a slot that migration may claim needs the counter to choose
-        // the island it ranks: every other block issues the island's
-        // fitness loads together with the counter's.  The argsort counters
-        // (their own LDS) are cleared first; one barrier below ends both.
-        t = *gen_slot(d, cur);
+        // the step counter is read while the island's fitness loads are in
+        // flight; only a slot that migration may claim needs it first, to
+        // choose the island it ranks.  The argsort counters (their own LDS)
+        // are cleared first; one barrier below ends both.
         argsort_clear<BLOCK>(ss, nb);
+        auto after_loads = [&]() {
+            if (!maybe_migrant) t = load_step_counter(gen_slot(d, cur));
+        };
         if (!maybe_migrant) {
             if (s < Ei || s < Pi - d.Mi)
-                island_rank_order<BLOCK>(d, island, cur, order, lds + sl.bytes);
-        } else if (t > 0 && (t % d.mig_interval) == 0) {
-            island_rank_order<BLOCK>(d, (island + I - 1) % I, cur, order, lds + sl.bytes);
-            mig_order = order;
-        } else if (s < Ei || s < Pi - d.Mi) {
-            island_rank_order<BLOCK>(d, island, cur, order, lds + sl.bytes);
+                island_rank_order<BLOCK>(d, island, cur, order, lds + sl.bytes, after_loads);
+            else
+                t = load_step_counter(gen_slot(d, cur));
+        } else {
+            t = load_step_counter(gen_slot(d, cur));
+            if (t > 0 && (t % d.mig_interval) == 0) {
+                island_rank_order<BLOCK>(d, (island + I - 1) % I, cur, order, lds + sl.bytes, after_loads);
+                mig_order = order;
+            } else if (s < Ei || s < Pi - d.Mi) {
+                island_rank_order<BLOCK>(d, island, cur, order, lds + sl.bytes, after_loads);
+            }
         }
         if (gs == 0 && threadIdx.x == 0) *gen_slot(d, cur ^ 1) = t + 1;
     } else {
-        t = *d.gen;
+        t = load_step_counter(d.gen);
         // stage the island rank order, clear the argsort counters
         if (Pi <= kMaxStagedOrder)
             for (int i = threadIdx.x; i < Pi; i += BLOCK) order[i] = perm_cur[i];
