@@ -101,6 +101,26 @@ def test_server_alone_equals_in_mix(engine):
     check(mixed[1], (alone[0], alone[1]), "request 1 alone vs in the mix")
 
 
+def test_server_long_run_with_joins_matches_solo(engine):
+    """A request alone for many rounds, then two more joining while it runs:
+    every result still equals its solo run."""
+    print("server long run with joins", flush=True)
+    reqs = build_requests(engine)
+    picks = [(reqs[1][0], reqs[1][1], 300), (reqs[16][0], reqs[16][1], 25), (reqs[0][0], reqs[0][1], 30)]
+    refs = [solo_best(gp, prm, G) for gp, prm, G in picks]
+    srv = brkga_amd.Server(engine, hbm_budget_bytes=512 << 20, ticks_per_round=2)
+    first = srv.submit(*picks[0])
+    time.sleep(0.005)  # rounds with one member, then two requests join
+    ids = [first] + [srv.submit(*p) for p in picks[1:]]
+    results = [srv.wait(r, timeout_ms=60000) for r in ids]
+    st = srv.stats()
+    srv.stop()
+    for i, (res, ref) in enumerate(zip(results, refs)):
+        check(res, ref, f"long run with joins, request {i}")
+        assert res[2]["generations"] == picks[i][2]
+    assert st["completed"] == 3 and st["rounds"] >= 150
+
+
 def test_server_rejects_and_reports(engine):
     print("server rejection / wait semantics", flush=True)
     gp, _ = make_problem(engine, "tsp", 2000, inst=7)
