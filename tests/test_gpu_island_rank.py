@@ -5,8 +5,9 @@ partition launch after each step (islands wider than the workgroup).
 Both must give the oracle's trajectory bit for bit: keys, fitness and rank
 orders.  The cases cover islands that are not powers of two (padded
 networks), islands of one wave and of several waves (LDS exchange steps),
-the widest islands (1024), ring migration on the generations the cases run
-through, maximisation with many equal fitness values (knapsack), and a mix
+the widest in-step islands (1024), islands wider than the workgroup's
+staged rank order (2560, 3000: the step reads the partition's order from
+global memory), ring migration on the generations the cases run through, maximisation with many equal fitness values (knapsack), and a mix
 of single launches and graph replays.
 """
 import numpy as np
@@ -26,6 +27,8 @@ RANK_CASES = [
     ("knap_threshold", 60, 64, 2, 10, 6, 2, 2, 5, (64, 128)),
     ("tsp", 40, 512, 2, 60, 40, 2, 4, 4, (256, 512)),
     ("tsp_matrix", 30, 1000, 1, 100, 80, 0, 0, 3, (512, 1024)),
+    ("tsp", 30, 2560, 1, 300, 200, 0, 0, 3, (64, 256)),
+    ("tsp", 25, 3000, 2, 400, 300, 2, 5, 4, (128,)),
 ]
 
 
