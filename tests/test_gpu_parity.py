@@ -30,7 +30,7 @@ def gpu_evaluate(prob, keys):
     return (f[:, 0] if prob.num_obj == 1 else f), d.cpu().numpy()
 
 
-EVAL_CASES = [("tsp", n) for n in (1, 2, 3, 52, 127, 128, 129, 1000, 5000)] + [
+EVAL_CASES = [("tsp", n) for n in (1, 2, 3, 52, 127, 128, 129, 1000, 5000, 12000)] + [
     ("tsp_matrix", 1), ("tsp_matrix", 60), ("tsp_matrix", 1000),
     ("knap_greedy", 1), ("knap_greedy", 63), ("knap_greedy", 500), ("knap_greedy", 2000),
     ("knap_threshold", 1), ("knap_threshold", 500), ("knap_threshold", 2000),
@@ -49,6 +49,17 @@ def test_evaluate_and_decode_parity(engine, kind, n):
         of, od = op.evaluate(keys)
         assert np.array_equal(gd, od), f"decode mismatch ({kind}, n={n}, ties={ties})"
         assert np.array_equal(gf, of), f"fitness mismatch ({kind}, n={n}, ties={ties})"
+
+
+def test_problem_size_limits(engine):
+    """Above 16000 genes, or beyond the decoder's LDS (TSP n = 16000 needs
+    more than 160 KiB), creation fails with a clear error."""
+    from brkga_amd import workloads
+    for n in (16001, 16000):
+        with pytest.raises(brkga_amd.BrkgaError) as e:
+            brkga_amd.Problem.tsp_coords(engine, workloads.tsp_coords(3, n, 100.0))
+        assert e.value.status == brkga_amd.ERR_INVALID_ARG
+        assert ("problem size" in str(e.value)) if n > 16000 else ("LDS" in str(e.value))
 
 
 def test_evaluate_rejects_wrong_chrom_len(engine):
