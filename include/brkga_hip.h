@@ -219,6 +219,20 @@ brkga_status brkga_decode(brkga_problem_t problem, const float* d_keys, int32_t*
 brkga_status brkga_evaluate_host(brkga_problem_t problem, const float* keys, int32_t count, float* fitness,
                                  int32_t* decoded);
 
+/* brkga_evaluate_host without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * per-individual evaluation loops such as a config's fitness_function
+ * callback, harnesses that forbid hipMalloc).  The device staging of keys,
+ * fitness and decoded rows lives in a static 4 MiB array per device,
+ * resolved when an engine is created on that device; a call needs
+ * align256(4 count chrom_len) + align256(4 count num_obj) (+ 4 count
+ * chrom_len when decoded is not NULL) bytes and is rejected with
+ * BRKGA_ERR_INVALID_ARG above 4 MiB.  Every call on a device shares that
+ * array: calls from several host threads are serialised by the library.
+ * Results are identical to brkga_evaluate_host.  Synchronous. */
+brkga_status brkga_evaluate_host_noalloc(brkga_problem_t problem, const float* keys, int32_t count, float* fitness,
+                                         int32_t* decoded);
+
 /* -------------------------------------------------------- populations --- */
 
 /* Device-resident population: ping-pong key buffers [P][ld], fitness,

@@ -145,6 +145,7 @@ _SIGNATURES = {
     "brkga_evaluate": ([_P, _P, _P, _I32, _I32, _P], ctypes.c_int),
     "brkga_decode": ([_P, _P, _P, _I32, _I32, _P], ctypes.c_int),
     "brkga_evaluate_host": ([_P, _PF, _I32, _PF, _PI], ctypes.c_int),
+    "brkga_evaluate_host_noalloc": ([_P, _PF, _I32, _PF, _PI], ctypes.c_int),
     "brkga_population_create": ([_P, ctypes.POINTER(CParams), ctypes.POINTER(_P)], ctypes.c_int),
     "brkga_population_destroy": ([_P], ctypes.c_int),
     "brkga_population_initialize": ([_P, _P], ctypes.c_int),

@@ -11,6 +11,44 @@
 #include "device_common.hpp"
 #include "launch.hpp"
 
+// Device staging of brkga_evaluate_host_noalloc: the keys, fitness and
+// decoded rows of one call.  Each device the library runs on holds its own
+// copy until the process ends.  Host code never uses this name as an
+// address: engine creation resolves the device address once per device
+// (hipGetSymbolAddress) into noalloc_scratch_ptr, and the entry point uses
+// only that.
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_resolve_mu;
+unsigned char* g_scratch_ptr[kMaxScratchDevices];  // device address of `scratch`, per device
+std::mutex g_scratch_call_mu[kMaxScratchDevices];  // one call at a time per device
+
+// Resolves the address of `scratch` on `device`, which must be current.
+brkga_status resolve_noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices)
+        return brkga::set_error(BRKGA_ERR_INVALID_ARG, "device index beyond the static scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_resolve_mu);
+    if (g_scratch_ptr[device]) return BRKGA_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return brkga::hip_error(e == hipSuccess ? hipErrorInvalidSymbol : e, "hipGetSymbolAddress(scratch)");
+    }
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    return BRKGA_OK;
+}
+
+size_t align256(size_t v)
+{
+    return (v + 255) & ~(size_t)255;
+}
+}  // namespace
+
 namespace brkga {
 
 namespace {
@@ -502,6 +540,10 @@ brkga_status brkga_engine_create(int32_t device, brkga_engine_t* out)
         lds_optin = 0;
     }
     BRKGA_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    // set-up of the allocation-free entry point, once per device, with the
+    // device current
+    const brkga_status sst = resolve_noalloc_scratch(device);
+    if (sst != BRKGA_OK) return sst;
     auto* eng = new brkga_engine_s();
     eng->device = device;
     eng->lds_limit = (size_t)std::max(lds, lds_optin);
@@ -799,6 +841,44 @@ brkga_status brkga_evaluate_host(brkga_problem_t problem, const float* keys, int
     for (void* q : {dk, df, dd})
         if (q) (void)hipFree(q);
     return st;
+}
+
+brkga_status brkga_evaluate_host_noalloc(brkga_problem_t problem, const float* keys, int32_t count, float* fitness,
+                                         int32_t* decoded)
+{
+    if (!problem || !keys || count < 0) return set_error(BRKGA_ERR_INVALID_ARG, "null problem / keys or negative count");
+    if (count == 0) return BRKGA_OK;
+    const ProblemDesc& p = problem->desc;
+    const int dev = problem->engine->device;
+    const size_t nk = (size_t)count * p.chrom_len;
+    const size_t off_fit = align256(nk * sizeof(float));
+    const size_t off_dec = off_fit + align256((size_t)count * p.num_obj * sizeof(float));
+    const size_t need = off_dec + (decoded ? nk * sizeof(int) : 0);
+    if (need > kNoAllocScratchBytes)
+        return set_error(BRKGA_ERR_INVALID_ARG, "evaluate_host_noalloc: " + std::to_string(count) + " rows of " +
+                                                    std::to_string(p.chrom_len) + " keys need " + std::to_string(need) +
+                                                    " bytes of device staging; the static scratch holds " +
+                                                    std::to_string(kNoAllocScratchBytes));
+    if (dev < 0 || dev >= kMaxScratchDevices || !g_scratch_ptr[dev])
+        return set_error(BRKGA_ERR_DEVICE, "static scratch not resolved for this device");
+    DeviceGuard g(dev);
+    // every call on a device shares its scratch: calls from several host
+    // threads run one after the other
+    std::lock_guard<std::mutex> lock(g_scratch_call_mu[dev]);
+    unsigned char* base = g_scratch_ptr[dev];
+    float* dk = reinterpret_cast<float*>(base);
+    float* df = reinterpret_cast<float*>(base + off_fit);
+    int* dd = decoded ? reinterpret_cast<int*>(base + off_dec) : nullptr;
+    // the decode writes every fitness and decoded entry of the rows it is
+    // given, so nothing left by an earlier call is read
+    hipError_t e = hipMemcpy(dk, keys, nk * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_evaluate(p, dk, p.chrom_len, count, df, dd, default_block(p.n), nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e == hipSuccess && fitness)
+        e = hipMemcpy(fitness, df, (size_t)count * p.num_obj * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && decoded) e = hipMemcpy(decoded, dd, nk * sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_error(e, "evaluate_host_noalloc");
+    return BRKGA_OK;
 }
 
 brkga_status brkga_population_create(brkga_problem_t problem, const brkga_params* params, brkga_pop_t* out)
