@@ -20,8 +20,10 @@ namespace brkga {
 // its city's coordinates c[i] at its tour position pos[i] (element mapping
 // CONTIG as in block_rank_by), then the edges are summed from consecutive
 // positions.  Tour positions go to the argsort counters, dead once the keys
-// are placed, when they hold n coordinate pairs (nb >= 2n); then no barrier
-// is needed against the ranking's reads of the items.  EARLY: c[] was
+// are placed, when cnt[0, nb) holds n coordinate pairs (nb >= 2n); then no
+// barrier is needed against the ranking's reads of the items, and cnt[nb + 1]
+// (the long-bucket flag every thread reads after the placement) is never
+// overwritten.  EARLY: c[] was
 // loaded during the ranking; otherwise load() fills it here.
 template <int BLOCK, int ITEMS, bool CONTIG, bool EARLY, typename Load>
 __device__ __forceinline__ void tsp_coords_tail(const ProblemDesc& p, char* lds, const DecodeLayout& lay,
@@ -30,7 +32,7 @@ __device__ __forceinline__ void tsp_coords_tail(const ProblemDesc& p, char* lds,
 {
     const int tid = threadIdx.x;
     const int n = p.n;
-    const bool in_cnt = nb + 4 >= 2 * n;
+    const bool in_cnt = nb >= 2 * n;
     float2* txy = reinterpret_cast<float2*>(in_cnt ? reinterpret_cast<char*>(ss.cnt) : lds + lay.items_off);
     if constexpr (!EARLY) load();
     if (!in_cnt) __syncthreads();  // every ranking read of the items is done

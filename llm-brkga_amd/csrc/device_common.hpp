@@ -374,9 +374,12 @@ __device__ __forceinline__ long long fx_value(const FxAcc& a)
 // ------------------------------------------------------------- argsort ----
 
 // Bucket of a key for nb (power of two) value buckets over [0,1); any other
-// value is clamped to the first/last bucket, which keeps the map monotone.
+// value is clamped to the first/last bucket (NaN to the last, where its
+// order key sorts after every number), which keeps the map monotone in the
+// order of key_order.
 __device__ __forceinline__ int key_bucket(float k, int nb)
 {
+    if (k != k) return nb - 1;
     if (!(k >= 0.0f)) return 0;
     if (k >= 1.0f) return nb - 1;
     int b = (int)__fmul_rn(k, (float)nb);
@@ -393,7 +396,8 @@ __device__ __forceinline__ uint32_t key_order(float k)
 }
 
 struct SortScratch {
-    int* cnt;                   // [nb + 1] bucket counts, then bucket starts
+    int* cnt;                   // [nb + 1] bucket counts, then bucket starts;
+                                // cnt[nb + 1]: some bucket is too long to scan
     unsigned long long* items;  // [n] (key order << 32 | index) in bucket order
     long long* red;             // [17] per-wave partials; red[16] = block sum
 };
@@ -463,6 +467,58 @@ __device__ __forceinline__ void scan_bucket_counts(int* cnt, int nb, int n, int*
     __syncthreads();
 }
 
+// Buckets longer than this are not ranked by scanning: a row with that many
+// keys in one bucket (equal keys, keys outside [0, 1), written by a caller)
+// is sorted instead (block_sort_items), so a row costs O(n log^2 n) however
+// its keys are distributed.  Philox keys fill about half a bucket each, and
+// a bucket over 64 keys has probability below 1e-30 per row.
+constexpr int kBucketScanLimit = 64;
+
+// Ascending sort of the n unique 64-bit items a[0..n) in LDS by the whole
+// workgroup: a bitonic network in flip form (every compare-exchange puts the
+// smaller item at the lower index) over pow2(n) positions, positions >= n
+// standing for +infinity, so a compare-exchange that reaches one leaves both
+// items in place and is skipped.  One barrier per stage; ends with one.
+template <int BLOCK>
+__device__ void block_sort_items(unsigned long long* a, int n)
+{
+    int pn = 1;
+    while (pn < n) pn <<= 1;
+    const int half = pn >> 1;
+    for (int k = 2; k <= pn; k <<= 1) {
+        for (int j = k >> 1; j >= 1; j >>= 1) {
+            const int lj = __builtin_ctz(j);
+            const bool flip = j == (k >> 1);
+            for (int c = threadIdx.x; c < half; c += BLOCK) {
+                const int base = (c >> lj) << (lj + 1);
+                const int off = c & (j - 1);
+                const int lo = base + off;
+                const int hi = flip ? base + 2 * j - 1 - off : lo + j;
+                if (hi < n) {
+                    const unsigned long long x = a[lo], y = a[hi];
+                    if (y < x) {
+                        a[lo] = y;
+                        a[hi] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Index of item v in the sorted unique items a[0..n).
+__device__ __forceinline__ int sorted_items_find(const unsigned long long* a, int n, unsigned long long v)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // Sorted positions of keys[0..n) in the stable ascending order (equal keys
 // keep ascending index order, i.e. std::sort on pair<float,int>(key, index)
 // as configs/tsp_config.hpp:209-216 does): on return pos[i] is the position
@@ -523,6 +579,7 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
     }
     __syncthreads();
     scan_bucket_counts<BLOCK>(s.cnt, nb, n, reinterpret_cast<int*>(s.red));
+    int longest = 0;
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
@@ -531,10 +588,27 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
             const int en = s.cnt[rng[i] + 1];
             s.items[st + slot[i]] = ((unsigned long long)ok[i] << 32) | (unsigned)e;
             rng[i] = st | ((en - st) << 16);
+            longest = en - st > longest ? en - st : longest;
         }
     }
+    if (longest > kBucketScanLimit) s.cnt[nb + 1] = 1;
     __syncthreads();
     pre_rank();
+    if (__builtin_amdgcn_readfirstlane(s.cnt[nb + 1])) {
+        // a bucket too long to scan: sort every item, then each key finds
+        // its own (unique) item
+        block_sort_items<BLOCK>(s.items, n);
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
+            if (e < n) {
+                const int p = sorted_items_find(s.items, n, ((unsigned long long)ok[i] << 32) | (unsigned)e);
+                pos[i] = p;
+                if (perm) perm[p] = e;
+            }
+        }
+        return;
+    }
     // A key alone in its bucket reads nothing; otherwise the first four
     // items of the bucket are read as independent loads and longer buckets
     // finish in a loop.  A key compared with itself counts nothing.

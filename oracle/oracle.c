@@ -115,14 +115,20 @@ static void merge_sort_idx(int32_t* idx, int n, orc_less_fn less, const void* ct
     free(tmp);
 }
 
+/* Key order: numbers ascending, -0 equal to +0, NaN after every number (and
+ * equal to another NaN). */
 static int key_less(int a, int b, const void* ctx)
 {
     const float* k = (const float*)ctx;
-    return k[a] < k[b];
+    const float x = k[a], y = k[b];
+    if (y != y) return x == x;
+    return x < y;
 }
 
 /* std::sort over pair<float,int>(key, index) (tsp_config.hpp:209-216):
- * ascending key, ties by ascending index == a stable sort of the indices. */
+ * ascending key, ties by ascending index == a stable sort of the indices.
+ * The reference leaves NaN keys undefined (std::sort needs a strict weak
+ * order); here they sort last, in index order. */
 void orc_argsort_keys(const float* keys, int n, int32_t* perm)
 {
     for (int i = 0; i < n; ++i) perm[i] = i;

@@ -49,6 +49,23 @@ def test_argsort_signed_zero_and_extremes():
     assert pyoracle.argsort_keys(k).tolist() == [4, 1, 2, 5, 0, 3]
 
 
+def test_argsort_nan_last_matches_numpy():
+    """NaN keys sort after every number, in index order (numpy's stable
+    argsort puts NaN last the same way); rows of equal keys, keys outside
+    [0, 1) and NaN mixed in."""
+    g = np.random.default_rng(11)
+    rows = [np.array([0.5, np.nan, 0.25, np.nan, -1.0, 2.0, 0.25], np.float32),
+            np.full(300, np.nan, np.float32),
+            np.full(300, 0.375, np.float32),
+            (g.integers(0, 100, 2000) + g.integers(0, 4, 2000) * 0.25).astype(np.float32)]
+    mixed = g.random(2000).astype(np.float32)
+    mixed[g.integers(0, 2000, 50)] = np.nan
+    mixed[:7] = -0.0
+    rows.append(mixed)
+    for row in rows:
+        assert np.array_equal(pyoracle.argsort_keys(row), np.argsort(row, kind="stable"))
+
+
 # -------------------------------------------------------------------- TSP --
 
 def tour_length_fp64(xy, perm):
