@@ -31,6 +31,9 @@ struct BRKGAEngineOptions {
     int ls_interval = 2;
     int ls_individuals = 100;
     int ls_moves = 5;
+    // multi-objective configs: Population::ensure_minimum_diversity after
+    // every 25th generation step, as Solver does (solver.hpp:812-814); 0 = off
+    int diversity_interval = 25;
 };
 
 template <typename T>
@@ -51,6 +54,7 @@ public:
         p.migration_interval = opt.migration_interval;
         p.num_migrants = opt.num_migrants;
         p.seed = opt.seed;
+        p.diversity_interval = opt.diversity_interval;
         if (opt.local_search && cfg->has_local_search()) {
             p.ls_interval = opt.ls_interval;
             p.ls_individuals = opt.ls_individuals;

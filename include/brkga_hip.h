@@ -94,6 +94,16 @@ typedef struct {
     int32_t ls_interval;
     int32_t ls_individuals;
     int32_t ls_moves;
+    /* NSGA-II diversity injection (Population::ensure_minimum_diversity,
+     * core/population.hpp:130-182, which Solver runs every 25 generations,
+     * core/solver.hpp:812-814): after every generation step t with
+     * t % diversity_interval == 0, members closer than 0.001 in objective
+     * space to an earlier kept member count as duplicates; when fewer than
+     * 70% are unique, the unique ones stay (in order) and the rest are
+     * replaced by random individuals, and the population is ranked and
+     * crowded again.  0 = off; 25 = the reference's schedule.  Ignored by
+     * single-objective problems, as in the reference. */
+    int32_t diversity_interval;
 } brkga_params;
 
 /* ------------------------------------------------------------ engine ---- */

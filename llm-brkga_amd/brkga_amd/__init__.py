@@ -71,6 +71,7 @@ class CParams(ctypes.Structure):
         ("ls_interval", ctypes.c_int32),
         ("ls_individuals", ctypes.c_int32),
         ("ls_moves", ctypes.c_int32),
+        ("diversity_interval", ctypes.c_int32),
     ]
 
 
@@ -90,11 +91,13 @@ class Params:
     ls_interval: int = 0      # TSPJ local search (tspj_config.hpp:77-135); 0 = off
     ls_individuals: int = 0
     ls_moves: int = 0
+    diversity_interval: int = 0  # NSGA-II ensure_minimum_diversity (population.hpp:130-182); 0 = off, 25 = reference
 
     def to_c(self) -> CParams:
         return CParams(self.population_size, self.elite_size, self.mutant_size, float(self.elite_prob),
                        self.num_islands, self.migration_interval, self.num_migrants,
-                       self.seed & 0xFFFFFFFFFFFFFFFF, self.ls_interval, self.ls_individuals, self.ls_moves)
+                       self.seed & 0xFFFFFFFFFFFFFFFF, self.ls_interval, self.ls_individuals, self.ls_moves,
+                       self.diversity_interval)
 
 
 class CServerConfig(ctypes.Structure):

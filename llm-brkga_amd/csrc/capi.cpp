@@ -194,6 +194,17 @@ int fixed_shift(double max_abs_term)
     return std::max(-60, std::min(60, 48 - e));
 }
 
+// Smallest double s whose correctly rounded square root is at least 0.001:
+// sqrt(s) < 0.001 (the reference's test, population.hpp:139-146) holds
+// exactly when s is below it, so the kernels compare squared distances.
+static double diversity_eps2()
+{
+    double s = 0.001 * 0.001;
+    while (std::sqrt(s) >= 0.001) s = std::nextafter(s, 0.0);
+    while (std::sqrt(s) < 0.001) s = std::nextafter(s, 1.0);
+    return s;
+}
+
 brkga_status fill_inst_desc(const brkga_problem_s* pr, const brkga_params* q, InstDesc* d, int* nsga2)
 {
     if (!pr || !q || !d) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
@@ -219,9 +230,14 @@ brkga_status fill_inst_desc(const brkga_problem_s* pr, const brkga_params* q, In
         if (P > kMaxNsgaPop)
             return set_error(BRKGA_ERR_INVALID_ARG, "NSGA-II population_size must be <= " +
                                                         std::to_string(kMaxNsgaPop));
+        if (q->diversity_interval < 0)
+            return set_error(BRKGA_ERR_INVALID_ARG, "diversity_interval must be >= 0");
         d->islands = 1;
         d->Pi = P;
         d->mut_rate = (float)(5.0 / (double)p.chrom_len);
+        d->div_interval = q->diversity_interval;
+        static const double eps2 = diversity_eps2();
+        d->div_eps2 = eps2;
         return BRKGA_OK;
     }
     const int I = q->num_islands < 1 ? 1 : q->num_islands;
@@ -461,6 +477,10 @@ brkga_status pop_step(brkga_pop_t pop, int init, int cur, hipStream_t s)
         e = launch_nsga2_offspring(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, init, s);
         if (e == hipSuccess) e = launch_nsga2_select(pop->d_desc, 1, init, d.pop, s);
         if (e == hipSuccess && !init) e = launch_nsga2_gather(pop->d_desc, 1, d.pop, s);
+        // the injection's kernels check on the device whether this step is
+        // due, so graphs replay at any generation
+        if (e == hipSuccess && !init && d.div_interval > 0)
+            e = launch_nsga2_diversity(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, d.pop, s);
     } else {
         const int rank = !init && pop_ranks(pop);
         e = launch_step_one(d, pop->block, cur, init, rank, s);
@@ -1193,6 +1213,9 @@ brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* ste
                 if (pop->nsga2) {
                     e = launch_nsga2_select(pop->d_desc, 1, 0, d.pop, s);
                     if (e == hipSuccess) e = launch_nsga2_gather(pop->d_desc, 1, d.pop, s);
+                    if (e == hipSuccess && d.div_interval > 0)
+                        e = launch_nsga2_diversity(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds,
+                                                   d.pop, s);
                 } else {
                     e = launch_partition_one(d, cur ^ 1, 1, s);
                     if (e == hipSuccess)

@@ -50,7 +50,8 @@ enum RngTag : uint32_t {
     TAG_MUTANT = 2,
     TAG_INIT = 3,
     TAG_MUTCOIN = 4,
-    TAG_LSEARCH = 5
+    TAG_LSEARCH = 5,
+    TAG_DIVERSITY = 6
 };
 
 struct ProblemDesc {
@@ -87,7 +88,8 @@ struct InstDesc {
     unsigned* dom;    // NSGA-II dominated-by bitsets [rows][rows / 32] (null: counted in the selection)
     int* gen;         // completed generation steps; gen[3] is the odd-parity slot of the ranking steps (gen_slot)
     int* done;        // arrival counter of the per-island partition blocks
-    int* ls_flag;     // 1 while a local search of the current generation is due
+    int* ls_flag;     // BRKGA: 1 while a local search of the current generation is due;
+                      // NSGA-II: members kept by a diversity injection in progress (0: none)
     int L, ld;
     int pop, Pi, Ei, Mi, islands;
     int mig_interval, mig_k;
@@ -99,6 +101,8 @@ struct InstDesc {
     int slot_begin;   // first block of this instance in a step launch
     int seg_begin;    // first block of this instance in a partition launch
     int gen_limit;    // steps are skipped once *gen >= gen_limit
+    int div_interval; // NSGA-II diversity injection after steps t % div_interval == 0 (0 = off)
+    double div_eps2;  // squared objective distance below which two members count as one
 };
 
 // NSGA-II populations of up to this many rows (parents + children) get

@@ -170,6 +170,7 @@ brkga_status Fleet::set_members(const std::vector<brkga_pop_s*>& pops, hipStream
             gr.block = std::max(gr.block, p->block);
             gr.lds = std::max(gr.lds, p->step_lds);
             gr.max_pop = std::max(gr.max_pop, d.pop);
+            gr.diversity = gr.diversity || d.div_interval > 0;
         }
         descs_[i] = d;
     }
@@ -420,6 +421,8 @@ brkga_status Fleet::tick(hipStream_t s)
         e = launch_nsga2_offspring(gr.kind, gr.block, d, gr.count, gr.rows, gr.lds, 0, ls);
         if (e == hipSuccess) e = launch_nsga2_select(d, gr.count, 0, gr.max_pop, ls);
         if (e == hipSuccess) e = launch_nsga2_gather(d, gr.count, gr.rows, ls);
+        if (e == hipSuccess && gr.diversity)
+            e = launch_nsga2_diversity(gr.kind, gr.block, d, gr.count, gr.rows, gr.lds, gr.max_pop, ls);
     }
     // join the BRKGA lanes before the partition, the NSGA-II lane at the end
     for (int k = 1; k < kLanes && e == hipSuccess; ++k)

@@ -52,6 +52,7 @@ struct FleetLaunch {
 
 struct FleetGroup {  // NSGA-II members of one problem kind
     int kind, first, count, rows, block, lds, max_pop;
+    bool diversity;  // some member runs the diversity injection
 };
 
 // Streams a tick spreads its independent launches over (the caller's stream

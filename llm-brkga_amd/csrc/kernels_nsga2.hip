@@ -173,17 +173,25 @@ static_assert(kDomRows <= kSelectBlock, "one dominance row per selection thread"
 // is set when row 32 w + j dominates row q.  One thread per (row, word);
 // grid (row-word chunks of kDomBlock, populations).  Populations without
 // bitsets (more than kDomRows rows) are skipped.
-__global__ __launch_bounds__(kDomBlock) void k_nsga2_dominance(const InstDesc* __restrict__ descs, int init)
+__global__ __launch_bounds__(kDomBlock) void k_nsga2_dominance(const InstDesc* __restrict__ descs, int mode)
 {
     __shared__ float2 o[kDomRows];
     const InstDesc& d = descs[blockIdx.y];
     if (!d.dom) return;
     const int N = d.pop;
-    const int t = init ? -1 : *d.gen;
-    if (t >= d.gen_limit) return;
-    const int first = t < 0;
-    const int cur = first ? 0 : (t & 1);
-    const int C = first ? N : 2 * N;
+    int cur, C;
+    if (mode == 2) {
+        // the N members of the current generation after a diversity injection
+        if (d.div_interval <= 0 || *d.ls_flag == 0) return;
+        cur = *d.gen & 1;
+        C = N;
+    } else {
+        const int t = mode ? -1 : *d.gen;
+        if (t >= d.gen_limit) return;
+        const int first = t < 0;
+        cur = first ? 0 : (t & 1);
+        C = first ? N : 2 * N;
+    }
     const int W = (C + 31) >> 5;
     if ((int)blockIdx.x * kDomBlock >= C * W) return;
     const float* fit = d.fit[cur];
@@ -203,20 +211,30 @@ __global__ __launch_bounds__(kDomBlock) void k_nsga2_dominance(const InstDesc* _
     d.dom[(size_t)q * W + w] = bits;
 }
 
-__global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* __restrict__ descs, int init)
+__global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* __restrict__ descs, int mode)
 {
     constexpr int B = kSelectBlock;
     extern __shared__ __align__(16) char lds[];
     const InstDesc& d = descs[blockIdx.x];
     const int N = d.pop;
-    const int forced = init;  // generation 0 built by the host path: the counter stays
-    const int t = init ? -1 : *d.gen;
-    if (t >= d.gen_limit) {
-        if (threadIdx.x == 0) *d.done = 0;  // nothing for the gather
-        return;
+    const int forced = mode == 1;  // generation 0 built by the host path: the counter stays
+    int t, init, cur;
+    if (mode == 2) {
+        // rank and crowd the N members of the current generation again
+        // (population.hpp:174-176); its counter and the gather flag stay
+        if (d.div_interval <= 0 || *d.ls_flag == 0) return;
+        t = *d.gen;
+        init = 1;
+        cur = t & 1;
+    } else {
+        t = mode ? -1 : *d.gen;
+        if (t >= d.gen_limit) {
+            if (threadIdx.x == 0) *d.done = 0;  // nothing for the gather
+            return;
+        }
+        init = t < 0;
+        cur = init ? 0 : (t & 1);
     }
-    init = t < 0;
-    const int cur = init ? 0 : (t & 1);
     const int C = init ? N : 2 * N;
     const int Cp = pow2_at_least(C < 2 ? 2 : C);
     const SelectLayout L = select_layout(C);
@@ -366,13 +384,17 @@ __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* _
     const int nxt = cur ^ 1;
     if (init) {
         for (int i = threadIdx.x; i < C; i += B) {
-            d.rank[0][i] = rk[i];
-            d.crowd[0][i] = cr[i];
+            d.rank[cur][i] = rk[i];
+            d.crowd[cur][i] = cr[i];
         }
-        if (!forced && threadIdx.x == 0) {
-            // counter -1: generation 0 is complete
-            *d.done = 0;
-            *d.gen = 0;
+        if (threadIdx.x == 0) {
+            if (mode == 2) {
+                *d.ls_flag = 0;  // every thread read it before the first barrier
+            } else if (!forced) {
+                // counter -1: generation 0 is complete
+                *d.done = 0;
+                *d.gen = 0;
+            }
         }
         return;
     }
@@ -433,6 +455,139 @@ __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* _
         *d.done = 1;
         *d.gen = t + 1;
     }
+}
+
+// ----------------------------------------------- diversity injection --
+
+constexpr int kDivBlock = 1024;
+
+// Whether two members count as one (population.hpp:139-146): the squared
+// Euclidean distance of their FP32 objectives, from FP64 differences, is
+// below eps2, the smallest double whose correctly rounded square root
+// reaches 0.001.
+__device__ __forceinline__ bool objectives_close(float2 a, float2 b, double eps2)
+{
+#pragma clang fp contract(off)
+    const double d0 = (double)a.x - (double)b.x;
+    const double d1 = (double)a.y - (double)b.y;
+    return d0 * d0 + d1 * d1 < eps2;
+}
+
+// Injection after the gather of step t (t % div_interval == 0), one
+// workgroup per population (Population::ensure_minimum_diversity,
+// population.hpp:130-182).  Members are taken in order; a member is kept
+// when it is not close to any member kept before it.  Candidates go 64 at a
+// time: every wave tests the 64 against its share of the members already
+// kept, then wave 0 settles the 64 among themselves in order (a candidate
+// kept here rules out the later ones close to it).  When fewer than 70% are
+// kept (10 U < 7 N, which is the reference's (double)U / N < 0.70), the
+// kept rows move, in order, to rows [0, U) of the new generation (through
+// its child rows [N, N + U), unused until the next step) and *ls_flag := U
+// for the refill and the ranking that follow.
+__global__ __launch_bounds__(kDivBlock) void k_nsga2_diversity(const InstDesc* __restrict__ descs)
+{
+    extern __shared__ __align__(16) char lds[];
+    __shared__ unsigned kill[2];
+    __shared__ int kept_count;
+    const InstDesc& d = descs[blockIdx.x];
+    if (d.div_interval <= 0) return;
+    const int t1 = *d.gen;  // the generation step t just built: t1 = t + 1
+    if (*d.done == 0 || t1 < 1 || (t1 - 1) % d.div_interval != 0) return;
+    const int N = d.pop, b = t1 & 1;
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid & (kWave - 1);
+    constexpr int W = kDivBlock / kWave;
+    float2* o = reinterpret_cast<float2*>(lds);
+    int* kept = reinterpret_cast<int*>(lds + 8 * N);
+    const float* fit = d.fit[b];
+    for (int i = tid; i < N; i += kDivBlock) o[i] = make_float2(fit[2 * i], fit[2 * i + 1]);
+    if (tid == 0) {
+        kill[0] = kill[1] = 0u;
+        kept_count = 0;
+    }
+    __syncthreads();
+    const double eps2 = d.div_eps2;
+    for (int c0 = 0; c0 < N; c0 += kWave) {
+        const int i = c0 + lane;
+        const bool valid = i < N;
+        const float2 oi = valid ? o[i] : make_float2(0.0f, 0.0f);
+        const int u = kept_count;
+        bool close = false;
+        if (valid)
+            for (int k = wid; k < u && !close; k += W) close = objectives_close(oi, o[kept[k]], eps2);
+        const unsigned long long bal = __ballot(close);
+        if (lane == 0 && bal) {
+            atomicOr(&kill[0], (unsigned)bal);
+            atomicOr(&kill[1], (unsigned)(bal >> 32));
+        }
+        __syncthreads();
+        if (wid == 0) {
+            unsigned long long km = ((unsigned long long)kill[1] << 32) | kill[0];
+            int nu = u;
+            const int lim = N - c0 < kWave ? N - c0 : kWave;
+            for (int l = 0; l < lim; ++l) {
+                if ((km >> l) & 1ull) continue;
+                if (lane == 0) kept[nu] = c0 + l;
+                ++nu;
+                const float2 ol = o[c0 + l];
+                km |= __ballot(lane > l && valid && objectives_close(oi, ol, eps2));
+            }
+            if (lane == 0) {
+                kept_count = nu;
+                kill[0] = kill[1] = 0u;
+            }
+        }
+        __syncthreads();
+    }
+    const int U = kept_count;
+    if (10 * U >= 7 * N) return;  // at least 70% unique: the population stays
+    const int nq = (d.L + 3) >> 2, ld4 = d.ld >> 2;
+    float4* rows = reinterpret_cast<float4*>(d.keys[b]);
+    for (int idx = tid; idx < U * nq; idx += kDivBlock) {
+        const int r = idx / nq, q = idx - r * nq;
+        rows[(size_t)(N + r) * ld4 + q] = rows[(size_t)kept[r] * ld4 + q];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < U * nq; idx += kDivBlock) {
+        const int r = idx / nq, q = idx - r * nq;
+        rows[(size_t)r * ld4 + q] = rows[(size_t)(N + r) * ld4 + q];
+    }
+    float* fo = d.fit[b];
+    for (int r = tid; r < U; r += kDivBlock) {
+        const float2 v = o[kept[r]];
+        fo[2 * r] = v.x;
+        fo[2 * r + 1] = v.y;
+    }
+    if (tid == 0) *d.ls_flag = U;
+}
+
+// New members of an injection: row r >= U of the current generation gets
+// fresh keys from Philox stream (0, t + 1, r, DIVERSITY) (the reference's
+// Individual::randomize, population.hpp:166-171) and is decoded.  Blocks
+// as the offspring launch; rows of populations without an injection in
+// progress are skipped.
+template <int KIND, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_nsga2_refill(const InstDesc* __restrict__ descs, int num_inst)
+{
+    extern __shared__ __align__(16) char lds[];
+    const int bidx = blockIdx.x;
+    const InstDesc& d = descs[find_inst(descs, num_inst, bidx)];
+    const int r = bidx - d.slot_begin;
+    if (r >= d.pop || d.div_interval <= 0) return;
+    const int U = *d.ls_flag;
+    if (U == 0 || r < U) return;
+    const int t1 = *d.gen, buf = t1 & 1;
+    const int L = d.L, nq = (L + 3) >> 2, nb = d.nb;
+    const DecodeLayout lay = decode_layout(KIND, d.prob.n, L, nb, d.prob.aux);
+    float4* kl = reinterpret_cast<float4*>(lds + lay.keys_off);
+    float4* out = reinterpret_cast<float4*>(d.keys[buf] + (size_t)r * d.ld);
+    for (int q = threadIdx.x; q < nq; q += BLOCK) {
+        const U4 w = rng_block(d.seed_lo, d.seed_hi, 0, (uint32_t)t1, (uint32_t)r, TAG_DIVERSITY, (uint32_t)q);
+        const float4 x = make_float4(unit_from_u32(w.x), unit_from_u32(w.y), unit_from_u32(w.z), unit_from_u32(w.w));
+        out[q] = x;
+        kl[q] = x;
+    }
+    __syncthreads();
+    decode_block<KIND, BLOCK>(d.prob, lds, lay, nb, d.fit[buf] + (size_t)r * 2, nullptr);
 }
 
 constexpr int kGatherRows = 4;  // rows per workgroup, one wave each
@@ -500,21 +655,66 @@ hipError_t launch_nsga2_offspring(int kind, int block, const InstDesc* d_descs, 
     }
 }
 
-hipError_t launch_nsga2_select(const InstDesc* d_descs, int num_inst, int init, int max_pop, hipStream_t s)
+hipError_t launch_nsga2_select(const InstDesc* d_descs, int num_inst, int mode, int max_pop, hipStream_t s)
 {
     if (num_inst <= 0) return hipSuccess;
-    const int C = init ? max_pop : 2 * max_pop;
+    const int C = mode ? max_pop : 2 * max_pop;
     const int Cd = C < kDomRows ? C : kDomRows;  // larger populations count in the selection
     const int dom_blocks = (Cd * ((Cd + 31) / 32) + kDomBlock - 1) / kDomBlock;
-    hipLaunchKernelGGL(k_nsga2_dominance, dim3(dom_blocks, num_inst), dim3(kDomBlock), 0, s, d_descs, init);
+    hipLaunchKernelGGL(k_nsga2_dominance, dim3(dom_blocks, num_inst), dim3(kDomBlock), 0, s, d_descs, mode);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int lds = select_layout(C).bytes;
     if (lds > 65536)
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_nsga2_select),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    hipLaunchKernelGGL(k_nsga2_select, dim3(num_inst), dim3(kSelectBlock), lds, s, d_descs, init);
+    hipLaunchKernelGGL(k_nsga2_select, dim3(num_inst), dim3(kSelectBlock), lds, s, d_descs, mode);
     return hipGetLastError();
+}
+
+namespace {
+
+template <int KIND, int BLOCK>
+hipError_t refill_one(const InstDesc* d, int num, int rows, int lds, hipStream_t s)
+{
+    if (lds > 65536)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_nsga2_refill<KIND, BLOCK>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    hipLaunchKernelGGL((k_nsga2_refill<KIND, BLOCK>), dim3(rows), dim3(BLOCK), lds, s, d, num);
+    return hipGetLastError();
+}
+
+template <int KIND>
+hipError_t refill_kind(int block, const InstDesc* d, int num, int rows, int lds, hipStream_t s)
+{
+    switch (block) {
+    case 64: return refill_one<KIND, 64>(d, num, rows, lds, s);
+    case 128: return refill_one<KIND, 128>(d, num, rows, lds, s);
+    case 256: return refill_one<KIND, 256>(d, num, rows, lds, s);
+    case 512: return refill_one<KIND, 512>(d, num, rows, lds, s);
+    case 1024: return refill_one<KIND, 1024>(d, num, rows, lds, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_nsga2_diversity(int kind, int block, const InstDesc* d_descs, int num_inst, int rows, int lds,
+                                  int max_pop, hipStream_t s)
+{
+    if (num_inst <= 0 || rows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nsga2_diversity, dim3(num_inst), dim3(kDivBlock), 12 * max_pop, s, d_descs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    switch (kind) {
+    case KIND_BITSP_COORDS: e = refill_kind<KIND_BITSP_COORDS>(block, d_descs, num_inst, rows, lds, s); break;
+    case KIND_BITSP_MATRIX: e = refill_kind<KIND_BITSP_MATRIX>(block, d_descs, num_inst, rows, lds, s); break;
+    case KIND_ZDT1: e = refill_kind<KIND_ZDT1>(block, d_descs, num_inst, rows, lds, s); break;
+    case KIND_MOVRP: e = refill_kind<KIND_MOVRP>(block, d_descs, num_inst, rows, lds, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    return launch_nsga2_select(d_descs, num_inst, 2, max_pop, s);
 }
 
 hipError_t launch_nsga2_gather(const InstDesc* d_descs, int num_inst, int total_rows, hipStream_t s)

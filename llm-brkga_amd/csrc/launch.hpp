@@ -78,8 +78,18 @@ int partition_lds_bytes(int Pi);
 hipError_t launch_nsga2_offspring(int kind, int block, const InstDesc* d_descs, int num_inst, int blocks,
                                   int lds, int init, hipStream_t s);
 // NSGA-II: ranking, crowding and survivor selection (one workgroup per
-// population).
-hipError_t launch_nsga2_select(const InstDesc* d_descs, int num_inst, int init, int max_pop, hipStream_t s);
+// population).  mode 0: a generation step (parents + children); 1: rank
+// generation 0 built by the host path (the counter stays); 2: rank the N
+// members of the current generation again after a diversity injection
+// (populations without one in progress are skipped).
+hipError_t launch_nsga2_select(const InstDesc* d_descs, int num_inst, int mode, int max_pop, hipStream_t s);
+// NSGA-II diversity injection after a step's gather (populations whose step
+// is not due, or with the injection off, are skipped on the device): the
+// unique members and the 70% test, the kept rows compacted, new rows drawn
+// and decoded (`rows` = child blocks of the offspring launch, `lds` its
+// LDS), then the members ranked and crowded again.
+hipError_t launch_nsga2_diversity(int kind, int block, const InstDesc* d_descs, int num_inst, int rows, int lds,
+                                  int max_pop, hipStream_t s);
 // NSGA-II: gather of survivor rows into the next buffer.
 hipError_t launch_nsga2_gather(const InstDesc* d_descs, int num_inst, int max_pop, hipStream_t s);
 

@@ -965,10 +965,73 @@ static int tournament(const int32_t* rank, const float* crowd, int a, int b)
  * solver.hpp:1058-1074) and per-gene resampling with probability 5/L
  * (solver.hpp:1077-1091); then parents + offspring are ranked, crowded and
  * truncated back to N. */
-void orc_nsga2_step(const orc_problem* p, const orc_params* q, int t,
-                    const float* keys_in, const float* obj_in,
-                    const int32_t* rank_in, const float* crowd_in,
-                    float* keys_out, float* obj_out, int32_t* rank_out, float* crowd_out)
+/* ensure_minimum_diversity, first pass (population.hpp:132-157): the members
+ * in order; a member is kept when its Euclidean distance in objective space
+ * to every member kept before it, in FP64 from the FP32 objectives with
+ * std::sqrt as the reference computes it, is not below 0.001. */
+int orc_diversity_unique(const float* obj, int count, int nobj, int32_t* kept)
+{
+    int u = 0;
+    for (int i = 0; i < count; ++i) {
+        int unique = 1;
+        for (int k = 0; k < u && unique; ++k) {
+            const float* a = obj + (size_t)i * nobj;
+            const float* b = obj + (size_t)kept[k] * nobj;
+            double dist = 0.0;
+            for (int m = 0; m < nobj; ++m) {
+                double diff = (double)a[m] - (double)b[m];
+                dist += diff * diff;
+            }
+            dist = sqrt(dist);
+            if (dist < 0.001) unique = 0;
+        }
+        if (unique) kept[u++] = i;
+    }
+    return u;
+}
+
+/* ensure_minimum_diversity (population.hpp:130-182) on the N members that
+ * NSGA-II step t selected, which the reference runs when its generation
+ * counter (t) is a multiple of 25 (solver.hpp:812-814).  When fewer than 70%
+ * are unique, the unique members stay, in order, in rows [0, U); rows
+ * [U, N) are new random individuals (keys from Philox stream (0, t + 1,
+ * row, DIVERSITY), in place of the reference's mt19937 randomize); the new
+ * rows are evaluated and all N members ranked and crowded again
+ * (population.hpp:174-176). */
+int orc_nsga2_diversity(const orc_problem* p, const orc_params* q, int t,
+                        float* keys, float* obj, int32_t* rank, float* crowd)
+{
+    int L = orc_chrom_len(p), M = orc_num_obj(p), N = q->pop;
+    int32_t* kept = (int32_t*)malloc(sizeof(int32_t) * (size_t)N);
+    int u = orc_diversity_unique(obj, N, M, kept);
+    double ratio = (double)u / (double)N;
+    int ran = 0;
+    if (ratio < 0.70) {
+        ran = 1;
+        for (int k = 0; k < u; ++k) {
+            /* kept[k] >= k: moving forward in order never overwrites a row
+             * still to be moved */
+            if (kept[k] == k) continue;
+            memmove(keys + (size_t)k * L, keys + (size_t)kept[k] * L, sizeof(float) * (size_t)L);
+            memmove(obj + (size_t)k * M, obj + (size_t)kept[k] * M, sizeof(float) * (size_t)M);
+        }
+        uint32_t g = (uint32_t)t + 1u;
+        for (int r = u; r < N; ++r)
+            for (int j = 0; j < L; ++j)
+                keys[(size_t)r * L + j] =
+                    orc_u32_to_unit(orc_philox_word(q->seed, 0, g, (uint32_t)r, ORC_TAG_DIVERSITY, (uint32_t)j));
+        orc_evaluate(p, keys + (size_t)u * L, N - u, obj + (size_t)u * M, NULL);
+        orc_nds_ranks(obj, N, M, rank);
+        orc_crowding(obj, rank, N, M, crowd);
+    }
+    free(kept);
+    return ran;
+}
+
+int orc_nsga2_step(const orc_problem* p, const orc_params* q, int t,
+                   const float* keys_in, const float* obj_in,
+                   const int32_t* rank_in, const float* crowd_in,
+                   float* keys_out, float* obj_out, int32_t* rank_out, float* crowd_out)
 {
     int L = orc_chrom_len(p), M = orc_num_obj(p), N = q->pop;
     uint32_t g = (uint32_t)t + 1u;
@@ -1010,4 +1073,7 @@ void orc_nsga2_step(const orc_problem* p, const orc_params* q, int t,
         crowd_out[i] = cc[s];
     }
     free(ck); free(co); free(cr); free(cc); free(order);
+    if (q->div_interval > 0 && t % q->div_interval == 0)
+        return orc_nsga2_diversity(p, q, t, keys_out, obj_out, rank_out, crowd_out);
+    return 0;
 }

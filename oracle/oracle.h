@@ -37,7 +37,8 @@ enum {
     ORC_TAG_MUTANT = 2,
     ORC_TAG_INIT = 3,
     ORC_TAG_MUTCOIN = 4,
-    ORC_TAG_LSEARCH = 5
+    ORC_TAG_LSEARCH = 5,
+    ORC_TAG_DIVERSITY = 6
 };
 
 enum {
@@ -83,6 +84,7 @@ typedef struct {
     int ls_interval;  /* TSPJ local search every ls_interval generations (0 = off) */
     int ls_count;     /* individuals per island (top of the rank order), capped at the elite */
     int ls_moves;     /* random 2-opt moves per individual */
+    int div_interval; /* NSGA-II diversity injection after steps t % div_interval == 0 (0 = off) */
 } orc_params;
 
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
@@ -128,10 +130,18 @@ void orc_nsga2_select(const float* obj, const int32_t* rank, const float* crowd,
                       int count, int nobj, int keep, int32_t* order);
 void orc_nsga2_init(const orc_problem* p, const orc_params* q,
                     float* keys, float* obj, int32_t* rank, float* crowd);
-void orc_nsga2_step(const orc_problem* p, const orc_params* q, int t,
-                    const float* keys_in, const float* obj_in,
-                    const int32_t* rank_in, const float* crowd_in,
-                    float* keys_out, float* obj_out, int32_t* rank_out, float* crowd_out);
+/* Returns 1 when the step ended with a diversity injection. */
+int orc_nsga2_step(const orc_problem* p, const orc_params* q, int t,
+                   const float* keys_in, const float* obj_in,
+                   const int32_t* rank_in, const float* crowd_in,
+                   float* keys_out, float* obj_out, int32_t* rank_out, float* crowd_out);
+/* First pass of ensure_minimum_diversity: indices of the members kept as
+ * unique, in order; returns their count. */
+int orc_diversity_unique(const float* obj, int count, int nobj, int32_t* kept);
+/* The whole injection on one generation of N members (in place); returns 1
+ * when it replaced members. */
+int orc_nsga2_diversity(const orc_problem* p, const orc_params* q, int t,
+                        float* keys, float* obj, int32_t* rank, float* crowd);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,7 @@ class OrcParams(ctypes.Structure):
         ("ls_interval", ctypes.c_int),
         ("ls_count", ctypes.c_int),
         ("ls_moves", ctypes.c_int),
+        ("div_interval", ctypes.c_int),
     ]
 
 
@@ -103,6 +104,12 @@ def lib():
         L.orc_crowding.argtypes = [_PF, _PI, ctypes.c_int, ctypes.c_int, _PF]
         L.orc_nsga2_select.argtypes = [_PF, _PI, _PF, ctypes.c_int, ctypes.c_int, ctypes.c_int, _PI]
         L.orc_nsga2_init.argtypes = [ctypes.POINTER(OrcProblem), ctypes.POINTER(OrcParams), _PF, _PF, _PI, _PF]
+        L.orc_diversity_unique.argtypes = [_PF, ctypes.c_int, ctypes.c_int, _PI]
+        L.orc_diversity_unique.restype = ctypes.c_int
+        L.orc_nsga2_diversity.argtypes = [ctypes.POINTER(OrcProblem), ctypes.POINTER(OrcParams), ctypes.c_int,
+                                          _PF, _PF, _PI, _PF]
+        L.orc_nsga2_diversity.restype = ctypes.c_int
+        L.orc_nsga2_step.restype = ctypes.c_int
         L.orc_nsga2_step.argtypes = [ctypes.POINTER(OrcProblem), ctypes.POINTER(OrcParams), ctypes.c_int,
                                      _PF, _PF, _PI, _PF, _PF, _PF, _PI, _PF]
         _lib = L
@@ -248,13 +255,22 @@ class Problem:
 
 
 def params(pop, elite, mutants, rho, islands=1, mig_interval=0, mig_count=0, seed=0, maximize=0,
-           ls_interval=0, ls_count=0, ls_moves=0):
+           ls_interval=0, ls_count=0, ls_moves=0, div_interval=0):
     q = OrcParams()
     q.pop, q.elite, q.mutants, q.rho = pop, elite, mutants, rho
     q.islands, q.mig_interval, q.mig_count = islands, mig_interval, mig_count
     q.seed, q.maximize = seed & 0xFFFFFFFFFFFFFFFF, maximize
     q.ls_interval, q.ls_count, q.ls_moves = ls_interval, ls_count, ls_moves
+    q.div_interval = div_interval
     return q
+
+
+def diversity_unique(obj):
+    """Indices of the members ensure_minimum_diversity keeps as unique."""
+    obj = np.ascontiguousarray(obj, np.float32)
+    kept = np.empty(obj.shape[0], np.int32)
+    u = lib().orc_diversity_unique(_f(obj), obj.shape[0], obj.shape[1], _i(kept))
+    return kept[:u].copy()
 
 
 def init_keys(seed, stream, slots, length):
@@ -333,6 +349,8 @@ class Nsga2State:
         self.rank = np.empty(N, np.int32)
         self.crowd = np.empty(N, np.float32)
         self.t = 0
+        self.injected = 0      # 1 when the last step ended with a diversity injection
+        self.injections = 0
         lib().orc_nsga2_init(ctypes.byref(problem.c), ctypes.byref(q), _f(self.keys), _f(self.obj),
                              _i(self.rank), _f(self.crowd))
 
@@ -342,8 +360,10 @@ class Nsga2State:
         o2 = np.empty((N, 2), np.float32)
         r2 = np.empty(N, np.int32)
         c2 = np.empty(N, np.float32)
-        lib().orc_nsga2_step(ctypes.byref(self.p.c), ctypes.byref(self.q), self.t, _f(self.keys), _f(self.obj),
-                             _i(self.rank), _f(self.crowd), _f(k2), _f(o2), _i(r2), _f(c2))
+        self.injected = lib().orc_nsga2_step(ctypes.byref(self.p.c), ctypes.byref(self.q), self.t, _f(self.keys),
+                                             _f(self.obj), _i(self.rank), _f(self.crowd), _f(k2), _f(o2), _i(r2),
+                                             _f(c2))
+        self.injections += self.injected
         self.keys, self.obj, self.rank, self.crowd = k2, o2, r2, c2
         self.t += 1
 
