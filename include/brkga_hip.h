@@ -63,7 +63,8 @@ typedef enum {
     BRKGA_JOB_SCHEDULING = 8,   /* configs/job_scheduling_gpu_config.hpp:28-52, 138-158 */
     BRKGA_ZDT1 = 9,             /* examples/zdt1.cu:14-63 (two objectives) */
     BRKGA_VRP_DAYS = 10,        /* configs/vrp_blazer_config.hpp:52-296 */
-    BRKGA_MOVRP = 11            /* configs/multi_objective_vrp_config.hpp:57-300 (two objectives) */
+    BRKGA_MOVRP = 11,           /* configs/multi_objective_vrp_config.hpp:57-300 (two objectives) */
+    BRKGA_HOST = 12             /* no device decoder: evaluated by the caller (brkga_problem_create_host) */
 } brkga_problem_kind;
 
 /* Continuous test functions (BRKGA_CONTINUOUS). */
@@ -202,6 +203,18 @@ brkga_status brkga_problem_create_vrp_days(brkga_engine_t engine, const float* t
 brkga_status brkga_problem_create_movrp(brkga_engine_t engine, const float* xy, const int32_t* demands, int32_t dim,
                                         int64_t capacity, int32_t vehicles, brkga_problem_t* out);
 
+/* A problem without a device decoder, evaluated by the caller on the host:
+ * the reference's BRKGAConfig with a host fitness_function /
+ * objective_functions (core/config.hpp:33-39), which it evaluates with
+ * Population::evaluate_all_cpu (core/population.hpp:626-653).  Its
+ * populations run through brkga_population_host_begin / _end (one call pair
+ * per generation); keys are built and partitioned (BRKGA) or ranked and
+ * selected (NSGA-II, num_obj = 2) on the device.  num_obj 1 or 2; maximize
+ * for single-objective problems only.  brkga_evaluate / brkga_decode /
+ * brkga_evolve and the batched and served forms refuse such problems. */
+brkga_status brkga_problem_create_host(brkga_engine_t engine, int32_t chrom_len, int32_t num_obj, int32_t maximize,
+                                       brkga_problem_t* out);
+
 /* Kind, chromosome length, objectives and direction of a problem. */
 brkga_status brkga_problem_info(brkga_problem_t problem, int32_t* kind, int32_t* chrom_len, int32_t* num_obj,
                                 int32_t* maximize);
@@ -268,6 +281,22 @@ brkga_status brkga_evolve(brkga_pop_t pop, int32_t generations, void* stream);
 /* Same steps replayed from a captured HIP graph of `generations` steps
  * (the graph is built on first use and cached per length). */
 brkga_status brkga_evolve_graph(brkga_pop_t pop, int32_t generations, void* stream);
+
+/* One generation of a host-evaluated population, in two calls.
+ * _begin builds generation 0 (initialize != 0) or the next generation's
+ * keys on the device and copies to the host: keys[rows][chrom_len];
+ * fitness[rows][num_obj] (may be NULL) holds the fitness of rows that are
+ * copies (BRKGA elite and migrant slots, as evaluate_all_cpu skips
+ * evaluated members), 0 elsewhere; needs[rows] (may be NULL) is 1 for every
+ * row the caller must evaluate.  rows = population_size: the population
+ * (BRKGA, generation 0) or the N children (NSGA-II steps).
+ * _end takes fitness[rows][num_obj] for every row (copied rows as _begin
+ * gave them) and completes the generation: partition by fitness (BRKGA) or
+ * NSGA-II ranking, crowding and survivor selection.  Both synchronise
+ * `stream`. */
+brkga_status brkga_population_host_begin(brkga_pop_t pop, int32_t initialize, float* keys, float* fitness,
+                                         int32_t* needs, void* stream);
+brkga_status brkga_population_host_end(brkga_pop_t pop, const float* fitness, void* stream);
 
 /* Completed generation steps since initialize. */
 brkga_status brkga_population_generation(brkga_pop_t pop, int32_t* generation);

@@ -46,6 +46,7 @@ JOB_SCHEDULING = 8
 ZDT1 = 9
 VRP_DAYS = 10
 MOVRP = 11
+HOST = 12
 
 SPHERE = 0
 RASTRIGIN = 1
@@ -143,6 +144,7 @@ _SIGNATURES = {
     "brkga_problem_create_zdt1": ([_P, _I32, ctypes.POINTER(_P)], ctypes.c_int),
     "brkga_problem_create_vrp_days": ([_P, _PF, _I32, _I32, ctypes.c_double, ctypes.POINTER(_P)], ctypes.c_int),
     "brkga_problem_create_movrp": ([_P, _PF, _PI, _I32, ctypes.c_int64, _I32, ctypes.POINTER(_P)], ctypes.c_int),
+    "brkga_problem_create_host": ([_P, _I32, _I32, _I32, ctypes.POINTER(_P)], ctypes.c_int),
     "brkga_problem_info": ([_P, _PI, _PI, _PI, _PI], ctypes.c_int),
     "brkga_problem_destroy": ([_P], ctypes.c_int),
     "brkga_evaluate": ([_P, _P, _P, _I32, _I32, _P], ctypes.c_int),
@@ -154,6 +156,8 @@ _SIGNATURES = {
     "brkga_population_initialize": ([_P, _P], ctypes.c_int),
     "brkga_evolve": ([_P, _I32, _P], ctypes.c_int),
     "brkga_evolve_graph": ([_P, _I32, _P], ctypes.c_int),
+    "brkga_population_host_begin": ([_P, _I32, _PF, _PF, _PI, _P], ctypes.c_int),
+    "brkga_population_host_end": ([_P, _PF, _P], ctypes.c_int),
     "brkga_population_generation": ([_P, _PI], ctypes.c_int),
     "brkga_population_read": ([_P, _PF, _PF, _PI, _PI, _PF, _P], ctypes.c_int),
     "brkga_population_write": ([_P, _PF, _P], ctypes.c_int),
@@ -328,6 +332,13 @@ class Problem:
         return cls._make(engine, lib().brkga_problem_create_movrp, _fptr(xy), _iptr(d), xy.shape[0], int(capacity),
                          int(vehicles))
 
+    @classmethod
+    def host(cls, engine, chrom_len, num_obj=1, maximize=False):
+        """A problem the caller evaluates on the host (a config with a host
+        fitness_function); its populations step with host_generation()."""
+        return cls._make(engine, lib().brkga_problem_create_host, int(chrom_len), int(num_obj),
+                         1 if maximize else 0)
+
     def evaluate_device(self, d_keys: int, d_fitness: int, pop_size: int, stream: int = 0):
         """evaluate_population_gpu on caller-owned device pointers (ints)."""
         _check(lib().brkga_evaluate(self.handle, d_keys, d_fitness, pop_size, self.chrom_len, stream or None))
@@ -414,9 +425,37 @@ class Population:
         M, L = self.problem.num_obj, self.problem.chrom_len
         fit = np.empty(M, np.float32)
         keys = np.empty(L, np.float32)
-        dec = np.empty(L, np.int32)
-        _check(lib().brkga_get_best(self.handle, _fptr(fit), _fptr(keys), _iptr(dec), stream or None))
+        host = self.problem.kind == HOST  # no device decoder: no decoded solution
+        dec = None if host else np.empty(L, np.int32)
+        _check(lib().brkga_get_best(self.handle, _fptr(fit), _fptr(keys), None if host else _iptr(dec),
+                                    stream or None))
         return (fit if M > 1 else float(fit[0])), keys, dec
+
+    def host_begin(self, initialize: bool = False, stream: int = 0):
+        """Keys [rows][L] of the next generation (or generation 0), the fitness
+        of copied rows and the mask of rows the caller must evaluate."""
+        L, M = self.problem.chrom_len, self.problem.num_obj
+        rows = self.size
+        keys = np.empty((rows, L), np.float32)
+        fit = np.empty((rows, M), np.float32)
+        needs = np.empty(rows, np.int32)
+        _check(lib().brkga_population_host_begin(self.handle, 1 if initialize else 0, _fptr(keys), _fptr(fit),
+                                                 _iptr(needs), stream or None))
+        return keys, fit, needs.astype(bool)
+
+    def host_end(self, fitness, stream: int = 0):
+        M = self.problem.num_obj
+        f = np.ascontiguousarray(fitness, dtype=np.float32).reshape(self.size, M)
+        _check(lib().brkga_population_host_end(self.handle, _fptr(f), stream or None))
+
+    def host_generation(self, evaluate, initialize: bool = False):
+        """One host-evaluated generation: evaluate(keys) -> fitness rows for
+        the rows that need it."""
+        keys, fit, needs = self.host_begin(initialize)
+        idx = np.nonzero(needs)[0]
+        if idx.size:
+            fit[idx] = np.asarray(evaluate(keys[idx]), np.float32).reshape(idx.size, -1)
+        self.host_end(fit)
 
     def set_block(self, block: int):
         """Workgroup size of the generation kernel (results never depend on it)."""

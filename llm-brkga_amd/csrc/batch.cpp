@@ -44,6 +44,11 @@ brkga_status brkga_batch_create(brkga_engine_t engine, int32_t count, const brkg
             st = set_error(BRKGA_ERR_INVALID_ARG, "batch member " + std::to_string(i) + ": problem of another engine");
             break;
         }
+        if (problems[i]->desc.kind == KIND_HOST) {
+            st = set_error(BRKGA_ERR_INVALID_ARG, "batch member " + std::to_string(i) +
+                                                      ": host-evaluated problems run through brkga_population_host_*");
+            break;
+        }
         brkga_pop_s* p = nullptr;
         st = create_population(problems[i], &params[i], nullptr, nullptr, 0, &p);
         if (st == BRKGA_OK) b->pops.push_back(p);

@@ -232,6 +232,9 @@ brkga_status fill_inst_desc(const brkga_problem_s* pr, const brkga_params* q, In
                                                         std::to_string(kMaxNsgaPop));
         if (q->diversity_interval < 0)
             return set_error(BRKGA_ERR_INVALID_ARG, "diversity_interval must be >= 0");
+        if (q->diversity_interval > 0 && p.kind == KIND_HOST)
+            return set_error(BRKGA_ERR_INVALID_ARG, "the diversity injection decodes its new members on the device; "
+                                                    "host-evaluated problems run without it");
         d->islands = 1;
         d->Pi = P;
         d->mut_rate = (float)(5.0 / (double)p.chrom_len);
@@ -456,6 +459,22 @@ namespace {
 hipStream_t as_stream(void* s)
 {
     return reinterpret_cast<hipStream_t>(s);
+}
+
+const char* const kNoDecoder =
+    "the problem is evaluated on the host (no device decoder): use brkga_population_host_begin / _end";
+
+bool host_kind(const brkga_pop_s* pop)
+{
+    return pop->problem->desc.kind == KIND_HOST;
+}
+
+// Device-decoded generations of a host-evaluated population are refused.
+brkga_status device_path_check(const brkga_pop_s* pop)
+{
+    if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
+    if (host_kind(pop)) return set_error(BRKGA_ERR_INVALID_ARG, kNoDecoder);
+    return BRKGA_OK;
 }
 
 // Whether the population's steps rank their parent islands themselves: the
@@ -787,6 +806,23 @@ brkga_status brkga_problem_create_movrp(brkga_engine_t engine, const float* xy, 
     return finish_problem(pr, st, out);
 }
 
+brkga_status brkga_problem_create_host(brkga_engine_t engine, int32_t chrom_len, int32_t num_obj, int32_t maximize,
+                                       brkga_problem_t* out)
+{
+    if (num_obj != 1 && num_obj != 2)
+        return set_error(BRKGA_ERR_INVALID_ARG, "host-evaluated problems have 1 or 2 objectives");
+    if (num_obj == 2 && maximize)
+        return set_error(BRKGA_ERR_INVALID_ARG, "NSGA-II minimises every objective (individual.hpp:99-114)");
+    brkga_problem_s* pr;
+    brkga_status st = new_problem(engine, KIND_HOST, chrom_len, &pr);
+    if (st != BRKGA_OK) return st;
+    pr->desc.num_obj = num_obj;
+    pr->desc.maximize = maximize ? 1 : 0;
+    pr->desc.dim = chrom_len;
+    set_scales(&pr->desc, 30, 30);
+    return finish_problem(pr, st, out);
+}
+
 brkga_status brkga_problem_info(brkga_problem_t problem, int32_t* kind, int32_t* chrom_len, int32_t* num_obj,
                                 int32_t* maximize)
 {
@@ -811,6 +847,7 @@ static brkga_status evaluate_impl(brkga_problem_t problem, const float* d_keys, 
                                   int32_t pop_size, int32_t chrom_len, void* stream)
 {
     if (!problem || !d_keys) return set_error(BRKGA_ERR_INVALID_ARG, "null problem or keys");
+    if (problem->desc.kind == KIND_HOST) return set_error(BRKGA_ERR_INVALID_ARG, kNoDecoder);
     if (pop_size < 0) return set_error(BRKGA_ERR_INVALID_ARG, "negative pop_size");
     if (chrom_len != problem->desc.chrom_len)
         return set_error(BRKGA_ERR_INVALID_ARG, "chrom_len " + std::to_string(chrom_len) + " does not match the problem (" +
@@ -841,6 +878,7 @@ brkga_status brkga_evaluate_host(brkga_problem_t problem, const float* keys, int
                                  int32_t* decoded)
 {
     if (!problem || !keys || count < 0) return set_error(BRKGA_ERR_INVALID_ARG, "null problem / keys or negative count");
+    if (problem->desc.kind == KIND_HOST) return set_error(BRKGA_ERR_INVALID_ARG, kNoDecoder);
     if (count == 0) return BRKGA_OK;
     DeviceGuard g(problem->engine->device);
     const ProblemDesc& p = problem->desc;
@@ -867,6 +905,7 @@ brkga_status brkga_evaluate_host_noalloc(brkga_problem_t problem, const float* k
                                          int32_t* decoded)
 {
     if (!problem || !keys || count < 0) return set_error(BRKGA_ERR_INVALID_ARG, "null problem / keys or negative count");
+    if (problem->desc.kind == KIND_HOST) return set_error(BRKGA_ERR_INVALID_ARG, kNoDecoder);
     if (count == 0) return BRKGA_OK;
     const ProblemDesc& p = problem->desc;
     const int dev = problem->engine->device;
@@ -927,6 +966,7 @@ brkga_status brkga_population_destroy(brkga_pop_t pop)
 
 brkga_status brkga_population_initialize(brkga_pop_t pop, void* stream)
 {
+    if (brkga_status hs = device_path_check(pop)) return hs;
     if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
     DeviceGuard g(pop->problem->engine->device);
     hipStream_t s = as_stream(stream);
@@ -937,6 +977,7 @@ brkga_status brkga_population_initialize(brkga_pop_t pop, void* stream)
 
 brkga_status brkga_evolve(brkga_pop_t pop, int32_t generations, void* stream)
 {
+    if (brkga_status hs = device_path_check(pop)) return hs;
     if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
     if (generations < 0) return set_error(BRKGA_ERR_INVALID_ARG, "negative generation count");
     DeviceGuard g(pop->problem->engine->device);
@@ -951,6 +992,7 @@ brkga_status brkga_evolve(brkga_pop_t pop, int32_t generations, void* stream)
 
 brkga_status brkga_evolve_graph(brkga_pop_t pop, int32_t generations, void* stream)
 {
+    if (brkga_status hs = device_path_check(pop)) return hs;
     if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
     if (generations < 0) return set_error(BRKGA_ERR_INVALID_ARG, "negative generation count");
     if (generations == 0) return BRKGA_OK;
@@ -1022,6 +1064,7 @@ brkga_status brkga_population_read(brkga_pop_t pop, float* keys, float* fitness,
 
 brkga_status brkga_population_write(brkga_pop_t pop, const float* keys, void* stream)
 {
+    if (brkga_status hs = device_path_check(pop)) return hs;
     if (!pop || !keys) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
     DeviceGuard g(pop->problem->engine->device);
     hipStream_t s = as_stream(stream);
@@ -1066,6 +1109,7 @@ brkga_status brkga_get_best(brkga_pop_t pop, float* fitness, float* keys, int32_
         for (int m = 0; m < M; ++m) fitness[m] = fit[(size_t)best * M + m];
     const float* row = d.keys[b] + (size_t)best * d.ld;
     if (keys) BRKGA_HIP_TRY(hipMemcpy(keys, row, sizeof(float) * d.L, hipMemcpyDeviceToHost));
+    if (decoded && host_kind(pop)) return set_error(BRKGA_ERR_INVALID_ARG, kNoDecoder);
     if (decoded) {
         hipError_t e = launch_evaluate(d.prob, row, d.ld, 1, pop->d_fit_row, pop->d_decode_row, pop->block, s);
         if (e != hipSuccess) return hip_error(e, "decode launch");
@@ -1093,6 +1137,7 @@ brkga_status brkga_population_device_buffers(brkga_pop_t pop, float** d_keys, fl
 
 brkga_status brkga_population_autotune(brkga_pop_t pop, int32_t reps, int32_t* block, void* stream)
 {
+    if (pop && host_kind(pop)) return set_error(BRKGA_ERR_INVALID_ARG, kNoDecoder);
     if (!pop || reps <= 0) return set_error(BRKGA_ERR_INVALID_ARG, "null population or no repetitions");
     std::lock_guard<std::mutex> lock(pop->mu);
     DeviceGuard g(pop->problem->engine->device);
@@ -1173,9 +1218,99 @@ brkga_status brkga_population_set_block(brkga_pop_t pop, int32_t block)
     return BRKGA_OK;
 }
 
+brkga_status brkga_population_host_begin(brkga_pop_t pop, int32_t initialize, float* keys, float* fitness,
+                                         int32_t* needs, void* stream)
+{
+    if (!pop || !keys) return set_error(BRKGA_ERR_INVALID_ARG, "null population or keys");
+    if (!host_kind(pop)) return set_error(BRKGA_ERR_INVALID_ARG, "the problem has a device decoder: use brkga_evolve");
+    if (pop->host_open) return set_error(BRKGA_ERR_INVALID_ARG, "a host-evaluated generation is already open");
+    DeviceGuard g(pop->problem->engine->device);
+    hipStream_t s = as_stream(stream);
+    const InstDesc& d = pop->host;
+    const int M = d.prob.num_obj;
+    if (initialize) {
+        BRKGA_HIP_TRY(hipMemsetAsync(d.gen, 0, 4 * sizeof(int), s));
+        pop->host_t = 0;
+    }
+    const int cur = pop->host_t & 1;
+    hipError_t e;
+    const float* rows;
+    const float* fit;
+    if (pop->nsga2) {
+        // generation 0, or the N children of step host_t (rows [N, 2N))
+        e = launch_nsga2_offspring(KIND_HOST, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, initialize, s);
+        const int buf = initialize ? 0 : cur;
+        const size_t first = initialize ? 0 : (size_t)d.pop;
+        rows = d.keys[buf] + first * d.ld;
+        fit = d.fit[buf] + first * M;
+    } else {
+        e = launch_host_children(d, cur, initialize, s);
+        const int nxt = initialize ? 0 : cur ^ 1;
+        rows = d.keys[nxt];
+        fit = d.fit[nxt];
+    }
+    if (e != hipSuccess) return hip_error(e, "kernel launch");
+    BRKGA_HIP_TRY(hipStreamSynchronize(s));
+    BRKGA_HIP_TRY(hipMemcpy2D(keys, sizeof(float) * d.L, rows, sizeof(float) * d.ld, sizeof(float) * d.L,
+                              (size_t)d.pop, hipMemcpyDeviceToHost));
+    // rows the step copied (elite, migrants) keep their fitness; the caller
+    // evaluates the others (population.hpp:626-653 skips evaluated members)
+    const int t = pop->host_t;
+    const bool migrates = !pop->nsga2 && !initialize && d.islands > 1 && d.mig_k > 0 && d.mig_interval > 0 &&
+                          t > 0 && t % d.mig_interval == 0;
+    std::vector<int32_t> need((size_t)d.pop, 1);
+    if (!pop->nsga2 && !initialize)
+        for (int i = 0; i < d.pop; ++i) {
+            const int sl = i % d.Pi;
+            if (sl < d.Ei || (migrates && sl >= d.Pi - d.mig_k)) need[i] = 0;
+        }
+    if (fitness) {
+        if (pop->nsga2 || initialize) {
+            std::fill(fitness, fitness + (size_t)d.pop * M, 0.0f);
+        } else {
+            BRKGA_HIP_TRY(hipMemcpy(fitness, fit, sizeof(float) * (size_t)d.pop * M, hipMemcpyDeviceToHost));
+            for (int i = 0; i < d.pop; ++i)
+                if (need[i])
+                    for (int m = 0; m < M; ++m) fitness[(size_t)i * M + m] = 0.0f;
+        }
+    }
+    if (needs) std::copy(need.begin(), need.end(), needs);
+    pop->host_open = initialize ? 1 : 2;
+    return BRKGA_OK;
+}
+
+brkga_status brkga_population_host_end(brkga_pop_t pop, const float* fitness, void* stream)
+{
+    if (!pop || !fitness) return set_error(BRKGA_ERR_INVALID_ARG, "null population or fitness");
+    if (!pop->host_open) return set_error(BRKGA_ERR_INVALID_ARG, "no host-evaluated generation is open");
+    DeviceGuard g(pop->problem->engine->device);
+    hipStream_t s = as_stream(stream);
+    const InstDesc& d = pop->host;
+    const int M = d.prob.num_obj;
+    const int init = pop->host_open == 1;
+    const int cur = pop->host_t & 1;
+    float* dst;
+    if (pop->nsga2) dst = d.fit[init ? 0 : cur] + (init ? 0 : (size_t)d.pop * M);
+    else dst = d.fit[init ? 0 : cur ^ 1];
+    BRKGA_HIP_TRY(hipMemcpyAsync(dst, fitness, sizeof(float) * (size_t)d.pop * M, hipMemcpyHostToDevice, s));
+    hipError_t e;
+    if (pop->nsga2) {
+        e = launch_nsga2_select(pop->d_desc, 1, init, d.pop, s);
+        if (e == hipSuccess && !init) e = launch_nsga2_gather(pop->d_desc, 1, d.pop, s);
+    } else {
+        e = launch_partition_one(d, init ? 0 : cur ^ 1, init ? 0 : 1, s);
+    }
+    if (e != hipSuccess) return hip_error(e, "kernel launch");
+    BRKGA_HIP_TRY(hipStreamSynchronize(s));
+    if (!init) ++pop->host_t;
+    pop->host_open = 0;
+    return BRKGA_OK;
+}
+
 brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* step_us, double* close_us,
                                       void* stream)
 {
+    if (pop && host_kind(pop)) return set_error(BRKGA_ERR_INVALID_ARG, kNoDecoder);
     if (!pop || reps <= 0) return set_error(BRKGA_ERR_INVALID_ARG, "null population or no repetitions");
     DeviceGuard g(pop->problem->engine->device);
     hipStream_t s = as_stream(stream);

@@ -91,6 +91,7 @@ struct brkga_pop_s {
     int host_t;               // generation steps enqueued since initialize
     int* d_decode_row;        // scratch for get_best decode
     float* d_fit_row;
+    int host_open = 0;        // host-evaluated generation begun and not yet ended: 1 initial, 2 step
     hipStream_t capture_stream;  // created on the first graph capture
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;  // (steps, start parity)
     std::mutex mu;

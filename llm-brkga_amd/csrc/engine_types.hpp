@@ -21,13 +21,16 @@ enum ProblemKind : int {
     KIND_ZDT1 = 9,            // examples/zdt1.cu:43-63 (two objectives)
     KIND_VRP_DAYS = 10,       // configs/vrp_blazer_config.hpp:192-296
     KIND_MOVRP = 11,          // configs/multi_objective_vrp_config.hpp:208-300 (two objectives)
-    KIND_COUNT = 12
+    KIND_HOST = 12,           // no device decoder: the caller evaluates on the host
+                              // (BRKGAConfig::fitness_function, population.hpp:626-653)
+    KIND_COUNT = 13
 };
 
 // Kinds whose decoder argsorts keys (and so has per-thread key registers).
 __host__ __device__ constexpr bool kind_sorts(int kind)
 {
-    return !(kind == KIND_KNAP_THRESHOLD || kind == KIND_CONTINUOUS || kind == KIND_JOBSHOP || kind == KIND_ZDT1);
+    return !(kind == KIND_KNAP_THRESHOLD || kind == KIND_CONTINUOUS || kind == KIND_JOBSHOP || kind == KIND_ZDT1 ||
+             kind == KIND_HOST);
 }
 
 // Continuous test functions (KIND_CONTINUOUS, ProblemDesc::ip0).
@@ -139,6 +142,12 @@ __host__ __device__ inline int coords_bytes(int kind, int n)
 __host__ __device__ inline DecodeLayout decode_layout(int kind, int n, int L, int nb, int aux)
 {
     DecodeLayout d;
+    if (kind == KIND_HOST) {
+        // no decode: the keys of the row being built, nothing else
+        d.red_off = d.items_off = d.keys_off = d.perm_off = d.cnt_off = d.aux_off = 0;
+        d.bytes = round_up_int(L, 32) * 4;
+        return d;
+    }
     int off = 0;
     d.red_off = off;
     off += 17 * 8;

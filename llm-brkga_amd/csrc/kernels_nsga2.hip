@@ -136,8 +136,12 @@ __global__ __launch_bounds__(BLOCK) void k_nsga2_offspring(const InstDesc* __res
             kl[q] = x;
         }
     }
-    __syncthreads();
-    decode_block<KIND, BLOCK>(d.prob, lds, lay, nb, d.fit[buf] + (size_t)row * 2, nullptr);
+    if constexpr (KIND == KIND_HOST) {
+        return;  // the caller evaluates the row on the host
+    } else {
+        __syncthreads();
+        decode_block<KIND, BLOCK>(d.prob, lds, lay, nb, d.fit[buf] + (size_t)row * 2, nullptr);
+    }
 }
 
 // LDS layout of the selection block for C = up to 2N individuals.
@@ -651,6 +655,7 @@ hipError_t launch_nsga2_offspring(int kind, int block, const InstDesc* d_descs, 
     case KIND_BITSP_MATRIX: return off_kind<KIND_BITSP_MATRIX>(block, d_descs, num_inst, blocks, lds, init, s);
     case KIND_ZDT1: return off_kind<KIND_ZDT1>(block, d_descs, num_inst, blocks, lds, init, s);
     case KIND_MOVRP: return off_kind<KIND_MOVRP>(block, d_descs, num_inst, blocks, lds, init, s);
+    case KIND_HOST: return off_kind<KIND_HOST>(block, d_descs, num_inst, blocks, lds, init, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -71,6 +71,12 @@ hipError_t launch_lane_probe(int iters, float* out, hipStream_t s);
 // instances.
 hipError_t launch_set_counters(const InstDesc* d_descs, int num, int gen, hipStream_t s);
 
+// Keys of the next generation of a host-evaluated BRKGA population (one
+// workgroup per slot, no decode): elite and migrant rows are copied with
+// their fitness, the others are left for the caller to evaluate.  The
+// partition that follows closes the generation.
+hipError_t launch_host_children(const InstDesc& d, int cur, int init, hipStream_t s);
+
 int step_lds_bytes(const InstDesc& d);
 int partition_lds_bytes(int Pi);
 

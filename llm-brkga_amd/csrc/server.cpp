@@ -426,6 +426,8 @@ brkga_status brkga_server_submit(brkga_server_t server, brkga_problem_t problem,
         return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
     if (problem->engine != server->engine) return set_error(BRKGA_ERR_INVALID_ARG, "problem of another engine");
     if (generations < 0) return set_error(BRKGA_ERR_INVALID_ARG, "negative generation count");
+    if (problem->desc.kind == KIND_HOST)
+        return set_error(BRKGA_ERR_INVALID_ARG, "host-evaluated problems run through brkga_population_host_*");
     std::unique_ptr<Request> r(new Request());
     r->problem = problem;
     r->own_problem = own_problem != 0;
