@@ -40,6 +40,12 @@ public:
         options_.num_migrants = migrants;
     }
     void set_use_graph(bool on) { options_.use_graph = on; }
+    // multi-objective configs: diversity injection every `interval` steps
+    // (solver.hpp:812-814 uses 25; 0 = off)
+    void set_diversity_interval(int interval) { options_.diversity_interval = interval; }
+    // configs evaluated on the host: threads for the callbacks (0 = one per
+    // hardware thread, 1 = serial)
+    void set_host_threads(int threads) { options_.host_threads = threads; }
     // local search of configs that have one (solver.hpp:74-77,166)
     void set_local_search(bool on, int interval = 2, int individuals = 100, int moves = 5)
     {
@@ -67,7 +73,11 @@ public:
     void evolve_generation()
     {
         if (!population_) initialize();
+        const auto t0 = std::chrono::steady_clock::now();
         population_->evolve(1);
+        population_->generation();  // waits for the step
+        step_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        ++steps_timed_;
         ++current_generation_;
         population_->record_fitness();
     }
@@ -83,7 +93,11 @@ public:
         int gen = 0;
         while (gen < G) {
             const int chunk = std::min(print_frequency_, G - gen);
+            const auto c0 = std::chrono::steady_clock::now();
             population_->evolve(chunk);
+            population_->generation();  // waits for the chunk
+            step_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count();
+            steps_timed_ += chunk;
             gen += chunk;
             current_generation_ = gen;
             population_->record_fitness();
@@ -101,6 +115,54 @@ public:
             std::cout << "  Pareto front size: " << population_->get_pareto_front().size() << std::endl;
         else
             std::cout << "  Best fitness: " << std::setprecision(6) << population_->get_best().fitness << std::endl;
+    }
+
+    // Solver::benchmark_modes (solver.hpp:1671-1724) compares one GPU with
+    // several over 10 generations.  This build runs on one GPU, so the two
+    // forms compared are its own: the generations launched one by one and
+    // replayed from a captured HIP graph (configs evaluated on the host have
+    // one form only).  The population is initialised again afterwards.
+    void benchmark_modes()
+    {
+        std::cout << "\n=== Launch-form benchmark (one GPU) ===" << std::endl;
+        const bool keep = options_.use_graph;
+        double secs[2] = {0.0, 0.0};
+        for (int graph = 0; graph < 2; ++graph) {
+            options_.use_graph = graph != 0;
+            population_.reset();
+            population_ = std::make_unique<Population<T>>(config_.get(), options_);
+            population_->initialize();
+            population_->evolve(1);  // warm-up (captures the graph)
+            population_->generation();
+            const auto t0 = std::chrono::steady_clock::now();
+            population_->evolve(10);
+            population_->generation();
+            secs[graph] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (population_->host_evaluated()) {
+                secs[1] = secs[0];
+                break;
+            }
+        }
+        std::cout << "Benchmark Results (10 generations):\n  Launches:    " << std::fixed << std::setprecision(3)
+                  << secs[0] << " seconds\n  Graph:       " << secs[1] << " seconds\n  Speedup:     "
+                  << std::setprecision(2) << (secs[1] > 0.0 ? secs[0] / secs[1] : 0.0) << "x" << std::endl;
+        std::cout << "===========================" << std::endl;
+        options_.use_graph = keep;
+        initialize();
+    }
+
+    // Solver::print_multi_gpu_performance (solver.hpp:1591-1625): time
+    // spent in generation steps so far, in total and per generation, on the
+    // one GPU this build runs on (evaluation, crossover and sorting are one
+    // fused step on the device, so they are not split).
+    void print_multi_gpu_performance() const
+    {
+        std::cout << "\n=== GPU Performance Statistics ===\nActive GPUs: 1 (device " << options_.device << ")\n";
+        std::cout << "Generation steps: " << steps_timed_ << ", " << std::fixed << std::setprecision(4) << step_seconds_
+                  << " s";
+        if (steps_timed_ > 0)
+            std::cout << " (" << std::setprecision(4) << step_seconds_ / steps_timed_ * 1000.0 << " ms per generation)";
+        std::cout << "\n=========================================" << std::endl;
     }
 
     const Individual<T>& get_best_individual() const { return population_->get_best(); }
@@ -137,4 +199,6 @@ private:
     std::unique_ptr<Population<T>> population_;
     int current_generation_ = 0;
     double elapsed_ = 0.0;
+    double step_seconds_ = 0.0;  // wall time of generation steps (run, evolve_generation)
+    long long steps_timed_ = 0;
 };
