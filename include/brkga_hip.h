@@ -312,6 +312,19 @@ brkga_status brkga_population_read(brkga_pop_t pop, float* keys, float* fitness,
  * re-partitions them as a fresh generation 0. */
 brkga_status brkga_population_write(brkga_pop_t pop, const float* keys, void* stream);
 
+/* Resumes a population from a checkpoint taken with brkga_population_read
+ * after `generation` completed steps: keys[P][chrom_len], fitness[P][num_obj]
+ * and, for NSGA-II, rank[P] and crowding[P] (they depend on the parents and
+ * children the last selection saw, so they are part of the state).  The rank
+ * order of a BRKGA population is recomputed from the fitness.  Steps after
+ * the restore are bit-identical to those of the run the checkpoint was taken
+ * from (the generation counter drives the Philox streams, migration, local
+ * search and diversity schedules).  The population must have been created
+ * with the same problem and parameters.  No reference counterpart (the
+ * reference has no checkpoint). */
+brkga_status brkga_population_restore(brkga_pop_t pop, const float* keys, const float* fitness, const int32_t* rank,
+                                      const float* crowding, int32_t generation, void* stream);
+
 /* Best individual over all islands (Population::get_best,
  * Solver::get_best_individual): fitness[num_obj], keys[chrom_len] and the
  * decoded solution[chrom_len]; any pointer may be NULL.  NSGA-II: the

@@ -162,6 +162,7 @@ _SIGNATURES = {
     "brkga_population_read": ([_P, _PF, _PF, _PI, _PI, _PF, _P], ctypes.c_int),
     "brkga_population_write": ([_P, _PF, _P], ctypes.c_int),
     "brkga_get_best": ([_P, _PF, _PF, _PI, _P], ctypes.c_int),
+    "brkga_population_restore": ([_P, _PF, _PF, _PI, _PF, _I32, _P], ctypes.c_int),
     "brkga_population_device_buffers": ([_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P), _PI],
                                         ctypes.c_int),
     "brkga_population_profile": ([_P, _I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double), _P],
@@ -420,6 +421,27 @@ class Population:
         keys = np.ascontiguousarray(keys, dtype=np.float32)
         assert keys.shape == (self.size, self.problem.chrom_len)
         _check(lib().brkga_population_write(self.handle, _fptr(keys), stream or None))
+
+    def checkpoint(self):
+        """State to resume from: read() plus the generation counter."""
+        state = self.read()
+        state["generation"] = self.generation
+        return state
+
+    def restore(self, state, stream: int = 0):
+        """Resume from checkpoint() of a population with the same problem and
+        parameters; later steps are bit-identical to the original run's."""
+        P, L, M = self.size, self.problem.chrom_len, self.problem.num_obj
+        keys = np.ascontiguousarray(state["keys"], dtype=np.float32).reshape(P, L)
+        fit = np.ascontiguousarray(state["fitness"], dtype=np.float32).reshape(P, M)
+        rank = crowd = None
+        if M > 1:
+            rank = np.ascontiguousarray(state["rank"], dtype=np.int32)
+            crowd = np.ascontiguousarray(state["crowding"], dtype=np.float32)
+        _check(lib().brkga_population_restore(self.handle, _fptr(keys), _fptr(fit),
+                                              None if rank is None else _iptr(rank),
+                                              None if crowd is None else _fptr(crowd),
+                                              int(state["generation"]), stream or None))
 
     def best(self, stream: int = 0):
         M, L = self.problem.num_obj, self.problem.chrom_len

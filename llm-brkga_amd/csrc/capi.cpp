@@ -1081,6 +1081,43 @@ brkga_status brkga_population_write(brkga_pop_t pop, const float* keys, void* st
     return BRKGA_OK;
 }
 
+brkga_status brkga_population_restore(brkga_pop_t pop, const float* keys, const float* fitness, const int32_t* rank,
+                                      const float* crowding, int32_t generation, void* stream)
+{
+    if (!pop || !keys || !fitness) return set_error(BRKGA_ERR_INVALID_ARG, "null population, keys or fitness");
+    if (generation < 0) return set_error(BRKGA_ERR_INVALID_ARG, "negative generation");
+    if (pop->nsga2 && (!rank || !crowding))
+        return set_error(BRKGA_ERR_INVALID_ARG, "an NSGA-II checkpoint needs its ranks and crowding distances");
+    if (pop->host_open) return set_error(BRKGA_ERR_INVALID_ARG, "a host-evaluated generation is open");
+    DeviceGuard g(pop->problem->engine->device);
+    hipStream_t s = as_stream(stream);
+    const InstDesc& d = pop->host;
+    const int b = generation & 1;  // step t reads buffer t & 1
+    const size_t P = (size_t)d.pop;
+    const int M = d.prob.num_obj;
+    BRKGA_HIP_TRY(hipStreamSynchronize(s));
+    BRKGA_HIP_TRY(hipMemcpy2D(d.keys[b], sizeof(float) * d.ld, keys, sizeof(float) * d.L, sizeof(float) * d.L, P,
+                              hipMemcpyHostToDevice));
+    BRKGA_HIP_TRY(hipMemcpy(d.fit[b], fitness, sizeof(float) * P * M, hipMemcpyHostToDevice));
+    if (pop->nsga2) {
+        BRKGA_HIP_TRY(hipMemcpy(d.rank[b], rank, sizeof(int) * P, hipMemcpyHostToDevice));
+        BRKGA_HIP_TRY(hipMemcpy(d.crowd[b], crowding, sizeof(float) * P, hipMemcpyHostToDevice));
+    }
+    // both step counter slots (the ranking steps read the slot of their
+    // parity), the gather / arrival counter and the flag
+    const int counters[4] = {generation, 0, 0, generation};
+    BRKGA_HIP_TRY(hipMemcpy(d.gen, counters, sizeof(counters), hipMemcpyHostToDevice));
+    pop->host_t = generation;
+    if (!pop->nsga2) {
+        // the rank order is the stable partition of the fitness (as after any
+        // generation), so the next step reads the same parents
+        hipError_t e = launch_partition_one(d, b, 0, s);
+        if (e != hipSuccess) return hip_error(e, "partition launch");
+    }
+    BRKGA_HIP_TRY(hipStreamSynchronize(s));
+    return BRKGA_OK;
+}
+
 brkga_status brkga_get_best(brkga_pop_t pop, float* fitness, float* keys, int32_t* decoded, void* stream)
 {
     if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
