@@ -101,12 +101,85 @@ def traffic_from_profiles():
     return None
 
 
+def cpu_identity():
+    """(nproc, CPU model name) of this host."""
+    model = ""
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    if not model and os.path.exists("/proc/cpuinfo"):
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    return os.cpu_count() or 1, model
+
+
+def reference_loop(xy, n, pop, islands, elite, mutants, gens, mig_interval, mig_k, threads, runs):
+    """The CPU restatement of the reference's host loop (oracle/cpu_reference_loop:
+    Individual objects, n x n matrix, std::sort decode, mt19937 crossover, sort
+    of the members every generation): one warm-up run, then `runs` timed runs;
+    the median."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "cpu_reference_loop")
+    with tempfile.NamedTemporaryFile(suffix=".f32", delete=False) as f:
+        np.ascontiguousarray(xy, np.float32).tofile(f)
+        path = f.name
+    try:
+        out = subprocess.run([exe, path, str(n), str(pop), str(islands), str(elite), str(mutants), str(RHO),
+                              str(gens), str(mig_interval), str(mig_k), str(threads), str(runs), "12345"],
+                             capture_output=True, text=True, timeout=600, check=True).stdout
+    finally:
+        os.unlink(path)
+    return json.loads(out)
+
+
 def cpu_baseline(budget_s):
-    """The CPU oracle (single thread) on the same instance; generations are
-    run until the time budget is spent (at least two)."""
+    """CPU baselines on this host's cores: the restatement of the reference's
+    host loop for the headline workload (4 islands x 256, n = 1000) and for
+    TSP n = 52, P = 256, 500 generations, single-threaded (as the reference's
+    evaluate_all_cpu runs) and with OpenMP over the members on every core of
+    this host's share; plus the flat C oracle (single thread) as a secondary
+    figure."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from brkga_amd import workloads
+
+    nproc, model = cpu_identity()
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = nproc
+    threads = max(1, min(16, share))
+    xy2 = workloads.tsp_coords(0, N_CITIES, 10000.0)
+    xy1 = workloads.tsp_coords(52, 52, 1000.0)
+    Pi, Ei, Mi = POP // ISLANDS, ELITE // ISLANDS, MUTANTS // ISLANDS
+    c2_gens = 20
+    c2_1 = reference_loop(xy2, N_CITIES, Pi, ISLANDS, Ei, Mi, c2_gens, MIG_INTERVAL, MIG_K, 1, 5)
+    c2_t = reference_loop(xy2, N_CITIES, Pi, ISLANDS, Ei, Mi, c2_gens, MIG_INTERVAL, MIG_K, threads, 5)
+    c1_1 = reference_loop(xy1, 52, 256, 1, 51, 25, 500, 0, 0, 1, 5)
+    c1_t = reference_loop(xy1, 52, 256, 1, 51, 25, 500, 0, 0, threads, 5)
+
+    # batched instance set: every 32nd of the 512 instances (sizes across the
+    # whole distribution), 30 generations each; solves/s scaled to 300
+    sizes = [int(v) for v in workloads.log_uniform_sizes(BATCH_COUNT, BATCH_N_LO, BATCH_N_HI)]
+    sample_idx = list(range(0, BATCH_COUNT, 32))
+    per_solve = []
+    for i in sample_idx:
+        r = reference_loop(workloads.tsp_coords(1000 + i, sizes[i], 1000.0), sizes[i], BATCH_POP, 1, BATCH_ELITE,
+                           BATCH_MUTANTS, 30, 0, 0, threads, 1)
+        per_solve.append(r["median_s"] * BATCH_GENS / 30.0)
+    batch_cpu = {"solves_per_s": round(len(per_solve) / sum(per_solve), 3), "threads": threads,
+                 "sample": f"instances {sample_idx[0]}, {sample_idx[1]}, ... {sample_idx[-1]} of the batched set "
+                           f"(n = {min(sizes[i] for i in sample_idx)}..{max(sizes[i] for i in sample_idx)}), "
+                           f"30 generations each, scaled to {BATCH_GENS}"}
 
     xy = workloads.tsp_coords(0, N_CITIES, 10000.0)
     prob = pyoracle.Problem.tsp_coords(xy)
@@ -118,9 +191,20 @@ def cpu_baseline(budget_s):
         st.step()
         gens += 1
     dt = time.perf_counter() - t0
-    return {"value": round(gens / dt, 3), "unit": "gens/s", "cores": 1, "kind": "port",
-            "sample": f"{gens} generations of the same n=1000 P=1024 4-island workload, CPU oracle (C, -O2, "
-                      f"one thread), {dt:.1f} s"}
+    pick = lambda r: {"gens_per_s": round(r["gens_per_s"], 3), "median_s": round(r["median_s"], 4),
+                      "threads": r["threads"], "runs_s": [round(x, 4) for x in r["runs_s"]]}
+    return {"value": round(c2_t["gens_per_s"], 3), "unit": "gens/s", "cores": threads, "kind": "port",
+            "sample": f"{c2_gens} generations of the headline workload (4 islands x 256, n = 1000, migration "
+                      f"top-2 every 50) per run, median of 5 runs after a warm-up, in the CPU restatement of the "
+                      f"reference's host loop (Individual objects, n x n FP32 matrix, std::sort decode, mt19937 "
+                      f"crossover, member sort per generation) with OpenMP over the members on {threads} threads",
+            "nproc": nproc, "cpu_model": model,
+            "headline_single_thread": pick(c2_1), "headline_all_threads": pick(c2_t),
+            "tsp52_p256_500_generations": {"single_thread": pick(c1_1), "all_threads": pick(c1_t)},
+            "batched_set": batch_cpu,
+            "oracle_single_thread": {"gens_per_s": round(gens / dt, 3),
+                                     "sample": f"{gens} generations of the headline workload, flat-array C oracle "
+                                               f"(-O2, one thread), {dt:.1f} s"}}
 
 
 BATCH_COUNT, BATCH_POP, BATCH_ELITE, BATCH_MUTANTS, BATCH_RHO, BATCH_GENS = 512, 256, 38, 38, 0.7, 300
@@ -306,7 +390,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=4.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-gens", type=int, default=BATCH_GENS)
