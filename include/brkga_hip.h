@@ -349,6 +349,19 @@ brkga_status brkga_population_autotune(brkga_pop_t pop, int32_t reps, int32_t* b
  * Results never depend on it. */
 brkga_status brkga_population_set_block(brkga_pop_t pop, int32_t block);
 
+/* Children per workgroup of the BRKGA ranking step (1, 2 or 4): with more
+ * than one, a workgroup builds that many children of one island and ranks
+ * the island once for all of them (Euclidean TSP whose islands fit the
+ * workgroup size and whose keys fit 4 per thread; otherwise 1 is used).
+ * Results never depend on it; brkga_population_autotune picks it together
+ * with the workgroup size. */
+brkga_status brkga_population_set_children(brkga_pop_t pop, int32_t children);
+
+/* Launch shape the generation kernel runs with: threads per child (*block)
+ * and children per workgroup (*children, 1 unless the multi-child step
+ * applies).  Either pointer may be NULL. */
+brkga_status brkga_population_step_shape(brkga_pop_t pop, int32_t* block, int32_t* children);
+
 /* Mean device time of the generation kernel (BRKGA step / NSGA-II
  * offspring) and of the kernels that close a generation (partition and any
  * local search / NSGA-II selection + gather), in microseconds, over `reps`

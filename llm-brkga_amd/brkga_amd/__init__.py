@@ -169,6 +169,8 @@ _SIGNATURES = {
                                  ctypes.c_int),
     "brkga_population_autotune": ([_P, _I32, _PI, _P], ctypes.c_int),
     "brkga_population_set_block": ([_P, _I32], ctypes.c_int),
+    "brkga_population_set_children": ([_P, _I32], ctypes.c_int),
+    "brkga_population_step_shape": ([_P, _PI, _PI], ctypes.c_int),
     "brkga_batch_create": ([_P, _I32, ctypes.POINTER(_P), ctypes.POINTER(CParams), ctypes.POINTER(_P)],
                            ctypes.c_int),
     "brkga_batch_destroy": ([_P], ctypes.c_int),
@@ -482,6 +484,17 @@ class Population:
     def set_block(self, block: int):
         """Workgroup size of the generation kernel (results never depend on it)."""
         _check(lib().brkga_population_set_block(self.handle, int(block)))
+
+    def set_children(self, children: int):
+        """Children per workgroup of the ranking step (1, 2, 4); results never
+        depend on it."""
+        _check(lib().brkga_population_set_children(self.handle, int(children)))
+
+    def step_shape(self):
+        """(threads per child, children per workgroup) of the generation kernel."""
+        b, c = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().brkga_population_step_shape(self.handle, ctypes.byref(b), ctypes.byref(c)))
+        return b.value, c.value
 
     def autotune(self, reps: int = 20, stream: int = 0) -> int:
         """Fastest workgroup size of the generation kernel (results unchanged)."""

@@ -486,6 +486,14 @@ bool pop_ranks(const brkga_pop_s* pop)
     return !pop->nsga2 && step_can_rank(pop->host, pop->block, pop->problem->engine->lds_limit);
 }
 
+// Children per workgroup the population's ranking steps run with (1 when
+// the multi-child form does not apply at the current workgroup size).
+int pop_children(const brkga_pop_s* pop)
+{
+    if (pop->children <= 1) return 1;
+    return step_multi_ok(pop->host, pop->block, pop->children, pop->problem->engine->lds_limit) ? pop->children : 1;
+}
+
 // Enqueues generation 0 (init) or one step whose parent generation has
 // buffer parity `cur`.
 brkga_status pop_step(brkga_pop_t pop, int init, int cur, hipStream_t s)
@@ -502,7 +510,7 @@ brkga_status pop_step(brkga_pop_t pop, int init, int cur, hipStream_t s)
             e = launch_nsga2_diversity(d.prob.kind, pop->block, pop->d_desc, 1, d.pop, pop->step_lds, d.pop, s);
     } else {
         const int rank = !init && pop_ranks(pop);
-        e = launch_step_one(d, pop->block, cur, init, rank, s);
+        e = launch_step_one(d, pop->block, cur, init, rank, s, nullptr, nullptr, rank ? pop_children(pop) : 1);
         if (e == hipSuccess && !rank) e = launch_partition_one(d, init ? 0 : cur ^ 1, init ? 0 : 1, s);
         // TSPJ local search: the kernels check on the device whether the
         // generation just built is due, so graphs replay at any generation
@@ -526,6 +534,26 @@ brkga_status pop_close(brkga_pop_t pop, int buf, hipStream_t s)
     if (e != hipSuccess) return hip_error(e, "partition launch");
     return BRKGA_OK;
 }
+
+}  // namespace
+
+brkga_status brkga::population_steps(brkga_pop_s* pop, int init, int steps, hipStream_t s)
+{
+    if (init) {
+        BRKGA_HIP_TRY(hipMemsetAsync(pop->host.gen, 0, 4 * sizeof(int), s));
+        pop->host_t = 0;
+        const brkga_status st = pop_step(pop, 1, 0, s);
+        if (st != BRKGA_OK) return st;
+    }
+    for (int i = 0; i < steps; ++i) {
+        const brkga_status st = pop_step(pop, 0, pop->host_t & 1, s);
+        if (st != BRKGA_OK) return st;
+        ++pop->host_t;
+    }
+    return steps > 0 ? pop_close(pop, pop->host_t & 1, s) : BRKGA_OK;
+}
+
+namespace {
 
 brkga_status read_gen(brkga_pop_t pop, hipStream_t s, int* gen)
 {
@@ -1201,37 +1229,44 @@ brkga_status brkga_population_autotune(brkga_pop_t pop, int32_t reps, int32_t* b
     // slot; the closing partition below sets both slots back.
     const size_t lds_limit = pop->problem->engine->lds_limit;
     bool ranked = false;
-    auto launch = [&](int blk) {
+    const int keep_ch = pop->children;
+    int best_ch = keep_ch;
+    auto launch = [&](int blk, int ch) {
         if (pop->nsga2)
             return launch_nsga2_offspring(d.prob.kind, blk, pop->d_desc, 1, d.pop, pop->step_lds, 0, s);
         if (step_can_rank(d, blk, lds_limit)) {
             ranked = true;
-            return launch_step_one(d, blk, cur, 0, 1, s);
+            return launch_step_one(d, blk, cur, 0, 1, s, nullptr, nullptr, ch);
         }
         hipError_t r = launch_step_one(d, blk, cur, 0, 0, s);
         return r == hipSuccess ? launch_partition_one(d, cur ^ 1, 0, s) : r;
     };
     for (int blk = 64; blk <= 1024 && e == hipSuccess; blk <<= 1) {
         if (blk * kSortItems < n) continue;
-        e = launch(blk);
-        if (e == hipSuccess) e = hipEventRecord(a, s);
-        for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch(blk);
-        if (e == hipSuccess) e = hipEventRecord(b, s);
-        if (e == hipSuccess) e = hipEventSynchronize(b);
-        float ms = 0.0f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
-        if (e == hipSuccess && (!have || ms < best_ms)) {
-            have = true;
-            best_ms = ms;
-            best = blk;
+        for (int ch = 1; ch <= 4 && e == hipSuccess; ch <<= 1) {
+            if (ch > 1 && (pop->nsga2 || !step_multi_ok(d, blk, ch, lds_limit))) continue;
+            e = launch(blk, ch);
+            if (e == hipSuccess) e = hipEventRecord(a, s);
+            for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch(blk, ch);
+            if (e == hipSuccess) e = hipEventRecord(b, s);
+            if (e == hipSuccess) e = hipEventSynchronize(b);
+            float ms = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+            if (e == hipSuccess && (!have || ms < best_ms)) {
+                have = true;
+                best_ms = ms;
+                best = blk;
+                best_ch = ch;
+            }
         }
     }
     if (e == hipSuccess && ranked) e = launch_partition_one(d, cur, 3, s);
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (e != hipSuccess) return hip_error(e, "autotune launches");
-    if (best != keep) {
+    if (best != keep || best_ch != keep_ch) {
         pop->block = best;
+        pop->children = best_ch;
         for (auto& kv : pop->graphs) (void)hipGraphExecDestroy(kv.second);
         pop->graphs.clear();
     }
@@ -1344,6 +1379,27 @@ brkga_status brkga_population_host_end(brkga_pop_t pop, const float* fitness, vo
     return BRKGA_OK;
 }
 
+brkga_status brkga_population_set_children(brkga_pop_t pop, int32_t children)
+{
+    if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
+    if (children != 1 && children != 2 && children != 4)
+        return set_error(BRKGA_ERR_INVALID_ARG, "children per workgroup must be 1, 2 or 4");
+    if (children != pop->children) {
+        pop->children = children;
+        for (auto& kv : pop->graphs) (void)hipGraphExecDestroy(kv.second);
+        pop->graphs.clear();
+    }
+    return BRKGA_OK;
+}
+
+brkga_status brkga_population_step_shape(brkga_pop_t pop, int32_t* block, int32_t* children)
+{
+    if (!pop) return set_error(BRKGA_ERR_INVALID_ARG, "null population");
+    if (block) *block = pop->block;
+    if (children) *children = pop_ranks(pop) ? pop_children(pop) : 1;
+    return BRKGA_OK;
+}
+
 brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* step_us, double* close_us,
                                       void* stream)
 {
@@ -1369,7 +1425,7 @@ brkga_status brkga_population_profile(brkga_pop_t pop, int32_t reps, double* ste
         hipEvent_t* q = &ev[4 * (size_t)i];
         const int cur = pop->host_t & 1;
         if (timed) {
-            e = launch_step_one(d, pop->block, cur, 0, ranks, s, q[0], q[1]);
+            e = launch_step_one(d, pop->block, cur, 0, ranks, s, q[0], q[1], ranks ? pop_children(pop) : 1);
             if (e == hipSuccess && !ranks) e = launch_partition_one(d, cur ^ 1, 1, s, q[2], q[3]);
         } else {
             e = hipEventRecord(q[0], s);

@@ -86,7 +86,8 @@ struct brkga_pop_s {
     brkga::InstDesc* d_desc;  // descriptor in device memory
     std::vector<void*> owned;
     int nsga2;
-    int block;                // workgroup size of the step kernel
+    int block;                // workgroup size of the step kernel (threads per child)
+    int children = 1;         // children per workgroup of the ranking step (1, 2, 4)
     int step_lds;             // LDS bytes of one step block
     int host_t;               // generation steps enqueued since initialize
     int* d_decode_row;        // scratch for get_best decode
@@ -117,6 +118,13 @@ brkga_status create_population(brkga_problem_s* pr, const brkga_params* q, Devic
 // Frees a population's buffers through its allocator (no device-wide wait
 // for arena-backed populations).
 void destroy_population(brkga_pop_s* pop);
+
+// Enqueues `steps` generation steps of one population through the
+// single-population launches (as brkga_evolve does), after building its
+// generation 0 first when init != 0; pop->host_t must hold the population's
+// current generation.  Steps ignore gen_limit: the caller runs exactly the
+// steps it wants.
+brkga_status population_steps(brkga_pop_s* pop, int init, int steps, hipStream_t s);
 
 // Slot of the best individual (Population::get_best): BRKGA, the better of
 // the islands' rank-0 slots (order_or_rank = rank order); NSGA-II, the

@@ -30,7 +30,7 @@ __device__ __forceinline__ void tsp_coords_tail(const ProblemDesc& p, char* lds,
                                                 const SortScratch& ss, int nb, float2 (&c)[ITEMS],
                                                 const int (&pos)[ITEMS], float* fit_out, int* decoded, Load load)
 {
-    const int tid = threadIdx.x;
+    const int tid = ltid<BLOCK>();
     const int n = p.n;
     const bool in_cnt = nb >= 2 * n;
     float2* txy = reinterpret_cast<float2*>(in_cnt ? reinterpret_cast<char*>(ss.cnt) : lds + lay.items_off);
@@ -59,34 +59,50 @@ __device__ __forceinline__ void tsp_coords_tail(const ProblemDesc& p, char* lds,
     if (tid == 0 && fit_out) fit_out[0] = fx_to_float(tot, p.inv0);
 }
 
-// TSP over coordinates with the child's keys in registers: thread t holds
-// keys 4t .. 4t+3 (the float4 column it built, so n <= 4 BLOCK) and ranks
-// them straight away, with no barrier between building the child and
-// ranking it.  The argsort counters were cleared before the caller's last
-// barrier.
-template <int BLOCK>
-__device__ __forceinline__ void decode_tsp_coords_reg(const ProblemDesc& p, char* lds, const DecodeLayout& lay,
-                                                      int nb, float* fit_out, float4 kv)
+__device__ __forceinline__ void key_components(const float4& v, float (&k)[4])
 {
-    const int tid = threadIdx.x;
+    k[0] = v.x;
+    k[1] = v.y;
+    k[2] = v.z;
+    k[3] = v.w;
+}
+__device__ __forceinline__ void key_components(const float2& v, float (&k)[2])
+{
+    k[0] = v.x;
+    k[1] = v.y;
+}
+
+// TSP over coordinates with the child's keys in registers: thread t holds
+// keys W t .. W t + W - 1 (the float4 or float2 column it built, so n <= W
+// BLOCK) and ranks them straight away, with no barrier between building
+// the child and ranking it.  The argsort counters were cleared before the
+// caller's last barrier.
+template <int BLOCK, typename V>
+__device__ __forceinline__ void decode_tsp_coords_reg(const ProblemDesc& p, char* lds, const DecodeLayout& lay,
+                                                      int nb, float* fit_out, V kv, int* long_flag = nullptr)
+{
+    constexpr int W = sizeof(V) / sizeof(float);
+    const int tid = ltid<BLOCK>();
     const int n = p.n;
     SortScratch ss;
     ss.cnt = reinterpret_cast<int*>(lds + lay.cnt_off);
     ss.items = reinterpret_cast<unsigned long long*>(lds + lay.items_off);
     ss.red = reinterpret_cast<long long*>(lds + lay.red_off);
-    const float kk[4] = {kv.x, kv.y, kv.z, kv.w};
+    ss.flag = long_flag;
+    float kk[W];
+    key_components(kv, kk);
     const float2* xy = reinterpret_cast<const float2*>(p.a);
-    float2 c[4];
-    int pos[4];
+    float2 c[W];
+    int pos[W];
     auto load = [&]() {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int e = 4 * tid + i;
+        for (int i = 0; i < W; ++i) {
+            const int e = W * tid + i;
             if (e < n) c[i] = xy[e];
         }
     };
-    block_rank_by<BLOCK, true, 4, true>([&](int i, int) { return kk[i]; }, n, nb, ss, pos, nullptr, load);
-    tsp_coords_tail<BLOCK, 4, true, true>(p, lds, lay, ss, nb, c, pos, fit_out, nullptr, load);
+    block_rank_by<BLOCK, true, W, true>([&](int i, int) { return kk[i]; }, n, nb, ss, pos, nullptr, load);
+    tsp_coords_tail<BLOCK, W, true, true>(p, lds, lay, ss, nb, c, pos, fit_out, nullptr, load);
 }
 
 // CLEARED: the argsort counters were cleared (and a barrier passed) by the

@@ -21,6 +21,18 @@ namespace brkga {
 
 constexpr int kWave = 64;
 
+// Thread index within a group of BLOCK threads (a power of two).  The
+// block-level helpers below work on such groups: in a kernel of BLOCK
+// threads this is threadIdx.x (launch bounds let the compiler drop the
+// mask); in a workgroup that runs several groups side by side (one child
+// each), every group has its own LDS and all groups pass the same
+// sequence of barriers.
+template <int BLOCK>
+__device__ __forceinline__ int ltid()
+{
+    return (int)threadIdx.x & (BLOCK - 1);
+}
+
 struct U4 {
     uint32_t x, y, z, w;
 };
@@ -280,7 +292,7 @@ __device__ __forceinline__ long long block_sum_i64(long long v, long long* red)
     constexpr int W = BLOCK / kWave;
     v = wave_sum_i64(v);
     if (W == 1) return v;
-    const int wid = threadIdx.x / kWave;
+    const int wid = ltid<BLOCK>() / kWave;
     if ((threadIdx.x & (kWave - 1)) == 0) red[wid] = v;
     __syncthreads();
     long long t = 0;
@@ -296,7 +308,7 @@ __device__ __forceinline__ long long block_max_i64(long long v, long long* red)
     constexpr int W = BLOCK / kWave;
     v = wave_max_i64(v);
     if (W == 1) return v;
-    const int wid = threadIdx.x / kWave;
+    const int wid = ltid<BLOCK>() / kWave;
     if ((threadIdx.x & (kWave - 1)) == 0) red[wid] = v;
     __syncthreads();
     long long t = red[0];
@@ -312,7 +324,7 @@ __device__ __forceinline__ int block_exclusive_scan_i32(int v, int* red, int* to
 {
     constexpr int W = BLOCK / kWave;
     const int inc = wave_inclusive_scan_i32(v);
-    const int wid = threadIdx.x / kWave;
+    const int wid = ltid<BLOCK>() / kWave;
     if (W == 1) {
         *total = __builtin_amdgcn_readlane(inc, kWave - 1);
         return inc - v;
@@ -337,7 +349,7 @@ __device__ __forceinline__ long long block_inclusive_scan_i64(long long v, long 
     constexpr int W = BLOCK / kWave;
     const long long inc = wave_inclusive_scan_i64(v);
     if (W == 1) return inc;
-    const int wid = threadIdx.x / kWave;
+    const int wid = ltid<BLOCK>() / kWave;
     if ((threadIdx.x & (kWave - 1)) == kWave - 1) red[wid] = inc;
     __syncthreads();
     long long base = 0;
@@ -400,6 +412,8 @@ struct SortScratch {
                                 // cnt[nb + 1]: some bucket is too long to scan
     unsigned long long* items;  // [n] (key order << 32 | index) in bucket order
     long long* red;             // [17] per-wave partials; red[16] = block sum
+    int* flag = nullptr;        // long-bucket flag shared by every group of a
+                                // multi-child workgroup (null: cnt[nb + 1])
 };
 
 // Clears the bucket counters (nb + 4 of them, 16-byte aligned) and the
@@ -408,8 +422,8 @@ template <int BLOCK>
 __device__ __forceinline__ void argsort_clear(const SortScratch& s, int nb)
 {
     int4* c4 = reinterpret_cast<int4*>(s.cnt);
-    for (int i = threadIdx.x; i <= (nb >> 2); i += BLOCK) c4[i] = make_int4(0, 0, 0, 0);
-    if (threadIdx.x == 0) s.red[16] = 0;
+    for (int i = ltid<BLOCK>(); i <= (nb >> 2); i += BLOCK) c4[i] = make_int4(0, 0, 0, 0);
+    if (ltid<BLOCK>() == 0) s.red[16] = 0;
 }
 
 // Exclusive scan of cnt[0..nb) in place (bucket starts), cnt[nb] = n.
@@ -417,7 +431,7 @@ template <int BLOCK>
 __device__ __forceinline__ void scan_bucket_counts(int* cnt, int nb, int n, int* red)
 {
     constexpr int W = BLOCK / kWave;
-    const int tid = threadIdx.x;
+    const int tid = ltid<BLOCK>();
     const int per = nb >= BLOCK ? nb / BLOCK : 1;
     const int lo = tid * per;
     int local = 0;
@@ -489,7 +503,7 @@ __device__ void block_sort_items(unsigned long long* a, int n)
         for (int j = k >> 1; j >= 1; j >>= 1) {
             const int lj = __builtin_ctz(j);
             const bool flip = j == (k >> 1);
-            for (int c = threadIdx.x; c < half; c += BLOCK) {
+            for (int c = ltid<BLOCK>(); c < half; c += BLOCK) {
                 const int base = (c >> lj) << (lj + 1);
                 const int off = c & (j - 1);
                 const int lo = base + off;
@@ -558,7 +572,7 @@ template <int BLOCK, bool CLEARED, int ITEMS, bool CONTIG, typename KeyAt, typen
 __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const SortScratch& s,
                                               int (&pos)[ITEMS], int* perm = nullptr, Hook pre_rank = Hook())
 {
-    const int tid = threadIdx.x;
+    const int tid = ltid<BLOCK>();
     if (!CLEARED) {
         argsort_clear<BLOCK>(s, nb);
         __syncthreads();
@@ -591,10 +605,11 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
             longest = en - st > longest ? en - st : longest;
         }
     }
-    if (longest > kBucketScanLimit) s.cnt[nb + 1] = 1;
+    int* const long_flag = s.flag ? s.flag : s.cnt + nb + 1;
+    if (longest > kBucketScanLimit) *long_flag = 1;
     __syncthreads();
     pre_rank();
-    if (__builtin_amdgcn_readfirstlane(s.cnt[nb + 1])) {
+    if (__builtin_amdgcn_readfirstlane(*long_flag)) {
         // a bucket too long to scan: sort every item, then each key finds
         // its own (unique) item
         block_sort_items<BLOCK>(s.items, n);

@@ -15,6 +15,7 @@
 // Buffers: keys[b] holds 2N rows (parents [0,N), children [N,2N)).
 #include "launch.hpp"
 #include "decoders.hpp"
+#include "step_device.hpp"
 
 namespace brkga {
 
@@ -144,9 +145,10 @@ __global__ __launch_bounds__(BLOCK) void k_nsga2_offspring(const InstDesc* __res
     }
 }
 
-// LDS layout of the selection block for C = up to 2N individuals.
+// LDS layout of the selection block for C = up to 2N individuals (xch: the
+// exchange buffers of the register sorting network, for C <= 1024).
 struct SelectLayout {
-    int o0, o1, rk, cnt, lst, cr, fst, fs, sk, bytes;
+    int o0, o1, rk, cnt, lst, cr, fst, fs, sk, xch, bytes;
 };
 
 __host__ __device__ inline SelectLayout select_layout(int C)
@@ -162,13 +164,39 @@ __host__ __device__ inline SelectLayout select_layout(int C)
     s.cr = off; off += C * 4;
     s.fst = off; off += (C + 2) * 4;
     s.fs = off; off += (C + 2) * 4;
-    off = round_up_int(off, 8);
+    off = round_up_int(off, 16);
     s.sk = off; off += Cp * 8;
+    s.xch = off; off += Cp <= 1024 ? island_rank_bytes(Cp < 64 ? 64 : Cp) : 0;
     s.bytes = round_up_int(off, 16);
     return s;
 }
 
 constexpr int kSelectBlock = 1024;
+
+// Ascending sort of the Cp (power of two) keys sk[0, Cp) of the selection
+// block, left in sk.  Up to kSelectBlock keys: one key per thread through
+// the register bitonic network of the island ranking (in-wave exchanges by
+// DPP / swizzle / permute, one LDS exchange and barrier per step across
+// waves: 10 barriers for 1024 keys); larger: the LDS network (a barrier per
+// stage).  Ends with a barrier.
+__device__ void select_sort(unsigned long long* sk, int Cp, unsigned long long* xch)
+{
+    constexpr int B = kSelectBlock;
+    if (Cp > B) {
+        bitonic_u64<B>(sk, Cp);
+        return;
+    }
+    const int t = threadIdx.x;
+    unsigned long long v = t < Cp ? sk[t] : ~0ull;
+    if (Cp <= 64) v = block_bitonic_sort<64>(v, xch);
+    else if (Cp <= 128) v = block_bitonic_sort<128>(v, xch);
+    else if (Cp <= 256) v = block_bitonic_sort<256>(v, xch);
+    else if (Cp <= 512) v = block_bitonic_sort<512>(v, xch);
+    else v = block_bitonic_sort<1024>(v, xch);
+    __syncthreads();  // every exchange read is done before sk is rewritten
+    if (t < Cp) sk[t] = v;
+    __syncthreads();
+}
 constexpr int kDomBlock = 256;
 static_assert(kDomRows <= kSelectBlock, "one dominance row per selection thread");
 
@@ -251,6 +279,7 @@ __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* _
     int* fst = reinterpret_cast<int*>(lds + L.fst);  // first sorted position of each front
     int* fs = reinterpret_cast<int*>(lds + L.fs);    // front sizes
     unsigned long long* sk = reinterpret_cast<unsigned long long*>(lds + L.sk);
+    unsigned long long* xch = reinterpret_cast<unsigned long long*>(lds + L.xch);
 
     const float* fit = d.fit[cur];
     for (int i = threadIdx.x; i < C; i += B) {
@@ -355,7 +384,7 @@ __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* _
                                 (unsigned long long)i
                           : ~0ull;
         __syncthreads();
-        bitonic_u64<B>(sk, Cp);
+        select_sort(sk, Cp, xch);
         for (int q = threadIdx.x; q < C; q += B) {
             const int e = (int)(sk[q] & 0xFFFull);
             const int re = rk[e];
@@ -446,7 +475,7 @@ __global__ __launch_bounds__(kSelectBlock) void k_nsga2_select(const InstDesc* _
             sk[i] = key;
         }
         __syncthreads();
-        bitonic_u64<B>(sk, Cp);
+        select_sort(sk, Cp, xch);
         for (int q = threadIdx.x; q < N - cum; q += B) {
             const int e = (int)(sk[q] & 0xFFFull);
             order[cum + q] = e;

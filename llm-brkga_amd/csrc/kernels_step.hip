@@ -19,9 +19,12 @@ bool step_can_rank(const InstDesc& d, int block, size_t lds_limit)
 }
 
 hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, int rank, hipStream_t s,
-                           hipEvent_t start, hipEvent_t stop)
+                           hipEvent_t start, hipEvent_t stop, int children)
 {
     rank = rank && !init;
+    // the multi-child form exists for ranking steps only; callers check
+    // step_multi_ok
+    if (children > 1 && rank) return launch_step_multi(block, children, d, cur, s, start, stop);
     const int lds = step_lds_bytes(d) + (rank ? island_rank_bytes(d.Pi) : 0);
     const int kind = d.prob.kind;
     switch (kind) {

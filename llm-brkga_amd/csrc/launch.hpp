@@ -20,11 +20,18 @@ hipError_t launch_evaluate(const ProblemDesc& p, const float* keys, long long ld
 // such steps ends with a partition in mode 3.
 // start / stop (optional): events that take the kernel's own start and end
 // times (hipExtLaunchKernel), for profiling without marker packets.
+// children > 1 (step_multi_ok): each workgroup builds that many children
+// of one island and ranks the island once for all of them (ranking steps,
+// Euclidean TSP with keys in registers); results do not change.
 hipError_t launch_step_one(const InstDesc& d, int block, int cur, int init, int rank, hipStream_t s,
-                           hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+                           hipEvent_t start = nullptr, hipEvent_t stop = nullptr, int children = 1);
 // Whether launch_step_one can rank in the step at this workgroup size with
 // `lds_limit` bytes of LDS per workgroup.
 bool step_can_rank(const InstDesc& d, int block, size_t lds_limit);
+// Whether the multi-child step runs this population at (block, children).
+bool step_multi_ok(const InstDesc& d, int block, int children, size_t lds_limit);
+hipError_t launch_step_multi(int block, int children, const InstDesc& d, int cur, hipStream_t s, hipEvent_t start,
+                             hipEvent_t stop);
 
 // Stable per-island sort of the fitness in buffer `buf` into the rank order.
 // mode 0: sort only; 1: closes a generation step (bumps the step counter and

@@ -129,6 +129,7 @@ struct Request {
     SlabAllocator alloc;
     brkga_pop_s* pop = nullptr;
     int ticks = 0;
+    double gen_us = 0.0;  // device time per generation in its last lone round
     brkga_request_info info{};
     std::vector<float> fitness;
     std::vector<int32_t> decoded;
@@ -148,6 +149,8 @@ struct brkga_server_s {
     std::unique_ptr<Fleet> fleet;
     hipStream_t stream = nullptr;     // ticks and population set-up
     hipStream_t collect_stream = nullptr;  // results of leaving requests
+    hipStream_t solo_streams[4] = {};      // one per request of a solo round
+    hipEvent_t setup_done = nullptr;       // population set-up on `stream` before solo steps
     char* staging = nullptr;               // pinned host staging of results
     size_t staging_bytes = 0;
     std::chrono::steady_clock::time_point t0;
@@ -168,8 +171,10 @@ struct brkga_server_s {
     std::vector<std::unique_ptr<Request>> resident;
     std::vector<std::unique_ptr<Request>> leaving;  // at their deadline, results not collected yet
     std::vector<brkga_problem_s*> doomed;           // owned problems to destroy after the round
+    bool fleet_stale = false;                       // resident set changed outside the ticks
 
     void run();
+    bool solo_round(double* waited);
     bool admit();
     void collect_leaving();
     void finish(std::unique_ptr<Request> r, brkga_status st);
@@ -317,6 +322,99 @@ void brkga_server_s::collect_leaving()
     leaving.clear();
 }
 
+// Generations a lone request runs per round through the single-population
+// launches (bounds how long a request arriving meanwhile waits to be
+// admitted: 64 generations of the largest requests take about a
+// millisecond).
+constexpr int kSoloChunk = 64;
+// A request alone on the device runs longer rounds, about kLoneRoundUs of
+// device time each (sized from its previous round; the first runs
+// kLoneFirst generations): few synchronisations for a short request, and a
+// request arriving meanwhile waits at most about that long to be admitted.
+constexpr int kLoneFirst = 32;
+constexpr int kLoneMax = 256;
+constexpr double kLoneRoundUs = 1500.0;
+// Most requests a solo round advances side by side, each on a stream of its
+// own; more resident requests share the batched ticks.
+constexpr int kSoloMax = 4;
+
+// One round for a few requests (at most kSoloMax) that have the device to
+// themselves with nothing pending: a chunk of each one's generations
+// through the single-population launches (one step launch per generation),
+// each on a stream of its own so they overlap on the device.  A few
+// requests finish sooner this way than through the batched ticks, whose
+// per-tick fork, partition and join cost more than a few members' steps.
+// Finished requests are collected in the same round.  A population's device
+// state after these steps is what the ticks would have left (solo
+// equivalence), so requests return to the ticks when more join.  False
+// when the round is not such a round.
+bool brkga_server_s::solo_round(double* waited_out)
+{
+    if (resident.empty() || (int)resident.size() > kSoloMax || !leaving.empty()) return false;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (!pending.empty()) return false;
+    }
+    brkga_status st = BRKGA_OK;
+    // populations admitted this round were set up on `stream`
+    hipError_t e = hipEventRecord(setup_done, stream);
+    std::vector<int> advanced(resident.size(), 0);
+    for (size_t i = 0; i < resident.size() && st == BRKGA_OK; ++i) {
+        Request* r = resident[i].get();
+        hipStream_t ss = solo_streams[i];
+        if (e == hipSuccess) e = hipStreamWaitEvent(ss, setup_done, 0);
+        if (e != hipSuccess) {
+            st = hip_error(e, "server solo streams");
+            break;
+        }
+        const int init = r->ticks == 0;
+        // tick 1 builds generation 0, tick g + 1 completes step g
+        if (!init) r->pop->host_t = r->ticks - 1;
+        const int done = init ? 0 : r->ticks - 1;
+        int chunk = kSoloChunk;
+        if (resident.size() == 1)
+            chunk = r->gen_us > 0.0 ? (int)std::min<double>(kLoneMax, std::max(16.0, kLoneRoundUs / r->gen_us))
+                                    : kLoneFirst;
+        const int steps = std::min(chunk, r->generations - done);
+        st = population_steps(r->pop, init, steps, ss);
+        advanced[i] = steps + init;
+    }
+    const double h0 = now_us(t0);
+    for (size_t i = 0; i < resident.size(); ++i) {
+        const hipError_t se = hipStreamSynchronize(solo_streams[i]);
+        if (se != hipSuccess && st == BRKGA_OK) st = hip_error(se, "server solo steps");
+    }
+    *waited_out = now_us(t0) - h0;
+    if (st == BRKGA_OK && resident.size() == 1 && advanced[0] > 0)
+        resident[0]->gen_us = *waited_out / advanced[0];
+    if (st == BRKGA_OK) {
+        std::vector<std::unique_ptr<Request>> keep;
+        int most = 0;
+        for (size_t i = 0; i < resident.size(); ++i) {
+            auto& r = resident[i];
+            r->ticks += advanced[i];
+            most = std::max(most, advanced[i]);
+            if (r->ticks >= r->generations + 1) leaving.push_back(std::move(r));
+            else keep.push_back(std::move(r));
+        }
+        resident.swap(keep);
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            stats.ticks += most;
+        }
+        collect_leaving();
+        return true;
+    }
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        worker_error = brkga_last_error();
+    }
+    std::vector<std::unique_ptr<Request>> rs;
+    rs.swap(resident);
+    for (auto& r : rs) finish(std::move(r), st);
+    return true;
+}
+
 void brkga_server_s::run()
 {
     DeviceGuard g(engine->device);
@@ -331,7 +429,19 @@ void brkga_server_s::run()
         double waited = 0.0;
         brkga_status st = BRKGA_OK;
         const bool admitted = admit();
-        if (admitted || !leaving.empty()) {
+        if (solo_round(&waited)) {
+            // the fleet's member list is rebuilt when the resident set
+            // next changes under the ticks
+            fleet_stale = true;
+            for (auto* p : doomed) (void)brkga_problem_destroy(p);
+            doomed.clear();
+            std::lock_guard<std::mutex> lock(mu);
+            stats.wait_us += waited;
+            stats.host_us += now_us(t0) - h0 - waited;
+            continue;
+        }
+        if (admitted || !leaving.empty() || fleet_stale) {
+            fleet_stale = false;
             std::vector<brkga_pop_s*> pops;
             for (auto& r : resident) pops.push_back(r->pop);
             st = fleet->set_members(pops, stream);
@@ -409,6 +519,9 @@ brkga_status brkga_server_start(brkga_engine_t engine, const brkga_server_config
     if (st != BRKGA_OK) return st;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->collect_stream, hipStreamNonBlocking);
+    for (int i = 0; i < kSoloMax && e == hipSuccess; ++i)
+        e = hipStreamCreateWithFlags(&s->solo_streams[i], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->setup_done, hipEventDisableTiming);
     if (e != hipSuccess) return hip_error(e, "hipStreamCreate");
     s->fleet.reset(new Fleet(engine));
     s->stats.budget_bytes = s->cfg.hbm_budget_bytes;
@@ -510,6 +623,9 @@ brkga_status brkga_server_stop(brkga_server_t server)
         server->fleet.reset();
         (void)hipStreamDestroy(server->stream);
         (void)hipStreamDestroy(server->collect_stream);
+        for (hipStream_t& ss : server->solo_streams)
+            if (ss) (void)hipStreamDestroy(ss);
+        if (server->setup_done) (void)hipEventDestroy(server->setup_done);
         if (server->staging) (void)hipHostFree(server->staging);
     }
     delete server;

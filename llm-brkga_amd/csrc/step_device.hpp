@@ -154,30 +154,159 @@ __device__ __forceinline__ void island_rank_order(const InstDesc& d, int island,
     if (t < Pi) order[t] = (int)(unsigned)v;
 }
 
+// ---- island rank order by one group of a multi-child workgroup ----------
+//
+// The same network as block_bitonic_sort over one (fitness key, slot) key
+// per thread of a group of threads (local index lt), in flip form.  `active`
+// is uniform per wave: inactive waves skip the arithmetic and the in-wave
+// exchanges but pass the same barriers (exchanges that cross waves go
+// through LDS, one barrier each), so groups that rank nothing and groups
+// that rank different islands of the same size stay in step.
+
+template <int PN, int X>
+__device__ __forceinline__ unsigned long long xor_exchange_g(unsigned long long v, unsigned long long* xch, int lt,
+                                                             int bp, int& half, bool active)
+{
+    if constexpr (X >= kWave) {
+        unsigned long long* b = xch + half * PN;
+        half ^= 1;
+        if (active && lt < PN) b[lt] = v;
+        __syncthreads();
+        return (active && lt < PN) ? b[lt ^ X] : v;
+    } else {
+        if (!active) return v;
+        const unsigned lo = xor_lane_u32<X>((unsigned)v, bp);
+        const unsigned hi = xor_lane_u32<X>((unsigned)(v >> 32), bp);
+        return ((unsigned long long)hi << 32) | lo;
+    }
+}
+
+template <int PN, int J>
+__device__ __forceinline__ void half_cleaners_g(unsigned long long& v, unsigned long long* xch, const bool* bit, int lt,
+                                                int bp32, int& half, bool active)
+{
+    const unsigned long long o = xor_exchange_g<PN, J>(v, xch, lt, bp32, half, active);
+    if (active && ((o < v) != bit[__builtin_ctz(J)])) v = o;
+    if constexpr (J > 1) half_cleaners_g<PN, J / 2>(v, xch, bit, lt, bp32, half, active);
+}
+
+template <int PN, int K>
+__device__ __forceinline__ void flip_levels_g(unsigned long long& v, unsigned long long* xch, const bool* bit, int lt,
+                                              int bp32, int bp63, int& half, bool active)
+{
+    const unsigned long long o = xor_exchange_g<PN, K - 1>(v, xch, lt, K - 1 == 63 ? bp63 : bp32, half, active);
+    if (active && ((o < v) != bit[__builtin_ctz(K) - 1])) v = o;
+    if constexpr (K >= 4) half_cleaners_g<PN, K / 4>(v, xch, bit, lt, bp32, half, active);
+    if constexpr (K < PN) flip_levels_g<PN, 2 * K>(v, xch, bit, lt, bp32, bp63, half, active);
+}
+
+template <int PN>
+__device__ __forceinline__ unsigned long long group_bitonic_sort(unsigned long long v, unsigned long long* xch, int lt,
+                                                                 bool active)
+{
+    bool bit[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) bit[i] = (lt >> i) & 1;
+    const int lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    const int bp32 = (lane ^ 32) << 2, bp63 = (lane ^ 63) << 2;
+    int half = 0;
+    flip_levels_g<PN, 2>(v, xch, bit, lt, bp32, bp63, half, active);
+    return v;
+}
+
+// Rank order of `island` (fitness f of its slot lt already loaded, for lt <
+// Pi) into order[0, Pi) when `active`; every group of the workgroup calls
+// it with the same Pi.  pow2(Pi) <= BLOCK; xch holds island_rank_bytes(Pi)
+// bytes per active group.  order is complete after the caller's next
+// barrier.
+template <int BLOCK>
+__device__ __forceinline__ void group_rank_order(const InstDesc& d, float f, int lt, bool active, int* order,
+                                                 unsigned long long* xch)
+{
+    const int Pi = d.Pi;
+    unsigned long long v = ~0ull;
+    if (active && lt < Pi) v = ((unsigned long long)fitness_order_key(f, d.prob.maximize) << 32) | (unsigned)lt;
+    if (Pi <= 64) v = group_bitonic_sort<64>(v, xch, lt, active);
+    else if constexpr (BLOCK >= 128) {
+        if (Pi <= 128) v = group_bitonic_sort<128>(v, xch, lt, active);
+        else if constexpr (BLOCK >= 256) {
+            if (Pi <= 256) v = group_bitonic_sort<256>(v, xch, lt, active);
+            else if constexpr (BLOCK >= 512) {
+                if (Pi <= 512) v = group_bitonic_sort<512>(v, xch, lt, active);
+                else if constexpr (BLOCK >= 1024) v = group_bitonic_sort<1024>(v, xch, lt, active);
+            }
+        }
+    }
+    if (active && lt < Pi) order[lt] = (int)(unsigned)v;
+}
+
 __device__ __forceinline__ float4 unit4(const U4& r)
 {
     return make_float4(unit_from_u32(r.x), unit_from_u32(r.y), unit_from_u32(r.z), unit_from_u32(r.w));
 }
 
+// A child row is built in columns of W consecutive keys (W = 4: float4, or
+// W = 2: float2 for workgroups with two keys per thread).  Key j of a row
+// always takes word j mod 4 of Philox block j / 4 of its stream, whatever W.
+template <int W>
+struct KeyCol;
+template <>
+struct KeyCol<4> {
+    using T = float4;
+    __device__ static T units(const U4& r, int) { return unit4(r); }
+    __device__ static T pick(const U4& r, int, const T& a, const T& b, float rho)
+    {
+        T x;
+        x.x = unit_from_u32(r.x) < rho ? a.x : b.x;
+        x.y = unit_from_u32(r.y) < rho ? a.y : b.y;
+        x.z = unit_from_u32(r.z) < rho ? a.z : b.z;
+        x.w = unit_from_u32(r.w) < rho ? a.w : b.w;
+        return x;
+    }
+};
+template <>
+struct KeyCol<2> {
+    using T = float2;
+    __device__ static T units(const U4& r, int c)
+    {
+        return (c & 1) ? make_float2(unit_from_u32(r.z), unit_from_u32(r.w))
+                       : make_float2(unit_from_u32(r.x), unit_from_u32(r.y));
+    }
+    __device__ static T pick(const U4& r, int c, const T& a, const T& b, float rho)
+    {
+        const uint32_t u0 = (c & 1) ? r.z : r.x, u1 = (c & 1) ? r.w : r.y;
+        T x;
+        x.x = unit_from_u32(u0) < rho ? a.x : b.x;
+        x.y = unit_from_u32(u1) < rho ? a.y : b.y;
+        return x;
+    }
+};
+
 // Builds child slot s of island `island` for the step from parent buffer
 // `cur` with step index t (generation 0 when init): the row goes to `out`
-// (global) and `kl` (LDS).  Elite and migrant slots are copied with their
-// fitness instead; then it returns false and nothing is left to decode.
-// `order` is the island's rank order (LDS-staged or global); `mig_order`
-// the rank order of the preceding island on the ring, or null to read it
-// from the partition's output.  mine (optional, nq <= BLOCK): the column
-// threadIdx.x this thread built, left unchanged for threads with none.
-template <int BLOCK>
+// (global) and `kl` (LDS), in columns of W keys.  Elite and migrant slots
+// are copied with their fitness instead; then it returns false and nothing
+// is left to decode.  `order` is the island's rank order (LDS-staged or
+// global); `mig_order` the rank order of the preceding island on the ring,
+// or null to read it from the partition's output.  mine (optional, columns
+// <= BLOCK): the column ltid this thread built (or copied), left unchanged
+// for threads with none.
+template <int BLOCK, int W = 4>
 __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s, int cur, int init, int t,
-                                            const int* order, const int* mig_order, float4* out, float4* kl,
-                                            float* fit_out, float4* mine = nullptr)
+                                            const int* order, const int* mig_order, typename KeyCol<W>::T* out,
+                                            typename KeyCol<W>::T* kl, float* fit_out,
+                                            typename KeyCol<W>::T* mine = nullptr)
 {
+    using Col = KeyCol<W>;
+    using V = typename Col::T;
     const int Pi = d.Pi, Ei = d.Ei, Mi = d.Mi, I = d.islands;
     const int M = d.prob.num_obj, ld = d.ld;
-    const int nq = (d.L + 3) >> 2;
+    const int nq = (d.L + W - 1) / W;
+    const int tid = ltid<BLOCK>();
+    constexpr int kShift = W == 4 ? 0 : 1;  // Philox block of column c: c >> kShift
     if (init) {
-        for (int c = threadIdx.x; c < nq; c += BLOCK) {
-            const float4 x = unit4(rng_block(d.seed_lo, d.seed_hi, island, 0, s, TAG_INIT, c));
+        for (int c = tid; c < nq; c += BLOCK) {
+            const V x = Col::units(rng_block(d.seed_lo, d.seed_hi, island, 0, s, TAG_INIT, c >> kShift), c);
             out[c] = x;
             kl[c] = x;
             if (mine) *mine = x;
@@ -195,14 +324,18 @@ __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s
         src = island * Pi + order[s];
     }
     if (src >= 0) {
-        const float4* in = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)src * ld);
-        for (int c = threadIdx.x; c < nq; c += BLOCK) out[c] = in[c];
-        if (threadIdx.x < M) fit_out[threadIdx.x] = d.fit[cur][(size_t)src * M + threadIdx.x];
+        const V* in = reinterpret_cast<const V*>(d.keys[cur] + (size_t)src * ld);
+        for (int c = tid; c < nq; c += BLOCK) {
+            const V x = in[c];
+            out[c] = x;
+            if (mine) *mine = x;
+        }
+        if (tid < M) fit_out[tid] = d.fit[cur][(size_t)src * M + tid];
         return false;
     }
     if (s >= Pi - Mi) {
-        for (int c = threadIdx.x; c < nq; c += BLOCK) {
-            const float4 x = unit4(rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_MUTANT, c));
+        for (int c = tid; c < nq; c += BLOCK) {
+            const V x = Col::units(rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_MUTANT, c >> kShift), c);
             out[c] = x;
             kl[c] = x;
             if (mine) *mine = x;
@@ -212,17 +345,12 @@ __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s
     const U4 pr = rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_PARENTS, 0);
     const int ep = island * Pi + order[(int)(pr.x % (uint32_t)Ei)];
     const int np = island * Pi + order[Ei + (int)(pr.y % (uint32_t)(Pi - Ei))];
-    const float4* pe = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)ep * ld);
-    const float4* pn = reinterpret_cast<const float4*>(d.keys[cur] + (size_t)np * ld);
+    const V* pe = reinterpret_cast<const V*>(d.keys[cur] + (size_t)ep * ld);
+    const V* pn = reinterpret_cast<const V*>(d.keys[cur] + (size_t)np * ld);
     const float rho = d.rho;
-    for (int c = threadIdx.x; c < nq; c += BLOCK) {
-        const U4 r = rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_COINS, c);
-        const float4 a = pe[c], b = pn[c];
-        float4 x;
-        x.x = unit_from_u32(r.x) < rho ? a.x : b.x;
-        x.y = unit_from_u32(r.y) < rho ? a.y : b.y;
-        x.z = unit_from_u32(r.z) < rho ? a.z : b.z;
-        x.w = unit_from_u32(r.w) < rho ? a.w : b.w;
+    for (int c = tid; c < nq; c += BLOCK) {
+        const U4 r = rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_COINS, c >> kShift);
+        const V x = Col::pick(r, c, pe[c], pn[c], rho);
         out[c] = x;
         kl[c] = x;
         if (mine) *mine = x;

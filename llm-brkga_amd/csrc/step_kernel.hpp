@@ -108,11 +108,15 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
     }
     __syncthreads();
 
-    if constexpr (KIND == KIND_TSP_COORDS && ITEMS == kCompactItems) {
-        // n <= 4 BLOCK: each thread ranks the one column it built, from
-        // registers, without waiting for the others
-        float4 mine = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (!build_child<BLOCK>(d, island, s, cur, init, t, own_order, mig_order, out, kl, fit_out, &mine)) return;
+    if constexpr (KIND == KIND_TSP_COORDS && (ITEMS == kCompactItems || ITEMS == 2)) {
+        // n <= ITEMS BLOCK: each thread ranks the one column of ITEMS keys it
+        // built, from registers, without waiting for the others
+        using V = typename KeyCol<ITEMS>::T;
+        V mine{};
+        V* out_w = reinterpret_cast<V*>(d.keys[nxt] + (size_t)gs * ld);
+        V* kl_w = reinterpret_cast<V*>(lds + sl.dec.keys_off);
+        if (!build_child<BLOCK, ITEMS>(d, island, s, cur, init, t, own_order, mig_order, out_w, kl_w, fit_out, &mine))
+            return;
         decode_tsp_coords_reg<BLOCK>(d.prob, lds, sl.dec, nb, fit_out, mine);
     } else {
         const bool child = build_child<BLOCK>(d, island, s, cur, init, t, own_order, mig_order, out, kl, fit_out);
@@ -143,6 +147,15 @@ template <int KIND, int BLOCK>
 inline hipError_t step_single(const InstDesc& d, int lds, int cur, int init, int rank, hipStream_t s,
                               hipEvent_t start, hipEvent_t stop)
 {
+    if constexpr (KIND == KIND_TSP_COORDS && BLOCK >= 512) {
+        // two keys per thread (float2 columns) when they fit
+        if (d.prob.n <= 2 * BLOCK) {
+            ensure_lds(k_step_one<KIND, BLOCK, 2>, lds);
+            launch_maybe_timed(k_step_one<KIND, BLOCK, 2>, dim3(d.pop), dim3(BLOCK), lds, s, start, stop, d, cur, init,
+                               rank);
+            return hipGetLastError();
+        }
+    }
     if constexpr (kind_sorts(KIND)) {
         if (d.prob.n <= kCompactItems * BLOCK) {
             ensure_lds(k_step_one<KIND, BLOCK, kCompactItems>, lds);

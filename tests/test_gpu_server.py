@@ -102,11 +102,12 @@ def test_server_alone_equals_in_mix(engine):
 
 
 def test_server_long_run_with_joins_matches_solo(engine):
-    """A request alone for many rounds, then two more joining while it runs:
-    every result still equals its solo run."""
+    """A request alone for many rounds (the single-population launches of a
+    lone request), then two more joining while it runs (batched ticks), then
+    alone again: every result still equals its solo run."""
     print("server long run with joins", flush=True)
     reqs = build_requests(engine)
-    picks = [(reqs[1][0], reqs[1][1], 300), (reqs[16][0], reqs[16][1], 25), (reqs[0][0], reqs[0][1], 30)]
+    picks = [(reqs[1][0], reqs[1][1], 3000), (reqs[16][0], reqs[16][1], 25), (reqs[0][0], reqs[0][1], 30)]
     refs = [solo_best(gp, prm, G) for gp, prm, G in picks]
     srv = brkga_amd.Server(engine, hbm_budget_bytes=512 << 20, ticks_per_round=2)
     first = srv.submit(*picks[0])
@@ -118,7 +119,23 @@ def test_server_long_run_with_joins_matches_solo(engine):
     for i, (res, ref) in enumerate(zip(results, refs)):
         check(res, ref, f"long run with joins, request {i}")
         assert res[2]["generations"] == picks[i][2]
-    assert st["completed"] == 3 and st["rounds"] >= 150
+    assert st["completed"] == 3
+    # the two short requests ran while the long one was resident
+    assert all(res[2]["done_us"] < results[0][2]["done_us"] for res in results[1:])
+
+
+def test_server_lone_requests_in_sequence(engine):
+    """Requests that each find the device empty (single-population rounds)
+    give their solo results, one after the other, for every kind."""
+    print("server lone requests", flush=True)
+    reqs = build_requests(engine)
+    srv = brkga_amd.Server(engine, hbm_budget_bytes=512 << 20)
+    for i in (0, 2, 3, 6, 8, 13, 16):
+        gp, prm, G = reqs[i]
+        res = srv.wait(srv.submit(gp, prm, G), timeout_ms=60000)
+        check(res, solo_best(gp, prm, G), f"lone request {i} {REQUESTS[i][:2]}")
+        assert res[2]["generations"] == G
+    srv.stop()
 
 
 def test_server_rejects_and_reports(engine):
