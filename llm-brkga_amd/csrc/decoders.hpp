@@ -23,24 +23,26 @@ namespace brkga {
 // are placed, when cnt[0, nb) holds n coordinate pairs (nb >= 2n); then no
 // barrier is needed against the ranking's reads of the items, and cnt[nb + 1]
 // (the long-bucket flag every thread reads after the placement) is never
-// overwritten.  EARLY: c[] was
-// loaded during the ranking; otherwise load() fills it here.
-template <int BLOCK, int ITEMS, bool CONTIG, bool EARLY, typename Load>
+// overwritten.  EARLY: c[] was loaded during the ranking; otherwise each
+// coordinate pair is read from xy just before it is written (no array of
+// ITEMS pairs held across the ranking: the widest decoders stay within
+// their registers).
+template <int BLOCK, int ITEMS, bool CONTIG, bool EARLY>
 __device__ __forceinline__ void tsp_coords_tail(const ProblemDesc& p, char* lds, const DecodeLayout& lay,
-                                                const SortScratch& ss, int nb, float2 (&c)[ITEMS],
-                                                const int (&pos)[ITEMS], float* fit_out, int* decoded, Load load)
+                                                const SortScratch& ss, int nb, const float2 (&c)[ITEMS],
+                                                const int (&pos)[ITEMS], float* fit_out, int* decoded,
+                                                const float2* xy = nullptr)
 {
     const int tid = ltid<BLOCK>();
     const int n = p.n;
     const bool in_cnt = nb >= 2 * n;
     float2* txy = reinterpret_cast<float2*>(in_cnt ? reinterpret_cast<char*>(ss.cnt) : lds + lay.items_off);
-    if constexpr (!EARLY) load();
     if (!in_cnt) __syncthreads();  // every ranking read of the items is done
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
         if (e < n) {
-            txy[pos[i]] = c[i];
+            txy[pos[i]] = EARLY ? c[i] : xy[e];
             if (decoded) decoded[pos[i]] = e;
         }
     }
@@ -102,7 +104,7 @@ __device__ __forceinline__ void decode_tsp_coords_reg(const ProblemDesc& p, char
         }
     };
     block_rank_by<BLOCK, true, W, true>([&](int i, int) { return kk[i]; }, n, nb, ss, pos, nullptr, load);
-    tsp_coords_tail<BLOCK, W, true, true>(p, lds, lay, ss, nb, c, pos, fit_out, nullptr, load);
+    tsp_coords_tail<BLOCK, W, true, true>(p, lds, lay, ss, nb, c, pos, fit_out, nullptr);
 }
 
 // CLEARED: the argsort counters were cleared (and a barrier passed) by the
@@ -132,20 +134,24 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
         // the ranking; with many they would hold too many registers.
         constexpr bool kEarly = ITEMS <= 8;
         const float2* xy = XY_LDS ? xy0 : reinterpret_cast<const float2*>(p.a);
-        float2 c[ITEMS];
+        float2 c[kEarly ? ITEMS : 1];
         int pos[ITEMS];
-        auto load = [&]() {
+        if constexpr (kEarly) {
+            auto load = [&]() {
 #pragma unroll
-            for (int i = 0; i < ITEMS; ++i) {
-                const int e = tid + i * BLOCK;
-                if (e < n) c[i] = xy[e];
-            }
-        };
-        if constexpr (kEarly)
+                for (int i = 0; i < ITEMS; ++i) {
+                    const int e = tid + i * BLOCK;
+                    if (e < n) c[i] = xy[e];
+                }
+            };
             block_rank<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, pos, nullptr, load);
-        else
-            block_rank<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, pos);
-        tsp_coords_tail<BLOCK, ITEMS, false, kEarly>(p, lds, lay, ss, nb, c, pos, fit_out, decoded, load);
+            tsp_coords_tail<BLOCK, ITEMS, false, true>(p, lds, lay, ss, nb, c, pos, fit_out, decoded);
+        } else {
+            // the keys stay in LDS until the tail (no perm is written)
+            block_rank<BLOCK, CLEARED, ITEMS, NoHook, true>(keys, n, nb, ss, pos);
+            const float2 none[ITEMS] = {};
+            tsp_coords_tail<BLOCK, ITEMS, false, false>(p, lds, lay, ss, nb, none, pos, fit_out, decoded, xy);
+        }
     } else if constexpr (KIND == KIND_TSP_MATRIX) {
         block_argsort<BLOCK, CLEARED, ITEMS>(keys, n, nb, ss, perm);
         // thread t sums the edges leaving positions [t*chunk, t*chunk+chunk)

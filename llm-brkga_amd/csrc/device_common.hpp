@@ -568,7 +568,12 @@ __device__ __forceinline__ int rank_elem(int tid, int i)
 // block_rank with the keys supplied by key_at(i, e) (the thread's i-th key,
 // element e) and the element mapping CONTIG; positions and perm follow the
 // same mapping.
-template <int BLOCK, bool CLEARED, int ITEMS, bool CONTIG, typename KeyAt, typename Hook = NoHook>
+// KEYS_STAY: key_at(i, e) still returns the key during the placement and
+// the ranking (the keys are not overwritten, perm is null); the order keys
+// are then recomputed there instead of held in registers, for the widest
+// decoders (16 keys per thread).
+template <int BLOCK, bool CLEARED, int ITEMS, bool CONTIG, typename KeyAt, typename Hook = NoHook,
+          bool KEYS_STAY = false>
 __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const SortScratch& s,
                                               int (&pos)[ITEMS], int* perm = nullptr, Hook pre_rank = Hook())
 {
@@ -579,16 +584,31 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
     }
     // rng[i]: the bucket, then (bucket start | bucket size << 16); both are
     // below 2^15 since n <= ITEMS * BLOCK <= kSortItems * 1024
-    int rng[ITEMS], slot[ITEMS];
-    uint32_t ok[ITEMS];
+    // With 16 keys per thread the slot a key got in its bucket travels in
+    // the upper half of rng (bucket < 2^11, slot < 2^14), which keeps the
+    // widest decoders within the registers of a 1024-thread workgroup.
+    constexpr bool kPack = ITEMS >= 16;
+    int rng[ITEMS], slot[kPack ? 1 : ITEMS];
+    constexpr bool kReload = kPack && KEYS_STAY;
+    uint32_t ok[kReload ? 1 : ITEMS];
+    auto order_of = [&](int i, int e) -> uint32_t {
+        if constexpr (kReload) return key_order(key_at(i, e));
+        else return ok[i];
+    };
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
         if (e < n) {
             const float k = key_at(i, e);
-            ok[i] = key_order(k);
-            rng[i] = key_bucket(k, nb);
-            slot[i] = atomicAdd(&s.cnt[rng[i]], 1);
+            if constexpr (!kReload) ok[i] = key_order(k);
+            const int b = key_bucket(k, nb);
+            const int sl = atomicAdd(&s.cnt[b], 1);
+            if constexpr (kPack) {
+                rng[i] = b | (sl << 16);
+            } else {
+                rng[i] = b;
+                slot[i] = sl;
+            }
         }
     }
     __syncthreads();
@@ -598,9 +618,11 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
     for (int i = 0; i < ITEMS; ++i) {
         const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
         if (e < n) {
-            const int st = s.cnt[rng[i]];
-            const int en = s.cnt[rng[i] + 1];
-            s.items[st + slot[i]] = ((unsigned long long)ok[i] << 32) | (unsigned)e;
+            const int b = kPack ? (rng[i] & 0xFFFF) : rng[i];
+            const int sl = kPack ? (rng[i] >> 16) : slot[kPack ? 0 : i];
+            const int st = s.cnt[b];
+            const int en = s.cnt[b + 1];
+            s.items[st + sl] = ((unsigned long long)order_of(i, e) << 32) | (unsigned)e;
             rng[i] = st | ((en - st) << 16);
             longest = en - st > longest ? en - st : longest;
         }
@@ -617,7 +639,7 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
         for (int i = 0; i < ITEMS; ++i) {
             const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
             if (e < n) {
-                const int p = sorted_items_find(s.items, n, ((unsigned long long)ok[i] << 32) | (unsigned)e);
+                const int p = sorted_items_find(s.items, n, ((unsigned long long)order_of(i, e) << 32) | (unsigned)e);
                 pos[i] = p;
                 if (perm) perm[p] = e;
             }
@@ -635,7 +657,7 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
             const int sz = rng[i] >> 16;
             int r = 0;
             if (sz > 1) {
-                const unsigned long long mine = ((unsigned long long)ok[i] << 32) | (unsigned)e;
+                const unsigned long long mine = ((unsigned long long)order_of(i, e) << 32) | (unsigned)e;
                 const unsigned long long* b = s.items + st;
                 const unsigned long long x0 = b[0];
                 const unsigned long long x1 = b[1];
@@ -650,19 +672,20 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
     }
 }
 
-template <int BLOCK, bool CLEARED = false, int ITEMS = kSortItems, typename Hook = NoHook>
+template <int BLOCK, bool CLEARED = false, int ITEMS = kSortItems, typename Hook = NoHook, bool KEYS_STAY = false>
 __device__ __forceinline__ void block_rank(const float* keys, int n, int nb, const SortScratch& s,
                                            int (&pos)[ITEMS], int* perm = nullptr, Hook pre_rank = Hook())
 {
-    block_rank_by<BLOCK, CLEARED, ITEMS, false>([&](int, int e) { return keys[e]; }, n, nb, s, pos, perm,
-                                                pre_rank);
+    auto key_at = [&](int, int e) { return keys[e]; };
+    block_rank_by<BLOCK, CLEARED, ITEMS, false, decltype(key_at), Hook, KEYS_STAY>(key_at, n, nb, s, pos, perm,
+                                                                                  pre_rank);
 }
 
 // perm[p] = index of the element at position p of the stable ascending
 // order of keys[0..n) (block_rank).  keys and perm live in LDS and may
 // overlay each other.  The result is ready after the final barrier.
 template <int BLOCK, bool CLEARED = false, int ITEMS = kSortItems>
-__device__ void block_argsort(const float* keys, int n, int nb, const SortScratch& s, int* perm)
+__device__ __forceinline__ void block_argsort(const float* keys, int n, int nb, const SortScratch& s, int* perm)
 {
     int pos[ITEMS];
     block_rank<BLOCK, CLEARED, ITEMS>(keys, n, nb, s, pos, perm);

@@ -232,7 +232,7 @@ __global__ __launch_bounds__(BLOCK) void k_tspj_local_search(const InstDesc* __r
         keys[n + i] = row[n + i];
     }
     __syncthreads();
-    decode_block<KIND_TSPJ, BLOCK>(d.prob, lds, lay, nb, &s_fit, nullptr);
+    [[clang::always_inline]] decode_block<KIND_TSPJ, BLOCK>(d.prob, lds, lay, nb, &s_fit, nullptr);
     __syncthreads();
     const float old = d.fit[b][slot];
     if (s_fit < old) {

@@ -122,7 +122,8 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
         const bool child = build_child<BLOCK>(d, island, s, cur, init, t, own_order, mig_order, out, kl, fit_out);
         if (!child) return;
         __syncthreads();
-        decode_block<KIND, BLOCK, true, false, ITEMS>(d.prob, lds, sl.dec, nb, fit_out, nullptr);
+        // inlined here: a call would keep a stack frame in scratch for the widest decoders
+        [[clang::always_inline]] decode_block<KIND, BLOCK, true, false, ITEMS>(d.prob, lds, sl.dec, nb, fit_out, nullptr);
     }
 }
 
