@@ -27,7 +27,7 @@ namespace brkga {
 // coordinate pair is read from xy just before it is written (no array of
 // ITEMS pairs held across the ranking: the widest decoders stay within
 // their registers).
-template <int BLOCK, int ITEMS, bool CONTIG, bool EARLY>
+template <int BLOCK, int ITEMS, bool CONTIG, bool EARLY, bool BATCH = false>
 __device__ __forceinline__ void tsp_coords_tail(const ProblemDesc& p, char* lds, const DecodeLayout& lay,
                                                 const SortScratch& ss, int nb, const float2 (&c)[ITEMS],
                                                 const int (&pos)[ITEMS], float* fit_out, int* decoded,
@@ -47,15 +47,22 @@ __device__ __forceinline__ void tsp_coords_tail(const ProblemDesc& p, char* lds,
         }
     }
     __syncthreads();
+    // BATCH (as in block_rank_by): the two ends of four edges are read
+    // before any is used; otherwise one edge at a time
+    constexpr int G = BATCH ? (ITEMS < 4 ? ITEMS : 4) : 1;
     FxAcc acc;
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        const int q = tid + i * BLOCK;
-        if (q < n) {
-            const float2 a = txy[q];
-            const float2 b = txy[q + 1 < n ? q + 1 : 0];
-            fx_add(acc, edge_xy(a.x, a.y, b.x, b.y), p.scale0);
+    for (int i0 = 0; i0 < ITEMS; i0 += G) {
+        float2 a[G], b[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const int q = tid + (i0 + j) * BLOCK;
+            a[j] = txy[q < n ? q : 0];
+            b[j] = txy[q + 1 < n ? q + 1 : 0];
         }
+#pragma unroll
+        for (int j = 0; j < G; ++j)
+            if (tid + (i0 + j) * BLOCK < n) fx_add(acc, edge_xy(a[j].x, a[j].y, b[j].x, b[j].y), p.scale0);
     }
     const long long tot = block_sum_fx<BLOCK>(fx_value(acc), ss);
     if (tid == 0 && fit_out) fit_out[0] = fx_to_float(tot, p.inv0);
@@ -103,8 +110,10 @@ __device__ __forceinline__ void decode_tsp_coords_reg(const ProblemDesc& p, char
             if (e < n) c[i] = xy[e];
         }
     };
-    block_rank_by<BLOCK, true, W, true>([&](int i, int) { return kk[i]; }, n, nb, ss, pos, nullptr, load);
-    tsp_coords_tail<BLOCK, W, true, true>(p, lds, lay, ss, nb, c, pos, fit_out, nullptr);
+    auto key_at = [&](int i, int) { return kk[i]; };
+    block_rank_by<BLOCK, true, W, true, decltype(key_at), decltype(load), false, true>(key_at, n, nb, ss, pos, nullptr,
+                                                                                      load);
+    tsp_coords_tail<BLOCK, W, true, true, true>(p, lds, lay, ss, nb, c, pos, fit_out, nullptr);
 }
 
 // CLEARED: the argsort counters were cleared (and a barrier passed) by the

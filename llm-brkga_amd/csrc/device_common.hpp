@@ -572,8 +572,14 @@ __device__ __forceinline__ int rank_elem(int tid, int i)
 // the ranking (the keys are not overwritten, perm is null); the order keys
 // are then recomputed there instead of held in registers, for the widest
 // decoders (16 keys per thread).
+// BATCH: each thread issues the LDS operations of a phase together and
+// waits once, which shortens a workgroup's chain of dependent LDS round
+// trips at the cost of registers: for latency-bound launches (one
+// population, a few workgroups per CU).  Otherwise each key's accesses are
+// waited for in turn, which keeps the registers, and so the occupancy, of
+// throughput-bound launches (batched and served fleets).
 template <int BLOCK, bool CLEARED, int ITEMS, bool CONTIG, typename KeyAt, typename Hook = NoHook,
-          bool KEYS_STAY = false>
+          bool KEYS_STAY = false, bool BATCH = false>
 __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const SortScratch& s,
                                               int (&pos)[ITEMS], int* perm = nullptr, Hook pre_rank = Hook())
 {
@@ -588,6 +594,10 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
     // the upper half of rng (bucket < 2^11, slot < 2^14), which keeps the
     // widest decoders within the registers of a 1024-thread workgroup.
     constexpr bool kPack = ITEMS >= 16;
+    // kSerial: one key at a time, each LDS access waited for where its
+    // result is used (also the widest forms, which have no registers to
+    // spare); otherwise a thread's LDS accesses of a phase go together.
+    constexpr bool kSerial = !BATCH || kPack || (BLOCK >= 1024 && ITEMS >= 8);
     int rng[ITEMS], slot[kPack ? 1 : ITEMS];
     constexpr bool kReload = kPack && KEYS_STAY;
     uint32_t ok[kReload ? 1 : ITEMS];
@@ -595,36 +605,78 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
         if constexpr (kReload) return key_order(key_at(i, e));
         else return ok[i];
     };
+    if constexpr (kSerial) {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
-        if (e < n) {
-            const float k = key_at(i, e);
-            if constexpr (!kReload) ok[i] = key_order(k);
-            const int b = key_bucket(k, nb);
-            const int sl = atomicAdd(&s.cnt[b], 1);
-            if constexpr (kPack) {
-                rng[i] = b | (sl << 16);
-            } else {
-                rng[i] = b;
-                slot[i] = sl;
+        for (int i = 0; i < ITEMS; ++i) {
+            const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
+            if (e < n) {
+                const float k = key_at(i, e);
+                if constexpr (!kReload) ok[i] = key_order(k);
+                const int b = key_bucket(k, nb);
+                const int sl = atomicAdd(&s.cnt[b], 1);
+                if constexpr (kPack) {
+                    rng[i] = b | (sl << 16);
+                } else {
+                    rng[i] = b;
+                    slot[i] = sl;
+                }
             }
+        }
+    } else {
+        // Each thread issues a phase's LDS operations back to back and waits
+        // once (no result is used inside the branch that issues it): the
+        // keys are read first, then every key takes its slot.  An element
+        // past n keeps bucket nb + 2, a cleared spare counter the placement
+        // reads and the scan never touches.
+        float kf[ITEMS];
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
+            kf[i] = e < n ? key_at(i, e) : 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
+            ok[i] = key_order(kf[i]);
+            rng[i] = e < n ? key_bucket(kf[i], nb) : nb + 2;
+            slot[i] = 0;
+            if (e < n) slot[i] = atomicAdd(&s.cnt[rng[i]], 1);
         }
     }
     __syncthreads();
     scan_bucket_counts<BLOCK>(s.cnt, nb, n, reinterpret_cast<int*>(s.red));
     int longest = 0;
+    if constexpr (kSerial) {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-        const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
-        if (e < n) {
-            const int b = kPack ? (rng[i] & 0xFFFF) : rng[i];
-            const int sl = kPack ? (rng[i] >> 16) : slot[kPack ? 0 : i];
-            const int st = s.cnt[b];
-            const int en = s.cnt[b + 1];
-            s.items[st + sl] = ((unsigned long long)order_of(i, e) << 32) | (unsigned)e;
-            rng[i] = st | ((en - st) << 16);
-            longest = en - st > longest ? en - st : longest;
+        for (int i = 0; i < ITEMS; ++i) {
+            const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
+            if (e < n) {
+                const int b = kPack ? (rng[i] & 0xFFFF) : rng[i];
+                const int sl = kPack ? (rng[i] >> 16) : slot[kPack ? 0 : i];
+                const int st = s.cnt[b];
+                const int en = s.cnt[b + 1];
+                s.items[st + sl] = ((unsigned long long)order_of(i, e) << 32) | (unsigned)e;
+                rng[i] = st | ((en - st) << 16);
+                longest = en - st > longest ? en - st : longest;
+            }
+        }
+    } else {
+        // every bucket start and end is read before the first item is
+        // placed; elements past n read the spare counters and keep rng = 0
+        int st[ITEMS], en[ITEMS];
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            st[i] = s.cnt[rng[i]];
+            en[i] = s.cnt[rng[i] + 1];
+        }
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
+            const bool valid = e < n;
+            if (valid) s.items[st[i] + slot[i]] = ((unsigned long long)ok[i] << 32) | (unsigned)e;
+            const int sz = en[i] - st[i];
+            rng[i] = valid ? (st[i] | (sz << 16)) : 0;
+            longest = valid && sz > longest ? sz : longest;
         }
     }
     int* const long_flag = s.flag ? s.flag : s.cnt + nb + 1;
@@ -646,29 +698,79 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
         }
         return;
     }
-    // A key alone in its bucket reads nothing; otherwise the first four
-    // items of the bucket are read as independent loads and longer buckets
-    // finish in a loop.  A key compared with itself counts nothing.
+    if constexpr (kSerial) {
+        // A key alone in its bucket reads nothing; otherwise the first four
+        // items of the bucket are read as independent loads and longer
+        // buckets finish in a loop.  A key compared with itself counts
+        // nothing.
+#pragma unroll
+        for (int i = 0; i < ITEMS; ++i) {
+            const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
+            if (e < n) {
+                const int st = rng[i] & 0xFFFF;
+                const int sz = rng[i] >> 16;
+                int r = 0;
+                if (sz > 1) {
+                    const unsigned long long mine = ((unsigned long long)order_of(i, e) << 32) | (unsigned)e;
+                    const unsigned long long* b = s.items + st;
+                    const unsigned long long x0 = b[0];
+                    const unsigned long long x1 = b[1];
+                    const unsigned long long x2 = sz > 2 ? b[2] : ~0ull;
+                    const unsigned long long x3 = sz > 3 ? b[3] : ~0ull;
+                    r = (x0 < mine) + (x1 < mine) + (x2 < mine) + (x3 < mine);
+                    for (int q = 4; q < sz; ++q) r += b[q] < mine ? 1 : 0;
+                }
+                pos[i] = st + r;
+                if (perm) perm[st + r] = e;
+            }
+        }
+        return;
+    }
+    // A key alone in its bucket reads nothing; every other key reads the
+    // first four items from its bucket start, the loads of four keys issued
+    // before any is used (the items array has four spare entries, so the
+    // reads stay inside it).  An item past the bucket end counts nothing,
+    // nor does the key compared with itself.  Buckets longer than four
+    // finish in a loop afterwards.
+    constexpr int G = ITEMS < 4 ? ITEMS : 4;
+    int r[ITEMS];
+#pragma unroll
+    for (int i0 = 0; i0 < ITEMS; i0 += G) {
+        unsigned long long x[G][4];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const unsigned long long* b = s.items + (rng[i0 + j] & 0xFFFF);
+            // unread slots hold the largest item, which counts nothing
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[j][q] = ~0ull;
+            if ((rng[i0 + j] >> 16) > 1) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) x[j][q] = b[q];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const int i = i0 + j;
+            const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
+            const int sz = rng[i] >> 16;
+            const unsigned long long mine = ((unsigned long long)ok[i] << 32) | (unsigned)e;
+            int c = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) c += (q < sz && x[j][q] < mine) ? 1 : 0;
+            r[i] = c;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
         const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
-        if (e < n) {
-            const int st = rng[i] & 0xFFFF;
-            const int sz = rng[i] >> 16;
-            int r = 0;
-            if (sz > 1) {
-                const unsigned long long mine = ((unsigned long long)order_of(i, e) << 32) | (unsigned)e;
-                const unsigned long long* b = s.items + st;
-                const unsigned long long x0 = b[0];
-                const unsigned long long x1 = b[1];
-                const unsigned long long x2 = sz > 2 ? b[2] : ~0ull;
-                const unsigned long long x3 = sz > 3 ? b[3] : ~0ull;
-                r = (x0 < mine) + (x1 < mine) + (x2 < mine) + (x3 < mine);
-                for (int q = 4; q < sz; ++q) r += b[q] < mine ? 1 : 0;
-            }
-            pos[i] = st + r;
-            if (perm) perm[st + r] = e;
+        const int st = rng[i] & 0xFFFF;
+        const int sz = rng[i] >> 16;
+        if (sz > 4) {
+            const unsigned long long mine = ((unsigned long long)ok[i] << 32) | (unsigned)e;
+            for (int q = 4; q < sz; ++q) r[i] += s.items[st + q] < mine ? 1 : 0;
         }
+        pos[i] = st + r[i];
+        if (perm && e < n) perm[st + r[i]] = e;
     }
 }
 

@@ -153,7 +153,7 @@ __host__ __device__ inline DecodeLayout decode_layout(int kind, int n, int L, in
     off += 17 * 8;
     off = round_up_int(off, 16);
     d.items_off = off;
-    off += n * 8;
+    off += (n + 4) * 8;  // four spare items: the ranking reads four from any bucket start
     off = round_up_int(off, 16);
     // the decoded order overlays the keys: the argsort reads each key for
     // the last time before it writes the first position
