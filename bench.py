@@ -268,6 +268,7 @@ def batched_leg(brkga_amd, workloads, eng, gens):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": B},
         "buckets": batch.buckets(),
+        "lanes": batch.lanes(),
     }
     batch.close()
     for p in probs:
@@ -369,7 +370,7 @@ def served_leg(brkga_amd, workloads, eng, count, rates, budget):
         p50, p99 = percentiles(p_lat)
         rows.append({"rate_req_per_s": rate if rate > 0 else "all at t=0", "requests": count,
                      "served": {"solves_per_s": round(count / s_span, 2), "p50_ms": s50, "p99_ms": s99,
-                                "peak_resident_bytes": st["peak_bytes"], "ticks": st["ticks"],
+                                "peak_resident_bytes": st["peak_bytes"], "ticks": st["ticks"], "lanes": st["lanes"],
                                 "rounds": st["rounds"], "worker_wait_ms": round(st["wait_us"] / 1000, 2),
                                 "worker_host_ms": round(st["host_us"] / 1000, 2)},
                      "plain": {"solves_per_s": round(count / p_span, 2), "p50_ms": p50, "p99_ms": p99},

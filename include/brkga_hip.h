@@ -419,6 +419,11 @@ brkga_status brkga_batch_population(brkga_batch_t batch, int32_t index, brkga_po
 brkga_status brkga_batch_num_buckets(brkga_batch_t batch, int32_t* count);
 brkga_status brkga_batch_bucket_info(brkga_batch_t batch, int32_t bucket, int32_t* kind, int32_t* n_min,
                                      int32_t* n_max, int32_t* members, int32_t* block, int32_t* chunk);
+/* Streams the batched launches are spread over: the caller's stream plus
+ * those a timed probe found running concurrently with it (1 .. 4), or the
+ * count set by the environment variable BRKGA_FLEET_LANES (1 .. 4).  1
+ * until the first batched step. */
+brkga_status brkga_batch_lanes(brkga_batch_t batch, int32_t* lanes);
 
 /* ------------------------------------------------------------ serving --- */
 
@@ -458,6 +463,8 @@ typedef struct {
     uint64_t rounds;  /* admission rounds */
     double wait_us;   /* worker time spent waiting for ticks to finish */
     double host_us;   /* worker time spent on admission, membership and results */
+    int32_t lanes;    /* streams the batched ticks are spread over (as brkga_batch_lanes; 0 while no
+                         batched tick has run, e.g. when every round ran its requests on their own streams) */
 } brkga_server_stats;
 
 /* Starts the worker thread (config may be NULL for defaults). */

@@ -116,7 +116,7 @@ class CServerStats(ctypes.Structure):
                 ("rejected", ctypes.c_uint64), ("resident", ctypes.c_int32), ("pending", ctypes.c_int32),
                 ("bytes_in_use", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64),
                 ("budget_bytes", ctypes.c_uint64), ("ticks", ctypes.c_uint64), ("rounds", ctypes.c_uint64),
-                ("wait_us", ctypes.c_double), ("host_us", ctypes.c_double)]
+                ("wait_us", ctypes.c_double), ("host_us", ctypes.c_double), ("lanes", ctypes.c_int32)]
 
 
 _lib = None
@@ -180,6 +180,7 @@ _SIGNATURES = {
     "brkga_batch_population": ([_P, _I32, ctypes.POINTER(_P)], ctypes.c_int),
     "brkga_batch_num_buckets": ([_P, _PI], ctypes.c_int),
     "brkga_batch_bucket_info": ([_P, _I32, _PI, _PI, _PI, _PI, _PI, _PI], ctypes.c_int),
+    "brkga_batch_lanes": ([_P, _PI], ctypes.c_int),
     "brkga_server_start": ([_P, ctypes.POINTER(CServerConfig), ctypes.POINTER(_P)], ctypes.c_int),
     "brkga_server_submit": ([_P, _P, ctypes.POINTER(CParams), _I32, _I32, ctypes.POINTER(ctypes.c_uint64)],
                             ctypes.c_int),
@@ -587,6 +588,12 @@ class Batch:
             out.append({"kind": kind, "n_min": n_min, "n_max": n_max, "members": count, "block": block,
                         "chunk": chunk})
         return out
+
+    def lanes(self):
+        """Streams the batched launches are spread over (1 until the first batched step)."""
+        c = ctypes.c_int32()
+        _check(lib().brkga_batch_lanes(self.handle, ctypes.byref(c)))
+        return c.value
 
     def close(self):
         if self.handle:

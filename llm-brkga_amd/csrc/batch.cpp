@@ -147,6 +147,13 @@ brkga_status brkga_batch_num_buckets(brkga_batch_t batch, int32_t* count)
     return BRKGA_OK;
 }
 
+brkga_status brkga_batch_lanes(brkga_batch_t batch, int32_t* lanes)
+{
+    if (!batch || !lanes) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
+    *lanes = batch->fleet->lanes();
+    return BRKGA_OK;
+}
+
 brkga_status brkga_batch_bucket_info(brkga_batch_t batch, int32_t bucket, int32_t* kind, int32_t* n_min,
                                      int32_t* n_max, int32_t* members, int32_t* block, int32_t* chunk)
 {

@@ -2,6 +2,7 @@
 #include "fleet.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <numeric>
 #include <tuple>
@@ -25,6 +26,7 @@ constexpr double kChildFixedWork = 1024.0;
 // and how many candidate streams are tried for the kLanes - 1 lanes.
 constexpr int kProbeIters = 15000;
 constexpr int kLaneCandidates = 8;
+constexpr int kProbeRounds = 3;
 constexpr int kChunks[] = {1, 2, 4, 8, 16};
 
 // The autotuner keeps the fastest of this many timed rounds per launch
@@ -320,11 +322,43 @@ void Fleet::assign_lanes()
     }
 }
 
+// Lanes wanted by the environment: BRKGA_FLEET_LANES = 1 .. kLanes fixes
+// the count (no probing); anything else, or unset, leaves it to the probes.
+static int fixed_lanes()
+{
+    const char* v = std::getenv("BRKGA_FLEET_LANES");
+    if (!v || !*v) return 0;
+    char* end = nullptr;
+    const long n = std::strtol(v, &end, 10);
+    return (end && *end == 0 && n >= 1 && n <= kLanes) ? (int)n : 0;
+}
+
 brkga_status Fleet::ensure_lanes(hipStream_t s)
 {
-    if (fork_) return BRKGA_OK;
-    BRKGA_HIP_TRY(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
-    for (int k = 1; k < kLanes; ++k) BRKGA_HIP_TRY(hipEventCreateWithFlags(&join_[k], hipEventDisableTiming));
+    if (calibrated_) return BRKGA_OK;
+    // events first (kept if the calibration below fails; it runs again on
+    // the next call)
+    if (!fork_) BRKGA_HIP_TRY(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+    for (int k = 1; k < kLanes; ++k)
+        if (!join_[k]) BRKGA_HIP_TRY(hipEventCreateWithFlags(&join_[k], hipEventDisableTiming));
+    const int fixed = fixed_lanes();
+    if (fixed > 0) {
+        int found = 1;
+        hipError_t e = hipSuccess;
+        for (; found < fixed && e == hipSuccess; ++found) {
+            if (lanes_[found]) continue;
+            e = hipStreamCreateWithFlags(&lanes_[found], hipStreamNonBlocking);
+        }
+        if (e != hipSuccess) return hip_error(e, "stream lanes");
+        nlanes_ = fixed;
+        calibrated_ = true;
+        return BRKGA_OK;
+    }
+    for (int k = 1; k < kLanes; ++k)
+        if (lanes_[k]) {
+            (void)hipStreamDestroy(lanes_[k]);
+            lanes_[k] = nullptr;
+        }
     float* probe = nullptr;
     hipEvent_t e0 = nullptr, ea = nullptr, eb = nullptr;
     std::vector<hipStream_t> rejected;
@@ -349,8 +383,19 @@ brkga_status Fleet::ensure_lanes(hipStream_t s)
         *ms = std::max(ta, tb);
         return r;
     };
+    // medians of kProbeRounds timings, so one disturbed probe (other work
+    // on the device while the fleet is created) decides nothing alone
+    auto median_timed = [&](hipStream_t a, hipStream_t b, float* ms) {
+        float t[kProbeRounds];
+        hipError_t r = hipSuccess;
+        for (int i = 0; i < kProbeRounds && r == hipSuccess; ++i) r = timed(a, b, &t[i]);
+        std::sort(t, t + kProbeRounds);
+        *ms = t[kProbeRounds / 2];
+        return r;
+    };
     float alone = 0.0f;
-    for (int w = 0; w < 2 && e == hipSuccess; ++w) e = timed(s, nullptr, &alone);  // the first one warms up
+    if (e == hipSuccess) e = timed(s, nullptr, &alone);  // warm-up
+    if (e == hipSuccess) e = median_timed(s, nullptr, &alone);
     int found = 1;
     for (int c = 0; c < kLaneCandidates && found < kLanes && e == hipSuccess; ++c) {
         hipStream_t cand = nullptr;
@@ -359,7 +404,7 @@ brkga_status Fleet::ensure_lanes(hipStream_t s)
         bool concurrent = true;
         for (int k = 0; k < found && concurrent && e == hipSuccess; ++k) {
             float ms = 0.0f;
-            e = timed(k == 0 ? s : lanes_[k], cand, &ms);
+            e = median_timed(k == 0 ? s : lanes_[k], cand, &ms);
             concurrent = ms < 1.5f * alone;
         }
         if (concurrent && e == hipSuccess) lanes_[found++] = cand;
@@ -370,8 +415,16 @@ brkga_status Fleet::ensure_lanes(hipStream_t s)
     if (ea) (void)hipEventDestroy(ea);
     if (eb) (void)hipEventDestroy(eb);
     if (probe) (void)hipFree(probe);
+    if (e != hipSuccess) {
+        for (int k = 1; k < found; ++k) {
+            (void)hipStreamDestroy(lanes_[k]);
+            lanes_[k] = nullptr;
+        }
+        nlanes_ = 1;
+        return hip_error(e, "stream lane calibration");
+    }
     nlanes_ = found;
-    if (e != hipSuccess) return hip_error(e, "stream lane calibration");
+    calibrated_ = true;
     return BRKGA_OK;
 }
 

@@ -87,6 +87,8 @@ public:
     brkga_status autotune_new(int reps, hipStream_t s);
 
     const std::vector<FleetBucket>& buckets() const { return buckets_; }
+    // Streams the ticks spread their launches over (1 until calibrated).
+    int lanes() const { return nlanes_; }
     const std::vector<brkga_pop_s*>& members() const { return members_; }
     size_t size() const { return members_.size(); }
 
@@ -96,7 +98,9 @@ private:
     brkga_status tune(int reps, bool only_new, hipStream_t s);
     brkga_status reserve(void** p, size_t* cap, size_t bytes, const char* what);
     // Creates the stream lanes once, keeping only streams that run
-    // concurrently with s and with each other (timed probes).
+    // concurrently with s and with each other (medians of timed probes),
+    // or BRKGA_FLEET_LANES of them when that variable is 1 .. kLanes.  A
+    // failed calibration leaves one lane and runs again on the next call.
     brkga_status ensure_lanes(hipStream_t s);
     void assign_lanes();
     void drop_graphs();
@@ -126,6 +130,7 @@ private:
     std::map<int, hipGraphExec_t> graphs_;
     hipStream_t lanes_[kLanes] = {};  // [0] unused: lane 0 is the caller's stream
     int nlanes_ = 1;                  // lanes in use: the caller's stream and lanes_[1, nlanes_)
+    bool calibrated_ = false;
     hipEvent_t fork_ = nullptr;
     hipEvent_t join_[kLanes] = {};
     int nsga_lane_ = 0;

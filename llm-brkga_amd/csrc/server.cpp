@@ -466,6 +466,7 @@ void brkga_server_s::run()
             {
                 std::lock_guard<std::mutex> lock(mu);
                 stats.ticks += cfg.ticks_per_round;
+                stats.lanes = fleet->lanes();
             }
             // tick 1 builds generation 0, tick g + 1 completes step g
             std::vector<std::unique_ptr<Request>> keep;

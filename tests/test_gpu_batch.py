@@ -141,3 +141,34 @@ def test_batch_member_matches_oracle(engine):
         batch.evolve(1, "launch")
         ost.step()
     batch.close()
+
+
+@pytest.mark.parametrize("lanes", ["1", "3"])
+def test_batch_fixed_lanes(engine, lanes, monkeypatch):
+    """BRKGA_FLEET_LANES fixes the stream count (no probing); results do not
+    depend on it, and the count is reported."""
+    print(f"batch with BRKGA_FLEET_LANES={lanes}", flush=True)
+    monkeypatch.setenv("BRKGA_FLEET_LANES", lanes)
+    probs, params = build(engine)
+    G = 4
+    solo = solo_states(probs, params, G)
+    batch = brkga_amd.Batch(engine, probs, params)
+    batch.initialize("launch")
+    batch.evolve(G, "launch")
+    torch.cuda.synchronize()
+    assert batch.lanes() == int(lanes)
+    assert_same(batch, solo, G, f"{lanes} lanes")
+    batch.close()
+
+
+def test_batch_probed_lanes(engine, monkeypatch):
+    """Without the variable the probes choose 1 to 4 lanes."""
+    print("batch with probed lanes", flush=True)
+    monkeypatch.delenv("BRKGA_FLEET_LANES", raising=False)
+    probs, params = build(engine)
+    batch = brkga_amd.Batch(engine, probs, params)
+    batch.initialize("launch")
+    batch.evolve(2, "launch")
+    torch.cuda.synchronize()
+    assert 1 <= batch.lanes() <= 4
+    batch.close()

@@ -86,6 +86,7 @@ def test_server_results_match_solo(engine, budget_mib, max_resident, ticks, tune
         assert res[2]["submit_us"] <= res[2]["admit_us"] <= res[2]["done_us"]
     assert st["completed"] == len(reqs) and st["submitted"] == len(reqs)
     assert st["peak_bytes"] <= st["budget_bytes"]
+    assert 0 <= st["lanes"] <= 4  # 0: every round ran the requests on their own streams
 
 
 def test_server_alone_equals_in_mix(engine):
