@@ -46,7 +46,8 @@ typedef enum {
     BRKGA_ERR_BUSY = 4,        /* result not ready yet */
     BRKGA_ERR_DEADLINE = 5,    /* request retired without completing */
     BRKGA_ERR_NOT_FOUND = 6,   /* unknown request id */
-    BRKGA_ERR_NO_DEVICE = 7    /* no usable GPU */
+    BRKGA_ERR_NO_DEVICE = 7,   /* no usable GPU */
+    BRKGA_ERR_CANCELLED = 8    /* request cancelled before its deadline (brkga_server_cancel) */
 } brkga_status;
 
 /* Problem kinds (decoders).  BRKGA runs the single-objective kinds,
@@ -486,6 +487,16 @@ brkga_status brkga_server_wait(brkga_server_t server, uint64_t request_id, int32
                                int32_t* decoded, brkga_request_info* info);
 
 brkga_status brkga_server_stats_get(brkga_server_t server, brkga_server_stats* stats);
+
+/* Cancels a request that has not reached its deadline.  A pending request
+ * leaves the queue at once; a resident one leaves the device at the start
+ * of the worker's next admission round (the generations it ran are dropped
+ * and its HBM returns to the budget).  brkga_server_wait then returns
+ * BRKGA_ERR_CANCELLED for it, with its info.  A request that already
+ * finished keeps its result (no effect, BRKGA_OK); BRKGA_ERR_NOT_FOUND for
+ * an unknown or collected id.  Thread-safe.  No reference counterpart (the
+ * reference solves one instance per process run). */
+brkga_status brkga_server_cancel(brkga_server_t server, uint64_t request_id);
 
 /* Finishes every submitted request, stops the worker and frees the server;
  * results not collected by then are dropped. */

@@ -33,6 +33,7 @@ ERR_BUSY = 4
 ERR_DEADLINE = 5
 ERR_NOT_FOUND = 6
 ERR_NO_DEVICE = 7
+ERR_CANCELLED = 8
 
 TSP_COORDS = 0
 TSP_MATRIX = 1
@@ -186,6 +187,7 @@ _SIGNATURES = {
                             ctypes.c_int),
     "brkga_server_wait": ([_P, ctypes.c_uint64, _I32, _PF, _PI, ctypes.POINTER(CRequestInfo)], ctypes.c_int),
     "brkga_server_stats_get": ([_P, ctypes.POINTER(CServerStats)], ctypes.c_int),
+    "brkga_server_cancel": ([_P, ctypes.c_uint64], ctypes.c_int),
     "brkga_server_stop": ([_P], ctypes.c_int),
 }
 
@@ -649,6 +651,11 @@ class Server:
         self._shape.pop(rid, None)
         d = {k: getattr(info, k) for k, _ in CRequestInfo._fields_}
         return (fit if M > 1 else float(fit[0])), dec, d
+
+    def cancel(self, rid: int):
+        """Cancels a request before its deadline; wait() then raises
+        BrkgaError(ERR_CANCELLED).  No effect on a finished request."""
+        _check(lib().brkga_server_cancel(self.handle, rid))
 
     def stats(self):
         st = CServerStats()

@@ -117,7 +117,7 @@ brkga_status brkga_batch_evolve(brkga_batch_t batch, int32_t generations, int32_
         return BRKGA_OK;
     }
     if (mode == BRKGA_BATCH_LAUNCH) {
-        for (int i = 0; i < generations && st == BRKGA_OK; ++i) st = batch->fleet->tick(s);
+        st = batch->fleet->ticks(generations, s);
     } else {
         st = batch->fleet->tick_graph(generations, s);
     }

@@ -66,6 +66,7 @@ Fleet::~Fleet()
     if (d_descs_) (void)hipFree(d_descs_);
     if (d_ls_descs_) (void)hipFree(d_ls_descs_);
     if (d_items_) (void)hipFree(d_items_);
+    if (d_segs_) (void)hipFree(d_segs_);
     if (d_queues_) (void)hipFree(d_queues_);
 }
 
@@ -254,6 +255,24 @@ brkga_status Fleet::build_items(hipStream_t s)
     if (st != BRKGA_OK) return st;
     if (!items.empty())
         BRKGA_HIP_TRY(hipMemcpyAsync(d_items_, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, s));
+    // the islands each lane ranks: those of its launches' members
+    segs_.clear();
+    for (int l = 0; l < kLanes; ++l) {
+        lane_seg_begin_[l] = (int)segs_.size();
+        for (const auto& L : launches_) {
+            if (L.lane != l) continue;
+            for (int bi : L.buckets) {
+                const FleetBucket& b = buckets_[bi];
+                for (int m = b.first; m < b.first + b.count; ++m)
+                    for (int isl = 0; isl < descs_[m].islands; ++isl) segs_.push_back(make_int2(m, isl));
+            }
+        }
+        lane_seg_count_[l] = (int)segs_.size() - lane_seg_begin_[l];
+    }
+    st = reserve(reinterpret_cast<void**>(&d_segs_), &segs_cap_, sizeof(int2) * segs_.size(), "fleet island list");
+    if (st != BRKGA_OK) return st;
+    if (!segs_.empty())
+        BRKGA_HIP_TRY(hipMemcpyAsync(d_segs_, segs_.data(), sizeof(int2) * segs_.size(), hipMemcpyHostToDevice, s));
     return BRKGA_OK;
 }
 
@@ -437,11 +456,68 @@ brkga_status Fleet::set_counters(int gen, hipStream_t s)
     return BRKGA_OK;
 }
 
-brkga_status Fleet::tick(hipStream_t s)
+brkga_status Fleet::ticks(int count, hipStream_t s)
 {
+    if (count <= 0) return BRKGA_OK;
     DeviceGuard g(eng_->device);
     brkga_status st = ensure_lanes(s);
     if (st != BRKGA_OK) return st;
+    if (!ls_descs_.empty()) {
+        for (int k = 0; k < count && st == BRKGA_OK; ++k) st = joined_tick(s);
+        return st;
+    }
+    bool used[kLanes] = {};
+    for (const auto& L : launches_) used[L.lane] = true;
+    if (!groups_.empty()) used[nsga_lane_] = true;
+    auto lane = [&](int k) { return k == 0 ? s : lanes_[k]; };
+    hipError_t e = hipSuccess;
+    bool forked = false;
+    for (int k = 1; k < kLanes && e == hipSuccess; ++k)
+        if (used[k]) {
+            if (!forked) e = hipEventRecord(fork_, s);
+            forked = true;
+            if (e == hipSuccess) e = hipStreamWaitEvent(lanes_[k], fork_, 0);
+        }
+    // Each lane launches its smallest-n (latency-bound) launches first, so
+    // the kernels still running at the end of its tick are the ones with the
+    // most parallel work; then it ranks its members' islands.
+    std::vector<int> order(launches_.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return launches_[a].n_max < launches_[b].n_max; });
+    for (int k = 0; k < count && e == hipSuccess; ++k) {
+        for (size_t r = 0; r < order.size() && e == hipSuccess; ++r) {
+            const int j = order[r];
+            const FleetLaunch& L = launches_[j];
+            e = launch_step_queue(L.kind, L.block, L.xys, L.n_max, d_descs_, d_items_ + L.item_begin, L.item_count,
+                                  d_queues_ + kQueueStride * (buckets_.size() + j), L.grid, L.lds, lane(L.lane));
+        }
+        for (int l = 0; l < kLanes && e == hipSuccess; ++l)
+            if (lane_seg_count_[l] > 0)
+                e = launch_partition_list(d_descs_, d_segs_ + lane_seg_begin_[l], lane_seg_count_[l], max_Pi_, -1, 1,
+                                          lane(l));
+        for (size_t i = 0; i < groups_.size() && e == hipSuccess; ++i) {
+            const FleetGroup& gr = groups_[i];
+            const InstDesc* d = d_descs_ + gr.first;
+            hipStream_t ls = lane(nsga_lane_);
+            e = launch_nsga2_offspring(gr.kind, gr.block, d, gr.count, gr.rows, gr.lds, 0, ls);
+            if (e == hipSuccess) e = launch_nsga2_select(d, gr.count, 0, gr.max_pop, ls);
+            if (e == hipSuccess) e = launch_nsga2_gather(d, gr.count, gr.rows, ls);
+            if (e == hipSuccess && gr.diversity)
+                e = launch_nsga2_diversity(gr.kind, gr.block, d, gr.count, gr.rows, gr.lds, gr.max_pop, ls);
+        }
+    }
+    for (int k = 1; k < kLanes && e == hipSuccess; ++k)
+        if (used[k]) {
+            e = hipEventRecord(join_[k], lanes_[k]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s, join_[k], 0);
+        }
+    if (e != hipSuccess) return hip_error(e, "fleet ticks");
+    return BRKGA_OK;
+}
+
+brkga_status Fleet::joined_tick(hipStream_t s)
+{
     bool used[kLanes] = {};
     for (const auto& L : launches_) used[L.lane] = true;
     if (!groups_.empty()) used[nsga_lane_] = true;
@@ -510,8 +586,7 @@ brkga_status Fleet::tick_graph(int ticks, hipStream_t s)
         if (ls != BRKGA_OK) return ls;
         hipGraph_t graph = nullptr;
         BRKGA_HIP_TRY(hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal));
-        brkga_status st = BRKGA_OK;
-        for (int i = 0; i < ticks && st == BRKGA_OK; ++i) st = tick(capture_);
+        brkga_status st = this->ticks(ticks, capture_);
         hipError_t e = hipStreamEndCapture(capture_, &graph);
         if (st != BRKGA_OK) {
             if (graph) (void)hipGraphDestroy(graph);

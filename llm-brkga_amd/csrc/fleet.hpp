@@ -74,7 +74,12 @@ public:
     // generation 0).
     brkga_status set_counters(int gen, hipStream_t s);
     // One generation of every member below its limit.
-    brkga_status tick(hipStream_t s);
+    brkga_status tick(hipStream_t s) { return ticks(1, s); }
+    // `count` generations of every member below its limit.  Each stream lane
+    // steps and ranks its own members tick after tick; the lanes join s once,
+    // at the end (every tick joins them when a member runs the TSPJ local
+    // search, which ranks every member again after it).
+    brkga_status ticks(int count, hipStream_t s);
     // `ticks` ticks replayed from a captured graph (cached per count).
     brkga_status tick_graph(int ticks, hipStream_t s);
     // Times every admissible (workgroup size, slots per item) pair of each
@@ -94,6 +99,8 @@ public:
 
 private:
     brkga_status build_items(hipStream_t s);
+    // One tick whose lanes join before one partition of every member.
+    brkga_status joined_tick(hipStream_t s);
     void form_launches();
     brkga_status tune(int reps, bool only_new, hipStream_t s);
     brkga_status reserve(void** p, size_t* cap, size_t bytes, const char* what);
@@ -124,6 +131,12 @@ private:
     int ls_items_ = 0, ls_block_ = 0, ls_lds_ = 0;
     int4* d_items_ = nullptr;
     size_t items_cap_ = 0;
+    // {member, island} of every BRKGA island, grouped by the lane that steps
+    // it (lane l: [lane_seg_begin_[l], + lane_seg_count_[l]))
+    std::vector<int2> segs_;
+    int2* d_segs_ = nullptr;
+    size_t segs_cap_ = 0;
+    int lane_seg_begin_[kLanes] = {}, lane_seg_count_[kLanes] = {};
     int* d_queues_ = nullptr;  // two counters per bucket (tuning launches), then per launch; 128 bytes apart
     size_t queues_cap_ = 0;
     hipStream_t capture_ = nullptr;

@@ -133,13 +133,9 @@ __global__ __launch_bounds__(kPartBlock) void k_partition_one(InstDesc d, int bu
     partition_body(d, blockIdx.x / parts, blockIdx.x % parts, parts, buf, mode, t, lds);
 }
 
-__global__ __launch_bounds__(kPartBlock) void k_partition_many(const InstDesc* __restrict__ descs, int num_inst,
-                                                               int buf, int mode)
+// Island `island` of member d of a fleet (k_partition_many / _list).
+__device__ __forceinline__ void partition_member(const InstDesc& d, int island, int buf, int mode, char* lds)
 {
-    extern __shared__ __align__(16) char lds[];
-    const InstDesc& d = descs[find_instance_by_seg(descs, num_inst, blockIdx.x)];
-    const int island = blockIdx.x - d.seg_begin;
-    if (island >= d.islands) return;
     int t = 0, b = buf;
     if (mode == 1) {
         // t = -1: generation 0 was just built into buffer 0; the counter
@@ -154,6 +150,26 @@ __global__ __launch_bounds__(kPartBlock) void k_partition_many(const InstDesc* _
         b = 0;
     }
     partition_body(d, island, 0, 1, b, mode, t, lds);
+}
+
+__global__ __launch_bounds__(kPartBlock) void k_partition_many(const InstDesc* __restrict__ descs, int num_inst,
+                                                               int buf, int mode)
+{
+    extern __shared__ __align__(16) char lds[];
+    const InstDesc& d = descs[find_instance_by_seg(descs, num_inst, blockIdx.x)];
+    const int island = blockIdx.x - d.seg_begin;
+    if (island >= d.islands) return;
+    partition_member(d, island, buf, mode, lds);
+}
+
+// One workgroup per listed {member, island}: the islands of the members one
+// stream lane of a fleet steps.
+__global__ __launch_bounds__(kPartBlock) void k_partition_list(const InstDesc* __restrict__ descs,
+                                                               const int2* __restrict__ segs, int buf, int mode)
+{
+    extern __shared__ __align__(16) char lds[];
+    const int2 sg = segs[blockIdx.x];
+    partition_member(descs[sg.x], sg.y, buf, mode, lds);
 }
 
 // TSPJ local search: one workgroup per (population, island, rank r < ls_k)
@@ -266,6 +282,16 @@ hipError_t launch_partition_many(const InstDesc* d_descs, int num_inst, int bloc
     const int lds = partition_lds_bytes(max_Pi);
     ensure_lds(k_partition_many, lds);
     hipLaunchKernelGGL(k_partition_many, dim3(blocks), dim3(kPartBlock), lds, s, d_descs, num_inst, buf, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_partition_list(const InstDesc* d_descs, const int2* d_segs, int count, int max_Pi, int buf,
+                                 int mode, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    const int lds = partition_lds_bytes(max_Pi);
+    ensure_lds(k_partition_list, lds);
+    hipLaunchKernelGGL(k_partition_list, dim3(count), dim3(kPartBlock), lds, s, d_descs, d_segs, buf, mode);
     return hipGetLastError();
 }
 

@@ -45,6 +45,9 @@ hipError_t launch_partition_one(const InstDesc& d, int buf, int mode, hipStream_
 // buf < 0: buffer from each population's step counter.
 hipError_t launch_partition_many(const InstDesc* d_descs, int num_inst, int blocks, int max_Pi, int buf, int mode,
                                  hipStream_t s);
+// The same for the `count` listed {member, island} pairs only.
+hipError_t launch_partition_list(const InstDesc* d_descs, const int2* d_segs, int count, int max_Pi, int buf,
+                                 int mode, hipStream_t s);
 
 // TSPJ local search of the current generation of every listed population
 // whose local-search flag is raised; `items` = largest islands x ls_k.

@@ -23,6 +23,7 @@
 #include <memory>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "capi_internal.hpp"
@@ -164,6 +165,7 @@ struct brkga_server_s {
     std::deque<std::unique_ptr<Request>> pending;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> finished;
     std::unordered_map<uint64_t, int> in_flight;  // ids admitted and not finished
+    std::unordered_set<uint64_t> cancelled;       // in-flight ids to retire at the next round
     brkga_server_stats stats{};
     std::string worker_error;
 
@@ -176,6 +178,7 @@ struct brkga_server_s {
     void run();
     bool solo_round(double* waited);
     bool admit();
+    bool retire_cancelled();
     void collect_leaving();
     void finish(std::unique_ptr<Request> r, brkga_status st);
 };
@@ -204,6 +207,33 @@ void brkga_server_s::finish(std::unique_ptr<Request> r, brkga_status st)
     stats.bytes_in_use = arena.used();
     finished[r->id] = std::move(r);
     cv_done.notify_all();
+}
+
+// Retires the resident requests cancelled since the last round: no launch
+// that uses their populations is queued any more (every round ends by
+// waiting for its steps).  True when the resident set changed.
+bool brkga_server_s::retire_cancelled()
+{
+    std::vector<uint64_t> ids;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        if (cancelled.empty()) return false;
+        ids.assign(cancelled.begin(), cancelled.end());
+        cancelled.clear();
+    }
+    bool changed = false;
+    std::vector<std::unique_ptr<Request>> keep;
+    for (auto& r : resident) {
+        if (std::find(ids.begin(), ids.end(), r->id) != ids.end()) {
+            r->info.generations = r->ticks > 0 ? r->ticks - 1 : 0;
+            finish(std::move(r), BRKGA_ERR_CANCELLED);
+            changed = true;
+        } else {
+            keep.push_back(std::move(r));
+        }
+    }
+    resident.swap(keep);
+    return changed;
 }
 
 // Admits pending requests in arrival order while they fit; true when the
@@ -421,13 +451,20 @@ void brkga_server_s::run()
     for (;;) {
         {
             std::unique_lock<std::mutex> lock(mu);
-            cv_work.wait(lock, [&] { return stopping || !pending.empty() || !resident.empty() || !leaving.empty(); });
+            cv_work.wait(lock, [&] {
+                return stopping || !pending.empty() || !resident.empty() || !leaving.empty() || !cancelled.empty();
+            });
             if (stopping && pending.empty() && resident.empty() && leaving.empty()) break;
+            if (pending.empty() && resident.empty() && leaving.empty()) {
+                cancelled.clear();  // ids that finished before their cancel was seen
+                continue;
+            }
             ++stats.rounds;
         }
         const double h0 = now_us(t0);
         double waited = 0.0;
         brkga_status st = BRKGA_OK;
+        if (retire_cancelled()) fleet_stale = true;
         const bool admitted = admit();
         if (solo_round(&waited)) {
             // the fleet's member list is rebuilt when the resident set
@@ -450,7 +487,7 @@ void brkga_server_s::run()
                 st = fleet->autotune_new(cfg.autotune_reps, stream);
         }
         const bool ticking = st == BRKGA_OK && !resident.empty();
-        for (int k = 0; ticking && k < cfg.ticks_per_round && st == BRKGA_OK; ++k) st = fleet->tick(stream);
+        if (ticking) st = fleet->ticks(cfg.ticks_per_round, stream);
         collect_leaving();
         // problems of the requests that left are referenced by no queued
         // launch; hipFree waits for the device, i.e. for this round's ticks
@@ -592,10 +629,46 @@ brkga_status brkga_server_wait(brkga_server_t server, uint64_t request_id, int32
     std::unique_ptr<Request> r = std::move(server->finished[request_id]);
     server->finished.erase(request_id);
     if (info) *info = r->info;
+    if (r->info.status == BRKGA_ERR_CANCELLED) return set_error(BRKGA_ERR_CANCELLED, "request cancelled");
     if (r->info.status != BRKGA_OK)
         return set_error((brkga_status)r->info.status, "request failed: " + server->worker_error);
     if (fitness) std::copy(r->fitness.begin(), r->fitness.end(), fitness);
     if (decoded) std::copy(r->decoded.begin(), r->decoded.end(), decoded);
+    return BRKGA_OK;
+}
+
+brkga_status brkga_server_cancel(brkga_server_t server, uint64_t request_id)
+{
+    if (!server) return set_error(BRKGA_ERR_INVALID_ARG, "null server");
+    std::unique_ptr<Request> dropped;
+    {
+        std::lock_guard<std::mutex> lock(server->mu);
+        if (server->finished.count(request_id)) return BRKGA_OK;
+        if (server->in_flight.count(request_id)) {
+            server->cancelled.insert(request_id);
+            server->cv_work.notify_one();
+            return BRKGA_OK;
+        }
+        for (auto it = server->pending.begin(); it != server->pending.end(); ++it)
+            if ((*it)->id == request_id) {
+                dropped = std::move(*it);
+                server->pending.erase(it);
+                break;
+            }
+        if (!dropped) return set_error(BRKGA_ERR_NOT_FOUND, "unknown or already collected request");
+        dropped->info.status = BRKGA_ERR_CANCELLED;
+        dropped->info.done_us = now_us(server->t0);
+        ++server->stats.completed;
+    }
+    // never admitted: no device memory of the request, and no launch uses
+    // its problem
+    if (dropped->own_problem && dropped->problem) {
+        (void)brkga_problem_destroy(dropped->problem);
+        dropped->problem = nullptr;
+    }
+    std::lock_guard<std::mutex> lock(server->mu);
+    server->finished[request_id] = std::move(dropped);
+    server->cv_done.notify_all();
     return BRKGA_OK;
 }
 

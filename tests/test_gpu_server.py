@@ -160,3 +160,41 @@ def test_server_rejects_and_reports(engine):
     st = srv.stats()
     assert st["rejected"] == 1 and st["completed"] == 1
     srv.stop()
+
+
+def test_server_cancel(engine):
+    """A cancelled pending request and a cancelled resident request end with
+    ERR_CANCELLED; every other request still equals its solo run; the HBM of
+    the cancelled ones returns to the budget."""
+    print("server cancel: one resident at a time, long first request", flush=True)
+    reqs = build_requests(engine)[:6]
+    long_gp, _ = make_problem(engine, "tsp", 400, inst=777)
+    long_prm = brkga_amd.Params(256, 38, 38, 0.7, 1, 0, 0, 0xC0FFEE)
+    refs = [solo_best(gp, prm, G) for gp, prm, G in reqs]
+    srv = brkga_amd.Server(engine, hbm_budget_bytes=512 << 20, max_resident=1, ticks_per_round=2)
+    long_id = srv.submit(long_gp, long_prm, 1_000_000)  # runs until cancelled
+    ids = [srv.submit(gp, prm, G) for gp, prm, G in reqs]
+    srv.cancel(ids[2])  # still pending: max_resident = 1 and the long request holds the device
+    time.sleep(0.05)
+    assert srv.stats()["resident"] == 1
+    srv.cancel(long_id)  # resident: leaves at the next round
+    for k, (rid, ref) in enumerate(zip(ids, refs)):
+        if k == 2:
+            with pytest.raises(brkga_amd.BrkgaError) as e:
+                srv.wait(rid, timeout_ms=60000)
+            assert e.value.status == brkga_amd.ERR_CANCELLED
+            continue
+        check(srv.wait(rid, timeout_ms=60000), ref, f"request {k} after cancels")
+    with pytest.raises(brkga_amd.BrkgaError) as e:
+        srv.wait(long_id, timeout_ms=60000)
+    assert e.value.status == brkga_amd.ERR_CANCELLED
+    st = srv.stats()
+    assert st["bytes_in_use"] == 0 and st["resident"] == 0 and st["pending"] == 0
+    # unknown ids; a collected id is unknown too
+    with pytest.raises(brkga_amd.BrkgaError) as e:
+        srv.cancel(10**9)
+    assert e.value.status == brkga_amd.ERR_NOT_FOUND
+    with pytest.raises(brkga_amd.BrkgaError) as e:
+        srv.cancel(ids[0])
+    assert e.value.status == brkga_amd.ERR_NOT_FOUND
+    srv.stop()
