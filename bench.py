@@ -20,7 +20,8 @@ generation 0 + 300 steps.  Timed three ways on the same instances: the
 plain per-instance form (one instance after the other, one launch sequence
 per generation), the batched launches, and the batched launches replayed
 from a captured graph (after the autotuner picked each size bucket's launch
-shape).  Every member's best fitness must be identical across the forms.
+shape); the batched figure is the faster of the two batched modes.  Every
+member's best fitness must be identical across the forms.
 
 Served leg ("served"): Poisson arrival streams (rates 50 ... 2000 req/s, and
 all requests at once) of mixed requests: 60% Euclidean TSP with n
@@ -252,17 +253,22 @@ def batched_leg(brkga_amd, workloads, eng, gens):
     t_plain = time.perf_counter() - t0
     same = same and bests() == ref
 
+    # the batched form's value: the faster of its two launch modes (both run
+    # the same kernels and give the same results)
+    t_best, mode = (t_graph, "graph") if t_graph <= t_launch else (t_launch, "launch")
     B = sum(algorithmic_bytes_per_generation(n, BATCH_POP, BATCH_MUTANTS) for n in sizes) * gens
-    achieved = B / t_graph / 1e9
+    achieved = B / t_best / 1e9
     out = {
         "workload": f"{BATCH_COUNT} Euclidean TSP instances, n log-uniform in [{BATCH_N_LO},{BATCH_N_HI}] "
                     f"(sum n = {sum(sizes)}), coordinates uniform FP32 in [0,1000)^2, P={BATCH_POP}, "
                     f"elite {BATCH_ELITE}, mutants {BATCH_MUTANTS}, rho {BATCH_RHO}, {gens} generations",
-        "solves_per_s": round(BATCH_COUNT / t_graph, 2),
+        "solves_per_s": round(BATCH_COUNT / t_best, 2),
+        "mode": mode,
+        "solves_per_s_graph": round(BATCH_COUNT / t_graph, 2),
         "solves_per_s_launch": round(BATCH_COUNT / t_launch, 2),
         "solves_per_s_plain": round(BATCH_COUNT / t_plain, 2),
-        "speedup_vs_plain": round(t_plain / t_graph, 2),
-        "ms_per_batch_generation": round(1000.0 * t_graph / gens, 4),
+        "speedup_vs_plain": round(t_plain / t_best, 2),
+        "ms_per_batch_generation": round(1000.0 * t_best / gens, 4),
         "plain_seconds": round(t_plain, 3),
         "results_identical": bool(same),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
