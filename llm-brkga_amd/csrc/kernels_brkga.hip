@@ -34,10 +34,11 @@ __host__ __device__ inline int partition_parts(int Pi)
 // slots) to t + 1 and raises the local-search flag when generation t + 1 is
 // due; mode 2: the last workgroup lowers the flag; mode 3: the last
 // workgroup sets both counter slots to t.
+// B: workgroup size (kPartBlock, or 256 for islands of at most 256).
+template <int B = kPartBlock>
 __device__ __forceinline__ void partition_body(const InstDesc& d, int island, int part, int parts, int buf,
                                                int mode, int t, char* lds)
 {
-    constexpr int B = kPartBlock;
     unsigned long long* sk = reinterpret_cast<unsigned long long*>(lds);
     const int Pi = d.Pi;
     const int M = d.prob.num_obj;
@@ -133,7 +134,9 @@ __global__ __launch_bounds__(kPartBlock) void k_partition_one(InstDesc d, int bu
     partition_body(d, blockIdx.x / parts, blockIdx.x % parts, parts, buf, mode, t, lds);
 }
 
-// Island `island` of member d of a fleet (k_partition_many / _list).
+// Island `island` of member d of a fleet (k_partition_many / _list), ranked
+// by a workgroup of B threads.
+template <int B = kPartBlock>
 __device__ __forceinline__ void partition_member(const InstDesc& d, int island, int buf, int mode, char* lds)
 {
     int t = 0, b = buf;
@@ -149,7 +152,7 @@ __device__ __forceinline__ void partition_member(const InstDesc& d, int island, 
     } else if (b < 0) {
         b = 0;
     }
-    partition_body(d, island, 0, 1, b, mode, t, lds);
+    partition_body<B>(d, island, 0, 1, b, mode, t, lds);
 }
 
 __global__ __launch_bounds__(kPartBlock) void k_partition_many(const InstDesc* __restrict__ descs, int num_inst,
@@ -163,13 +166,17 @@ __global__ __launch_bounds__(kPartBlock) void k_partition_many(const InstDesc* _
 }
 
 // One workgroup per listed {member, island}: the islands of the members one
-// stream lane of a fleet steps.
-__global__ __launch_bounds__(kPartBlock) void k_partition_list(const InstDesc* __restrict__ descs,
-                                                               const int2* __restrict__ segs, int buf, int mode)
+// stream lane of a fleet steps.  Islands of at most 256 are ranked by 256
+// threads (one key each): a lane's ranking launch runs beside the other
+// lanes' step launches, and four-wave workgroups find room on a CU sooner
+// than sixteen-wave ones.
+template <int B>
+__global__ __launch_bounds__(B) void k_partition_list(const InstDesc* __restrict__ descs,
+                                                      const int2* __restrict__ segs, int buf, int mode)
 {
     extern __shared__ __align__(16) char lds[];
     const int2 sg = segs[blockIdx.x];
-    partition_member(descs[sg.x], sg.y, buf, mode, lds);
+    partition_member<B>(descs[sg.x], sg.y, buf, mode, lds);
 }
 
 // TSPJ local search: one workgroup per (population, island, rank r < ls_k)
@@ -290,8 +297,13 @@ hipError_t launch_partition_list(const InstDesc* d_descs, const int2* d_segs, in
 {
     if (count <= 0) return hipSuccess;
     const int lds = partition_lds_bytes(max_Pi);
-    ensure_lds(k_partition_list, lds);
-    hipLaunchKernelGGL(k_partition_list, dim3(count), dim3(kPartBlock), lds, s, d_descs, d_segs, buf, mode);
+    if (max_Pi <= 256) {
+        hipLaunchKernelGGL(k_partition_list<256>, dim3(count), dim3(256), lds, s, d_descs, d_segs, buf, mode);
+    } else {
+        ensure_lds(k_partition_list<kPartBlock>, lds);
+        hipLaunchKernelGGL(k_partition_list<kPartBlock>, dim3(count), dim3(kPartBlock), lds, s, d_descs, d_segs, buf,
+                           mode);
+    }
     return hipGetLastError();
 }
 
