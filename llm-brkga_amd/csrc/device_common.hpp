@@ -391,20 +391,20 @@ __device__ __forceinline__ long long fx_value(const FxAcc& a)
 // order of key_order.
 __device__ __forceinline__ int key_bucket(float k, int nb)
 {
-    if (k != k) return nb - 1;
-    if (!(k >= 0.0f)) return 0;
-    if (k >= 1.0f) return nb - 1;
-    int b = (int)__fmul_rn(k, (float)nb);
-    return b < nb - 1 ? b : nb - 1;
+    // clamped in FP32 to [0, nb - 1] first (fmaxf of NaN and 0 is 0, so the
+    // conversion always gets a value in range); NaN goes to the last bucket
+    const float x = fminf(fmaxf(__fmul_rn(k, (float)nb), 0.0f), (float)(nb - 1));
+    return k != k ? nb - 1 : (int)x;
 }
 
 // Order-preserving FP32 -> u32 (-0 == +0, NaN last).
 __device__ __forceinline__ uint32_t key_order(float k)
 {
-    if (k != k) return 0xFFFFFFFFu;
-    if (k == 0.0f) k = 0.0f;
-    const uint32_t u = __float_as_uint(k);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    // k + 0 turns -0 into +0 (IEEE addition) and keeps every other value;
+    // negative numbers are flipped whole, others get the sign bit
+    const uint32_t u = __float_as_uint(__fadd_rn(k, 0.0f));
+    const uint32_t o = u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
+    return k != k ? 0xFFFFFFFFu : o;
 }
 
 struct SortScratch {
