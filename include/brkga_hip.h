@@ -493,8 +493,8 @@ brkga_status brkga_server_stats_get(brkga_server_t server, brkga_server_stats* s
  * of the worker's next admission round (the generations it ran are dropped
  * and its HBM returns to the budget).  brkga_server_wait then returns
  * BRKGA_ERR_CANCELLED for it, with its info.  A request that already
- * finished keeps its result (no effect, BRKGA_OK); BRKGA_ERR_NOT_FOUND for
- * an unknown or collected id.  Thread-safe.  No reference counterpart (the
+ * reached its deadline keeps its result (no effect, BRKGA_OK);
+ * BRKGA_ERR_NOT_FOUND for an unknown or collected id.  Thread-safe.  No reference counterpart (the
  * reference solves one instance per process run). */
 brkga_status brkga_server_cancel(brkga_server_t server, uint64_t request_id);
 

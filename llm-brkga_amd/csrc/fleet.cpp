@@ -507,10 +507,13 @@ brkga_status Fleet::ticks(int count, hipStream_t s)
                 e = launch_nsga2_diversity(gr.kind, gr.block, d, gr.count, gr.rows, gr.lds, gr.max_pop, ls);
         }
     }
-    for (int k = 1; k < kLanes && e == hipSuccess; ++k)
+    // every lane that was forked joins s, also after a failed launch, so
+    // that waiting for s waits for everything the call queued
+    for (int k = 1; k < kLanes && forked; ++k)
         if (used[k]) {
-            e = hipEventRecord(join_[k], lanes_[k]);
-            if (e == hipSuccess) e = hipStreamWaitEvent(s, join_[k], 0);
+            hipError_t j = hipEventRecord(join_[k], lanes_[k]);
+            if (j == hipSuccess) j = hipStreamWaitEvent(s, join_[k], 0);
+            if (e == hipSuccess) e = j;
         }
     if (e != hipSuccess) return hip_error(e, "fleet ticks");
     return BRKGA_OK;
@@ -571,7 +574,14 @@ brkga_status Fleet::joined_tick(hipStream_t s)
         e = hipEventRecord(join_[nsga_lane_], lanes_[nsga_lane_]);
         if (e == hipSuccess) e = hipStreamWaitEvent(s, join_[nsga_lane_], 0);
     }
-    if (e != hipSuccess) return hip_error(e, "fleet tick");
+    if (e != hipSuccess) {
+        // after a failed launch every forked lane still joins s, so that
+        // waiting for s waits for everything the tick queued
+        for (int k = 1; k < kLanes && forked; ++k)
+            if (used[k] && hipEventRecord(join_[k], lanes_[k]) == hipSuccess)
+                (void)hipStreamWaitEvent(s, join_[k], 0);
+        return hip_error(e, "fleet tick");
+    }
     return BRKGA_OK;
 }
 
