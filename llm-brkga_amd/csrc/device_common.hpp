@@ -361,26 +361,23 @@ __device__ __forceinline__ long long block_inclusive_scan_i64(long long v, long 
 
 // --------------------------------------------------------- fixed point ----
 
-// value = hi * 2^24 + lo: 32-bit partial sums of at most 127 terms each.
+// A thread's sum of at most 16 terms llrint(t * 2^S), |t * 2^S| <= 2^48,
+// held exactly in FP64: each term rounds to an integer in FP32 (a float of
+// magnitude >= 2^23 already is one, a smaller one rounds to an integer below
+// 2^24, which FP32 holds), and FP64 adds such integers exactly while the sum
+// stays below 2^53 (16 x 2^48 = 2^52).
 struct FxAcc {
-    int hi = 0, lo = 0;
+    double s = 0.0;
 };
 
-// Adds llrint(t * 2^S) for |t * 2^S| <= 2^48: the value splits exactly into
-// a multiple of 2^24 and a remainder below 2^24; only the remainder can need
-// rounding, and only when the whole value is below 2^24 (then hi = 0).
 __device__ __forceinline__ void fx_add(FxAcc& a, float t, float scale)
 {
-    const float x = t * scale;
-    const float h = __builtin_truncf(x * 0x1p-24f);
-    const float l = __builtin_fmaf(-h, 0x1p24f, x);
-    a.hi += (int)h;
-    a.lo += (int)__builtin_rintf(l);
+    a.s += (double)__builtin_rintf(t * scale);
 }
 
 __device__ __forceinline__ long long fx_value(const FxAcc& a)
 {
-    return ((long long)a.hi << 24) + (long long)a.lo;
+    return (long long)a.s;
 }
 
 // ------------------------------------------------------------- argsort ----
