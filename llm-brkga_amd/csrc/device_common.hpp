@@ -636,8 +636,9 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
             const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
             ok[i] = key_order(kf[i]);
             rng[i] = e < n ? key_bucket(kf[i], nb) : nb + 2;
-            slot[i] = 0;
-            if (e < n) slot[i] = atomicAdd(&s.cnt[rng[i]], 1);
+            // an element past n counts itself in the spare counter, which
+            // nothing reads as a bucket (no per-key branch)
+            slot[i] = atomicAdd(&s.cnt[rng[i]], 1);
         }
     }
     __syncthreads();
@@ -670,7 +671,9 @@ __device__ __forceinline__ void block_rank_by(KeyAt key_at, int n, int nb, const
         for (int i = 0; i < ITEMS; ++i) {
             const int e = rank_elem<BLOCK, ITEMS, CONTIG>(tid, i);
             const bool valid = e < n;
-            if (valid) s.items[st[i] + slot[i]] = ((unsigned long long)ok[i] << 32) | (unsigned)e;
+            // an element past n writes the first spare item, which the ranking
+            // reads only as padding past a bucket's end
+            s.items[valid ? st[i] + slot[i] : n] = ((unsigned long long)ok[i] << 32) | (unsigned)e;
             const int sz = en[i] - st[i];
             rng[i] = valid ? (st[i] | (sz << 16)) : 0;
             longest = valid && sz > longest ? sz : longest;
