@@ -61,8 +61,17 @@ __device__ __forceinline__ void tsp_coords_tail(const ProblemDesc& p, char* lds,
             b[j] = txy[q + 1 < n ? q + 1 : 0];
         }
 #pragma unroll
-        for (int j = 0; j < G; ++j)
-            if (tid + (i0 + j) * BLOCK < n) fx_add(acc, edge_xy(a[j].x, a[j].y, b[j].x, b[j].y), p.scale0);
+        for (int j = 0; j < G; ++j) {
+            const bool in = tid + (i0 + j) * BLOCK < n;
+            if constexpr (BATCH) {
+                // few positions past n (a thread's four keys are its own
+                // column): those add a zero edge, with no per-edge branch
+                const float ej = edge_xy(a[j].x, a[j].y, b[j].x, b[j].y);
+                fx_add(acc, in ? ej : 0.0f, p.scale0);
+            } else if (in) {
+                fx_add(acc, edge_xy(a[j].x, a[j].y, b[j].x, b[j].y), p.scale0);
+            }
+        }
     }
     const long long tot = block_sum_fx<BLOCK>(fx_value(acc), ss);
     if (tid == 0 && fit_out) fit_out[0] = fx_to_float(tot, p.inv0);
@@ -107,7 +116,7 @@ __device__ __forceinline__ void decode_tsp_coords_reg(const ProblemDesc& p, char
 #pragma unroll
         for (int i = 0; i < W; ++i) {
             const int e = W * tid + i;
-            if (e < n) c[i] = xy[e];
+            c[i] = xy[e < n ? e : n - 1];  // elements past n are never used
         }
     };
     auto key_at = [&](int i, int) { return kk[i]; };
