@@ -284,14 +284,15 @@ struct KeyCol<2> {
 
 // Builds child slot s of island `island` for the step from parent buffer
 // `cur` with step index t (generation 0 when init): the row goes to `out`
-// (global) and `kl` (LDS), in columns of W keys.  Elite and migrant slots
+// (global) and, with TO_LDS, to `kl` (LDS; callers that decode from `mine`
+// need no copy there), in columns of W keys.  Elite and migrant slots
 // are copied with their fitness instead; then it returns false and nothing
 // is left to decode.  `order` is the island's rank order (LDS-staged or
 // global); `mig_order` the rank order of the preceding island on the ring,
 // or null to read it from the partition's output.  mine (optional, columns
 // <= BLOCK): the column ltid this thread built (or copied), left unchanged
 // for threads with none.
-template <int BLOCK, int W = 4>
+template <int BLOCK, int W = 4, bool TO_LDS = true>
 __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s, int cur, int init, int t,
                                             const int* order, const int* mig_order, typename KeyCol<W>::T* out,
                                             typename KeyCol<W>::T* kl, float* fit_out,
@@ -308,7 +309,7 @@ __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s
         for (int c = tid; c < nq; c += BLOCK) {
             const V x = Col::units(rng_block(d.seed_lo, d.seed_hi, island, 0, s, TAG_INIT, c >> kShift), c);
             out[c] = x;
-            kl[c] = x;
+            if constexpr (TO_LDS) kl[c] = x;
             if (mine) *mine = x;
         }
         return true;
@@ -337,7 +338,7 @@ __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s
         for (int c = tid; c < nq; c += BLOCK) {
             const V x = Col::units(rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_MUTANT, c >> kShift), c);
             out[c] = x;
-            kl[c] = x;
+            if constexpr (TO_LDS) kl[c] = x;
             if (mine) *mine = x;
         }
         return true;
@@ -352,7 +353,7 @@ __device__ __forceinline__ bool build_child(const InstDesc& d, int island, int s
         const U4 r = rng_block(d.seed_lo, d.seed_hi, island, g, s, TAG_COINS, c >> kShift);
         const V x = Col::pick(r, c, pe[c], pn[c], rho);
         out[c] = x;
-        kl[c] = x;
+        if constexpr (TO_LDS) kl[c] = x;
         if (mine) *mine = x;
     }
     return true;

@@ -114,8 +114,9 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
         using V = typename KeyCol<ITEMS>::T;
         V mine{};
         V* out_w = reinterpret_cast<V*>(d.keys[nxt] + (size_t)gs * ld);
-        V* kl_w = reinterpret_cast<V*>(lds + sl.dec.keys_off);
-        if (!build_child<BLOCK, ITEMS>(d, island, s, cur, init, t, own_order, mig_order, out_w, kl_w, fit_out, &mine))
+        // the decode ranks the keys from `mine`: no copy of the row in LDS
+        if (!build_child<BLOCK, ITEMS, false>(d, island, s, cur, init, t, own_order, mig_order, out_w, nullptr,
+                                              fit_out, &mine))
             return;
         decode_tsp_coords_reg<BLOCK>(d.prob, lds, sl.dec, nb, fit_out, mine);
     } else {

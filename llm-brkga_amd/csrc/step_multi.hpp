@@ -100,9 +100,9 @@ __global__ __launch_bounds__(BLOCK * CH) void k_step_multi_tsp(InstDesc d, int c
     const int gs = base + g, s = s0 + g;
     float4* out = reinterpret_cast<float4*>(d.keys[cur ^ 1] + (size_t)gs * d.ld);
     float* fit_out = d.fit[cur ^ 1] + (size_t)gs * d.prob.num_obj;
-    float4* kl = reinterpret_cast<float4*>(glds + dl.keys_off);
     float4 mine = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const bool built = build_child<BLOCK>(d, island, s, cur, 0, t, own_order, prev_order, out, kl, fit_out, &mine);
+    const bool built = build_child<BLOCK, 4, false>(d, island, s, cur, 0, t, own_order, prev_order, out, nullptr,
+                                                    fit_out, &mine);
     if (all_copies) return;
     decode_tsp_coords_reg<BLOCK>(d.prob, glds, dl, nb, built ? fit_out : nullptr, mine, long_flag);
 }
