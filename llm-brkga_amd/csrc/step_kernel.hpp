@@ -131,18 +131,29 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
 // Child slot of workgroup b.  Workgroups go to the 8 XCDs round-robin
 // (XCD b % 8), so XCD x gets the contiguous slots [x P/8, (x+1) P/8): an
 // island's children, whose rows and fitness the island's next generation
-// reads, stay in the L2 of one or two XCDs.  Identity when 8 does not
-// divide P.  A bijection on [0, P) either way.
-__device__ __forceinline__ int xcd_slot(int b, int P)
+// reads, stay in the L2 of one or two XCDs.  A range that starts an island
+// and ends inside it is rotated by the elite count, so that its workgroups
+// of lower index (dispatched first, and older on their CU, whose wave
+// arbiter favours older waves) build offspring and its elite copies, the
+// shortest work, come last.  Identity when 8 does not divide P.  A
+// bijection on [0, P) either way.
+__device__ __forceinline__ int xcd_slot(int b, int P, int Pi, int Ei)
 {
-    return (P & 7) ? b : (b & 7) * (P >> 3) + (b >> 3);
+    if (P & 7) return b;
+    const int R = P >> 3, lo = (b & 7) * R;
+    int j = b >> 3;
+    if (R <= Pi && Ei < R && lo % Pi == 0) {
+        j += Ei;
+        if (j >= R) j -= R;
+    }
+    return lo + j;
 }
 
 template <int KIND, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_step_one(InstDesc d, int cur, int init, int rank)
 {
     extern __shared__ __align__(16) char lds[];
-    step_body<KIND, BLOCK, ITEMS>(d, xcd_slot(blockIdx.x, d.pop), cur, init, rank, lds);
+    step_body<KIND, BLOCK, ITEMS>(d, xcd_slot(blockIdx.x, d.pop, d.Pi, d.Ei), cur, init, rank, lds);
 }
 
 template <int KIND, int BLOCK>
