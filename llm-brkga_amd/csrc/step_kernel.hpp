@@ -37,11 +37,10 @@ __device__ __forceinline__ int load_step_counter(const int* slot)
 // island for a migrant slot) and reads the step counter from its parity
 // slot; block 0 writes t + 1 to the other slot.
 template <int KIND, int BLOCK, int ITEMS>
-__device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, int init, int rank, char* lds)
+__device__ __forceinline__ void step_body(const InstDesc& d, int gs, int island, int s, int cur, int init, int rank,
+                                          char* lds)
 {
     const int Pi = d.Pi, Ei = d.Ei, I = d.islands;
-    const int island = gs / Pi;
-    const int s = gs - island * Pi;
     const int nxt = init ? 0 : cur ^ 1;
     const int M = d.prob.num_obj;
     const int ld = d.ld, L = d.L, n = d.prob.n;
@@ -128,32 +127,45 @@ __device__ __forceinline__ void step_body(const InstDesc& d, int gs, int cur, in
     }
 }
 
-// Child slot of workgroup b.  Workgroups go to the 8 XCDs round-robin
-// (XCD b % 8), so XCD x gets the contiguous slots [x P/8, (x+1) P/8): an
-// island's children, whose rows and fitness the island's next generation
-// reads, stay in the L2 of one or two XCDs.  A range that starts an island
-// and ends inside it is rotated by the elite count, so that its workgroups
-// of lower index (dispatched first, and older on their CU, whose wave
-// arbiter favours older waves) build offspring and its elite copies, the
-// shortest work, come last.  Identity when 8 does not divide P.  A
-// bijection on [0, P) either way.
-__device__ __forceinline__ int xcd_slot(int b, int P, int Pi, int Ei)
+// Child slot gs of workgroup b, with its island and its slot s in the
+// island.  Workgroups go to the 8 XCDs round-robin (XCD b % 8), so XCD x
+// gets the contiguous slots [x P/8, (x+1) P/8): an island's children,
+// whose rows and fitness the island's next generation reads, stay in the
+// L2 of one or two XCDs.  A range that starts an island and ends inside it
+// is rotated by the elite count, so that its workgroups of lower index
+// (dispatched first, and older on their CU, whose wave arbiter favours
+// older waves) build offspring and its elite copies, the shortest work,
+// come last.  Identity when 8 does not divide P.  A bijection on [0, P)
+// either way.  One integer division: a range no longer than an island
+// crosses at most one island boundary.
+struct SlotPos {
+    int gs, island, s;
+};
+__device__ __forceinline__ SlotPos xcd_slot(int b, int P, int Pi, int Ei)
 {
-    if (P & 7) return b;
-    const int R = P >> 3, lo = (b & 7) * R;
-    int j = b >> 3;
-    if (R <= Pi && Ei < R && lo % Pi == 0) {
-        j += Ei;
-        if (j >= R) j -= R;
+    const int R = P >> 3;
+    if ((P & 7) == 0 && R <= Pi) {
+        const int lo = (b & 7) * R;
+        const int il = lo / Pi, sl = lo - il * Pi;
+        int j = b >> 3;
+        if (sl == 0 && Ei < R) {
+            j += Ei;
+            if (j >= R) j -= R;
+        }
+        const int s = sl + j;
+        return s >= Pi ? SlotPos{lo + j, il + 1, s - Pi} : SlotPos{lo + j, il, s};
     }
-    return lo + j;
+    const int gs = (P & 7) ? b : (b & 7) * R + (b >> 3);
+    const int island = gs / Pi;
+    return SlotPos{gs, island, gs - island * Pi};
 }
 
 template <int KIND, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_step_one(InstDesc d, int cur, int init, int rank)
 {
     extern __shared__ __align__(16) char lds[];
-    step_body<KIND, BLOCK, ITEMS>(d, xcd_slot(blockIdx.x, d.pop, d.Pi, d.Ei), cur, init, rank, lds);
+    const SlotPos q = xcd_slot(blockIdx.x, d.pop, d.Pi, d.Ei);
+    step_body<KIND, BLOCK, ITEMS>(d, q.gs, q.island, q.s, cur, init, rank, lds);
 }
 
 template <int KIND, int BLOCK>
