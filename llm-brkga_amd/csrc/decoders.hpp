@@ -217,24 +217,38 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
         // capacity (a block prefix scan finds f), rejects item f, and from
         // there on the remaining capacity is below w[f]: wave 0 walks the
         // rest 64 items at a time, packing the first open item that fits.
+        // The weights and values are gathered in key order into LDS first
+        // (over the argsort's items, free after its final barrier), every
+        // thread's loads issued together, so neither the scan nor the walk
+        // waits on global memory.
+        int2* wv = reinterpret_cast<int2*>(ss.items);
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const int q = tid + k * BLOCK;
+            if (q < n) {
+                const int item = perm[q];
+                wv[q] = make_int2(p.w[item], p.v[item]);
+            }
+        }
+        __syncthreads();
         const int chunk = (n + BLOCK - 1) / BLOCK;
         const int lo = tid * chunk < n ? tid * chunk : n;
         const int hi = lo + chunk < n ? lo + chunk : n;
         long long wsum = 0;
-        for (int q = lo; q < hi; ++q) wsum += p.w[perm[q]];
+        for (int q = lo; q < hi; ++q) wsum += wv[q].x;
         long long run = block_inclusive_scan_i64<BLOCK>(wsum, red) - wsum;
         int first = n;
         long long vsum = 0, rem_at = 0;
         for (int q = lo; q < hi; ++q) {
-            const int item = perm[q];
-            const long long w = p.w[item];
+            const int2 e = wv[q];
+            const long long w = e.x;
             if (run + w > p.cap) {
                 first = q;
                 rem_at = p.cap - run;
                 break;
             }
             run += w;
-            vsum += p.v[item];
+            vsum += e.y;
         }
         const int f = (int)-block_max_i64<BLOCK>(-(long long)first, red);
         long long val = block_sum_i64<BLOCK>(vsum, red);
@@ -249,9 +263,9 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
             for (int base = f + 1; base < n; base += kWave) {
                 const int q = base + lane;
                 const bool valid = q < n;
-                const int item = valid ? perm[q] : 0;
-                const long long w = valid ? (long long)p.w[item] : 0;
-                const long long v = valid ? (long long)p.v[item] : 0;
+                const int2 e = valid ? wv[q] : make_int2(0, 0);
+                const long long w = e.x;
+                const long long v = e.y;
                 bool open = valid;
                 bool taken = false;
                 while (true) {
@@ -263,7 +277,7 @@ __device__ void decode_block(const ProblemDesc& p, char* lds, const DecodeLayout
                     taken = taken || lane == fl;
                     open = open && lane > fl;
                 }
-                if (decoded && valid) decoded[item] = taken ? 1 : 0;
+                if (decoded && valid) decoded[perm[q]] = taken ? 1 : 0;
             }
             if (lane == 0 && fit_out) fit_out[0] = __ll2float_rn(val);
         }
