@@ -15,9 +15,14 @@ namespace brkga {
 // Euclidean instances, the coordinates once per item, then builds and
 // decodes the item's children one after another.  Each instance's step
 // counter selects its buffers: -1 builds generation 0 into buffer 0,
-// t >= gen_limit leaves the instance untouched.  queue[0] hands out items
-// and queue[1] counts retired workgroups; the last workgroup re-arms both,
-// so the kernel can be relaunched (or replayed from a graph) as is.
+// t >= gen_limit leaves the instance untouched.  Workgroup b takes item b
+// first, with no atomic; items past the grid are handed out by queue[0]
+// (item gridDim.x + queue[0]++), and queue[1] counts retired workgroups, the
+// last of which re-arms both, so the kernel can be relaunched (or replayed
+// from a graph) as is.  A launch with no more items than workgroups touches
+// neither counter: atomics on one address serialise (about 12 ns each), so
+// thousands of short items with one or two atomics each used to take longer
+// than their work (4608 single-child items of n <= 126: 161 us).
 // XYS: Euclidean coordinates are staged in LDS per item (otherwise the
 // decoder reads them through L2, leaving LDS to more resident workgroups).
 // Forms with at most 4 keys per thread ask for 8 waves per SIMD (at most 64
@@ -32,11 +37,16 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 4 ? 8 : 1) void k_step_queue(const 
     extern __shared__ __align__(16) char lds[];
     __shared__ int s_item;
     constexpr bool kXY = KIND == KIND_TSP_COORDS && XYS;
-    for (;;) {
+    const bool dynamic = num_items > (int)gridDim.x;
+    for (int round = 0;; ++round) {
         __syncthreads();  // the previous item is done with LDS and s_item
-        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
-        __syncthreads();
-        const int it = s_item;
+        int it = blockIdx.x;
+        if (round > 0) {
+            if (!dynamic) break;
+            if (threadIdx.x == 0) s_item = (int)gridDim.x + atomicAdd(queue, 1);
+            __syncthreads();
+            it = s_item;
+        }
         if (it >= num_items) break;
         const int4 w = items[it];
         const InstDesc& d = descs[w.x];
@@ -77,7 +87,7 @@ __global__ __launch_bounds__(BLOCK, ITEMS <= 4 ? 8 : 1) void k_step_queue(const 
                                                  nullptr);
         }
     }
-    if (threadIdx.x == 0) {
+    if (dynamic && threadIdx.x == 0) {
         const int retired = atomicAdd(queue + 1, 1);
         if (retired == (int)gridDim.x - 1) {
             atomicExch(queue, 0);
