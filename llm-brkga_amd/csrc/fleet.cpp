@@ -34,14 +34,18 @@ constexpr int kChunks[] = {1, 2, 4, 8, 16};
 // decide the shape.
 constexpr int kTuneRounds = 3;
 
-// Default slots per work item: enough decode work per item to hide the
-// queue pop and the per-item staging.
+// Default slots per work item (size class n, a power of two), from the
+// tuner's picks on the served mix: small children want one item each (the
+// first item of a workgroup needs no queue atomic, so short items cost
+// only their staging), mid-size ones a few slots to share the staging of
+// the island's rank order and coordinates, the largest one again.
 int default_chunk(int n)
 {
-    if (n >= 512) return 1;
-    if (n >= 256) return 2;
-    if (n >= 128) return 4;
-    return 8;
+    if (n <= 256) return 1;
+    if (n <= 512) return 2;
+    if (n <= 2048) return 4;
+    if (n <= 4096) return 2;
+    return 1;
 }
 
 }  // namespace
