@@ -369,7 +369,7 @@ brkga_status population_bytes(const brkga_problem_s* pr, const brkga_params* q, 
 }
 
 brkga_status create_population(brkga_problem_s* problem, const brkga_params* params, DeviceAllocator* a,
-                               hipStream_t s, int gen0, brkga_pop_s** out)
+                               hipStream_t s, int gen0, brkga_pop_s** out, PopSetup* deferred)
 {
     if (!problem || !params || !out) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
@@ -392,7 +392,17 @@ brkga_status create_population(brkga_problem_s* problem, const brkga_params* par
             (size_t)pop->step_lds > problem->engine->lds_limit)
             st = set_error(BRKGA_ERR_INVALID_ARG, "population or chromosome too large for the LDS kernels");
     }
-    if (st == BRKGA_OK) {
+    if (st == BRKGA_OK && deferred) {
+        const InstDesc& d = pop->host;
+        const size_t rows = pop->nsga2 ? (size_t)2 * d.pop : (size_t)d.pop;
+        deferred->keys[0] = d.keys[0];
+        deferred->keys[1] = d.keys[1];
+        deferred->key_floats = rows * d.ld;
+        deferred->gen = d.gen;
+        deferred->gen_edge = gen0 < 0 ? -1 : 0;
+        deferred->d_desc = pop->d_desc;
+        deferred->desc = d;
+    } else if (st == BRKGA_OK) {
         // clear the key buffers (their padding columns are never written by
         // the RNG paths, and copies must stay deterministic) and the
         // counters; pop->host outlives the descriptor copy

@@ -112,9 +112,12 @@ brkga_status population_bytes(const brkga_problem_s* pr, const brkga_params* q, 
 
 // Population over `pr` with buffers from `a` (nullptr: hipMalloc).  The
 // step counter starts at 0, or at -1 when gen0 < 0 (the first fleet tick
-// then builds generation 0).  Uploads run on stream s; the caller orders later work after them.
+// then builds generation 0).  Uploads run on stream s; the caller orders
+// later work after them.  With `deferred`, nothing is enqueued: the
+// device set-up (cleared key buffers, counters, descriptor) is described
+// there instead, for the caller to batch with others (launch_setup_pops).
 brkga_status create_population(brkga_problem_s* pr, const brkga_params* q, DeviceAllocator* a, hipStream_t s,
-                               int gen0, brkga_pop_s** out);
+                               int gen0, brkga_pop_s** out, PopSetup* deferred = nullptr);
 // Frees a population's buffers through its allocator (no device-wide wait
 // for arena-backed populations).
 void destroy_population(brkga_pop_s* pop);

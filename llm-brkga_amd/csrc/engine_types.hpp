@@ -180,6 +180,19 @@ constexpr int kCompactItems = 4;
 // Rank orders longer than this are read from global memory, not staged.
 constexpr int kMaxStagedOrder = 2048;
 
+// Device set-up of one new population, for set-ups batched into one launch
+// (launch_setup_pops): both key buffers cleared (key_floats floats each,
+// a multiple of 4), step counter slots gen[0..3] := {gen_edge, 0, 0,
+// gen_edge}, and the descriptor copied to d_desc.
+struct PopSetup {
+    float* keys[2];
+    size_t key_floats;
+    int* gen;
+    int gen_edge;
+    InstDesc* d_desc;
+    InstDesc desc;
+};
+
 // Step counter a ranking step reads when its parent buffer has parity
 // `parity`: gen[0] (even) or gen[3] (odd).  The step writes t + 1 to the
 // other slot, so no workgroup of a launch reads what that launch writes.

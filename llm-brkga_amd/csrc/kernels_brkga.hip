@@ -272,6 +272,37 @@ int partition_lds_bytes(int Pi)
 
 // ------------------------------------------------------------- launchers --
 
+// Workgroups (x, p) clear population p's key buffers with float4 stores
+// (grid-stride over x); workgroup (0, p) also sets its counters and
+// copies its descriptor.
+constexpr int kSetupBlock = 256;
+constexpr int kSetupBlocksX = 32;
+
+__global__ __launch_bounds__(kSetupBlock) void k_setup_pops(const PopSetup* __restrict__ list)
+{
+    const PopSetup& p = list[blockIdx.y];
+    const size_t q = p.key_floats >> 2;
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (size_t i = (size_t)blockIdx.x * kSetupBlock + threadIdx.x; i < q; i += (size_t)gridDim.x * kSetupBlock) {
+        reinterpret_cast<float4*>(p.keys[0])[i] = z;
+        reinterpret_cast<float4*>(p.keys[1])[i] = z;
+    }
+    if (blockIdx.x == 0) {
+        if (threadIdx.x < 4) p.gen[threadIdx.x] = (threadIdx.x == 0 || threadIdx.x == 3) ? p.gen_edge : 0;
+        constexpr int kWords = (int)(sizeof(InstDesc) / sizeof(int));
+        const int* src = reinterpret_cast<const int*>(&p.desc);
+        int* dst = reinterpret_cast<int*>(p.d_desc);
+        for (int w = threadIdx.x; w < kWords; w += kSetupBlock) dst[w] = src[w];
+    }
+}
+
+hipError_t launch_setup_pops(const PopSetup* d_list, int count, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_setup_pops, dim3(kSetupBlocksX, count), dim3(kSetupBlock), 0, s, d_list);
+    return hipGetLastError();
+}
+
 hipError_t launch_partition_one(const InstDesc& d, int buf, int mode, hipStream_t s, hipEvent_t start,
                                 hipEvent_t stop)
 {

@@ -40,6 +40,10 @@ hipError_t launch_step_multi(int block, int children, const InstDesc& d, int cur
 // raised, and lowers it (buf is then taken from the step counter); 3: closes
 // a run of ranking steps whose last generation is in buffer `buf` (both
 // step counter slots := that generation's counter).
+// The set-ups of `count` new populations (list in device memory), one
+// launch.
+hipError_t launch_setup_pops(const PopSetup* d_list, int count, hipStream_t s);
+
 hipError_t launch_partition_one(const InstDesc& d, int buf, int mode, hipStream_t s, hipEvent_t start = nullptr,
                                 hipEvent_t stop = nullptr);
 // buf < 0: buffer from each population's step counter.

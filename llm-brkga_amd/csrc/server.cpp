@@ -154,6 +154,9 @@ struct brkga_server_s {
     hipEvent_t setup_done = nullptr;       // population set-up on `stream` before solo steps
     char* staging = nullptr;               // pinned host staging of results
     size_t staging_bytes = 0;
+    brkga::PopSetup* setup_host = nullptr;  // pinned: set-ups of a round's admissions
+    brkga::PopSetup* setup_dev = nullptr;   // their device copy (launch_setup_pops)
+    size_t setup_cap = 0;
     std::chrono::steady_clock::time_point t0;
     std::thread worker;
 
@@ -237,10 +240,15 @@ bool brkga_server_s::retire_cancelled()
 }
 
 // Admits pending requests in arrival order while they fit; true when the
-// resident set changed.
+// resident set changed.  The new populations' device set-up (cleared key
+// buffers, counters, descriptors) is one upload and one launch on `stream`
+// for all of them: issued one by one, its six runtime calls per request
+// held a burst of 200 requests about 8 ms in admission.
 bool brkga_server_s::admit()
 {
     bool changed = false;
+    std::vector<PopSetup> setups;
+    std::vector<Request*> fresh;
     for (;;) {
         std::unique_ptr<Request> r;
         {
@@ -261,15 +269,52 @@ bool brkga_server_s::admit()
         r->alloc.base = r->slab;
         r->alloc.size = r->bytes;
         r->info.admit_us = now_us(t0);
-        brkga_status st = create_population(r->problem, &r->params, &r->alloc, stream, -1, &r->pop);
+        PopSetup su;
+        brkga_status st = create_population(r->problem, &r->params, &r->alloc, stream, -1, &r->pop, &su);
         if (st != BRKGA_OK) {
             r->pop = nullptr;
             finish(std::move(r), st);
             continue;
         }
         r->pop->host.gen_limit = r->generations;
+        su.desc.gen_limit = r->generations;
+        setups.push_back(su);
+        fresh.push_back(r.get());
         resident.push_back(std::move(r));
         changed = true;
+    }
+    if (setups.empty()) return changed;
+    brkga_status st = BRKGA_OK;
+    hipError_t e = hipSuccess;
+    if (setups.size() > setup_cap) {
+        // the previous lists are idle: every round ends by waiting for `stream`
+        if (setup_host) (void)hipHostFree(setup_host);
+        if (setup_dev) (void)hipFree(setup_dev);
+        setup_host = nullptr;
+        setup_dev = nullptr;
+        setup_cap = 0;
+        const size_t cap = std::max<size_t>(setups.size(), 256);
+        e = hipHostMalloc(reinterpret_cast<void**>(&setup_host), cap * sizeof(PopSetup), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&setup_dev), cap * sizeof(PopSetup));
+        if (e == hipSuccess) setup_cap = cap;
+        else st = hip_error(e, "population set-up lists");
+    }
+    if (st == BRKGA_OK) {
+        std::copy(setups.begin(), setups.end(), setup_host);
+        e = hipMemcpyAsync(setup_dev, setup_host, setups.size() * sizeof(PopSetup), hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = launch_setup_pops(setup_dev, (int)setups.size(), stream);
+        if (e != hipSuccess) st = hip_error(e, "population set-up");
+    }
+    if (st != BRKGA_OK) {
+        // the new requests end with the error; their populations were never set up
+        (void)hipStreamSynchronize(stream);
+        std::vector<std::unique_ptr<Request>> keep, failed;
+        for (auto& r : resident) {
+            if (std::find(fresh.begin(), fresh.end(), r.get()) != fresh.end()) failed.push_back(std::move(r));
+            else keep.push_back(std::move(r));
+        }
+        resident.swap(keep);
+        for (auto& r : failed) finish(std::move(r), st);
     }
     return changed;
 }
@@ -367,6 +412,14 @@ constexpr double kLoneRoundUs = 1500.0;
 // Most requests a solo round advances side by side, each on a stream of its
 // own; more resident requests share the batched ticks.
 constexpr int kSoloMax = 4;
+// A burst of submissions is admitted together: while at least two requests
+// are pending and the newest arrived less than kBurstGapUs ago, the worker
+// waits for the next one (at most kBurstMaxUs in all) before a round, so
+// the burst's requests start their generations side by side rather than a
+// few rounds apart (the last of them then finishes that many rounds
+// sooner).  A lone pending request is admitted at once.
+constexpr double kBurstGapUs = 200.0;
+constexpr double kBurstMaxUs = 20000.0;
 
 // One round for a few requests (at most kSoloMax) that have the device to
 // themselves with nothing pending: a chunk of each one's generations
@@ -458,6 +511,13 @@ void brkga_server_s::run()
             if (pending.empty() && resident.empty() && leaving.empty()) {
                 cancelled.clear();  // ids that finished before their cancel was seen
                 continue;
+            }
+            const double b0 = now_us(t0);
+            while (!stopping && pending.size() >= 2) {
+                const double now = now_us(t0);
+                const double gap = now - pending.back()->info.submit_us;
+                if (gap >= kBurstGapUs || now - b0 >= kBurstMaxUs) break;
+                cv_work.wait_for(lock, std::chrono::microseconds((long long)(kBurstGapUs - gap) + 1));
             }
             ++stats.rounds;
         }
@@ -701,6 +761,8 @@ brkga_status brkga_server_stop(brkga_server_t server)
             if (ss) (void)hipStreamDestroy(ss);
         if (server->setup_done) (void)hipEventDestroy(server->setup_done);
         if (server->staging) (void)hipHostFree(server->staging);
+        if (server->setup_host) (void)hipHostFree(server->setup_host);
+        if (server->setup_dev) (void)hipFree(server->setup_dev);
     }
     delete server;
     return BRKGA_OK;
