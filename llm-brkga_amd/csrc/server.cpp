@@ -156,6 +156,7 @@ struct brkga_server_s {
     size_t staging_bytes = 0;
     brkga::PopSetup* setup_host = nullptr;  // pinned: set-ups of a round's admissions
     brkga::PopSetup* setup_dev = nullptr;   // their device copy (launch_setup_pops)
+    int round_ticks = 0;                   // ticks of the last fleet round (grows while quiet)
     size_t setup_cap = 0;
     std::chrono::steady_clock::time_point t0;
     std::thread worker;
@@ -420,6 +421,12 @@ constexpr int kSoloMax = 4;
 // sooner).  A lone pending request is admitted at once.
 constexpr double kBurstGapUs = 200.0;
 constexpr double kBurstMaxUs = 20000.0;
+// Rounds with nothing pending and nothing admitted grow (doubling from
+// ticks_per_round up to kQuietRoundMax ticks, and not past the earliest
+// resident deadline beyond ticks_per_round): each round joins the stream
+// lanes and waits for the device once, which costs several percent at 4
+// ticks; a request arriving meanwhile waits for the round in progress.
+constexpr int kQuietRoundMax = 16;
 
 // One round for a few requests (at most kSoloMax) that have the device to
 // themselves with nothing pending: a chunk of each one's generations
@@ -547,7 +554,22 @@ void brkga_server_s::run()
                 st = fleet->autotune_new(cfg.autotune_reps, stream);
         }
         const bool ticking = st == BRKGA_OK && !resident.empty();
-        if (ticking) st = fleet->ticks(cfg.ticks_per_round, stream);
+        bool quiet = !admitted;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            quiet = quiet && pending.empty();
+        }
+        const int base = cfg.ticks_per_round;
+        int ticks = base;
+        if (quiet) {
+            round_ticks = std::min(std::max(2 * round_ticks, base), std::max(base, kQuietRoundMax));
+            int rem = round_ticks;
+            for (const auto& r : resident) rem = std::min(rem, r->generations + 1 - r->ticks);
+            ticks = std::max(base, rem);
+        } else {
+            round_ticks = base;
+        }
+        if (ticking) st = fleet->ticks(ticks, stream);
         collect_leaving();
         // problems of the requests that left are referenced by no queued
         // launch; hipFree waits for the device, i.e. for this round's ticks
@@ -562,13 +584,13 @@ void brkga_server_s::run()
         if (ticking && st == BRKGA_OK) {
             {
                 std::lock_guard<std::mutex> lock(mu);
-                stats.ticks += cfg.ticks_per_round;
+                stats.ticks += ticks;
                 stats.lanes = fleet->lanes();
             }
             // tick 1 builds generation 0, tick g + 1 completes step g
             std::vector<std::unique_ptr<Request>> keep;
             for (auto& r : resident) {
-                r->ticks += cfg.ticks_per_round;
+                r->ticks += ticks;
                 if (r->ticks >= r->generations + 1) leaving.push_back(std::move(r));
                 else keep.push_back(std::move(r));
             }
