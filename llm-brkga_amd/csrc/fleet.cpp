@@ -451,6 +451,12 @@ brkga_status Fleet::ensure_lanes(hipStream_t s)
     return BRKGA_OK;
 }
 
+brkga_status Fleet::prepare_lanes(hipStream_t s)
+{
+    DeviceGuard g(eng_->device);
+    return ensure_lanes(s);
+}
+
 brkga_status Fleet::set_counters(int gen, hipStream_t s)
 {
     DeviceGuard g(eng_->device);

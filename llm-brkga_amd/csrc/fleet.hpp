@@ -91,6 +91,11 @@ public:
     // tuned shape for yet.
     brkga_status autotune_new(int reps, hipStream_t s);
 
+    // Calibrates the stream lanes now rather than in the first ticks (about
+    // 10 ms of probe launches): a server does it when it starts, before any
+    // request waits for it.  A failure leaves the calibration to the ticks.
+    brkga_status prepare_lanes(hipStream_t s);
+
     const std::vector<FleetBucket>& buckets() const { return buckets_; }
     // Streams the ticks spread their launches over (1 until calibrated).
     int lanes() const { return nlanes_; }

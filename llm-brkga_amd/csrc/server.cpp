@@ -407,7 +407,7 @@ constexpr int kSoloChunk = 64;
 // device time each (sized from its previous round; the first runs
 // kLoneFirst generations): few synchronisations for a short request, and a
 // request arriving meanwhile waits at most about that long to be admitted.
-constexpr int kLoneFirst = 32;
+constexpr int kLoneFirst = 8;
 constexpr int kLoneMax = 256;
 constexpr double kLoneRoundUs = 1500.0;
 // Most requests a solo round advances side by side, each on a stream of its
@@ -644,6 +644,9 @@ brkga_status brkga_server_start(brkga_engine_t engine, const brkga_server_config
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->setup_done, hipEventDisableTiming);
     if (e != hipSuccess) return hip_error(e, "hipStreamCreate");
     s->fleet.reset(new Fleet(engine));
+    // the lanes are calibrated here, not when the first requests arrive
+    // (the server's own streams exist first, as when the ticks calibrate)
+    (void)s->fleet->prepare_lanes(s->stream);
     s->stats.budget_bytes = s->cfg.hbm_budget_bytes;
     s->t0 = std::chrono::steady_clock::now();
     brkga_server_s* raw = s.get();
