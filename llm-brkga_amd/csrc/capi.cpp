@@ -95,14 +95,21 @@ void set_scales(ProblemDesc* d, int s0, int s1)
     d->inv1 = std::ldexp(1.0f, -s1);
 }
 
+// Problem data go up asynchronously on the legacy stream (the host copy is
+// staged by the runtime before the call returns, so the caller's buffer is
+// free again); finish_problem records the problem's `ready` event after
+// them, and problem_ready waits for it before any device use.  A
+// synchronous copy per array cost about 17 us, most of it the wait, which
+// made building many small problems (a burst of served requests) slow.
 brkga_status upload(brkga_problem_s* pr, const void* host, size_t bytes, const void** dst)
 {
     void* p = nullptr;
     brkga_status st = device_alloc(&p, bytes, "problem data");
     if (st != BRKGA_OK) return st;
     pr->owned.push_back(p);
-    hipError_t e = hipMemcpy(p, host, bytes, hipMemcpyHostToDevice);
-    if (e != hipSuccess) return hip_error(e, "hipMemcpy(problem data)");
+    hipError_t e = hipMemcpyAsync(p, host, bytes, hipMemcpyHostToDevice, nullptr);
+    if (e != hipSuccess) return hip_error(e, "hipMemcpyAsync(problem data)");
+    pr->uploading = true;
     *dst = p;
     return BRKGA_OK;
 }
@@ -145,7 +152,14 @@ brkga_status new_problem(brkga_engine_t engine, int kind, int n, brkga_problem_s
 brkga_status finish_problem(brkga_problem_s* pr, brkga_status st, brkga_problem_t* out)
 {
     if (st == BRKGA_OK) st = check_lds(pr->engine, pr->desc);
+    if (st == BRKGA_OK && pr->uploading) {
+        hipError_t e = hipEventCreateWithFlags(&pr->ready, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(pr->ready, nullptr);
+        if (e != hipSuccess) st = hip_error(e, "problem upload event");
+    }
     if (st != BRKGA_OK) {
+        (void)hipStreamSynchronize(nullptr);  // no upload may still read or write what is freed
+        if (pr->ready) (void)hipEventDestroy(pr->ready);
         free_all(pr->owned);
         delete pr;
         return st;
@@ -155,6 +169,16 @@ brkga_status finish_problem(brkga_problem_s* pr, brkga_status st, brkga_problem_
 }
 
 }  // namespace
+
+brkga_status problem_ready(brkga_problem_s* pr)
+{
+    if (!pr->uploading.load()) return BRKGA_OK;
+    DeviceGuard g(pr->engine->device);
+    const hipError_t e = hipEventSynchronize(pr->ready);
+    if (e != hipSuccess) return hip_error(e, "problem upload");
+    pr->uploading = false;
+    return BRKGA_OK;
+}
 
 brkga_status set_error(brkga_status st, const std::string& msg)
 {
@@ -373,6 +397,8 @@ brkga_status create_population(brkga_problem_s* problem, const brkga_params* par
 {
     if (!problem || !params || !out) return set_error(BRKGA_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
+    const brkga_status rs = problem_ready(problem);
+    if (rs != BRKGA_OK) return rs;
     auto* pop = new brkga_pop_s();
     pop->problem = problem;
     pop->allocator = a;
@@ -876,6 +902,8 @@ brkga_status brkga_problem_destroy(brkga_problem_t problem)
 {
     if (!problem) return BRKGA_OK;
     DeviceGuard g(problem->engine->device);
+    (void)problem_ready(problem);
+    if (problem->ready) (void)hipEventDestroy(problem->ready);
     free_all(problem->owned);
     delete problem;
     return BRKGA_OK;
@@ -892,6 +920,8 @@ static brkga_status evaluate_impl(brkga_problem_t problem, const float* d_keys, 
                                                     std::to_string(problem->desc.chrom_len) + ")");
     if (pop_size == 0) return BRKGA_OK;
     DeviceGuard g(problem->engine->device);
+    const brkga_status rs = problem_ready(problem);
+    if (rs != BRKGA_OK) return rs;
     hipError_t e = launch_evaluate(problem->desc, d_keys, chrom_len, pop_size, d_fitness, d_decoded,
                                    default_block(problem->desc.n), as_stream(stream));
     if (e != hipSuccess) return hip_error(e, "evaluate launch");
@@ -922,7 +952,9 @@ brkga_status brkga_evaluate_host(brkga_problem_t problem, const float* keys, int
     const ProblemDesc& p = problem->desc;
     const size_t nk = (size_t)count * p.chrom_len;
     void *dk = nullptr, *df = nullptr, *dd = nullptr;
-    brkga_status st = device_alloc(&dk, nk * sizeof(float), "keys");
+    brkga_status st = problem_ready(problem);
+    if (st != BRKGA_OK) return st;
+    st = device_alloc(&dk, nk * sizeof(float), "keys");
     if (st == BRKGA_OK) st = device_alloc(&df, (size_t)count * p.num_obj * sizeof(float), "fitness");
     if (st == BRKGA_OK) st = device_alloc(&dd, nk * sizeof(int), "decoded");
     hipError_t e = hipSuccess;
@@ -959,6 +991,8 @@ brkga_status brkga_evaluate_host_noalloc(brkga_problem_t problem, const float* k
     if (dev < 0 || dev >= kMaxScratchDevices || !g_scratch_ptr[dev])
         return set_error(BRKGA_ERR_DEVICE, "static scratch not resolved for this device");
     DeviceGuard g(dev);
+    const brkga_status rs = problem_ready(problem);
+    if (rs != BRKGA_OK) return rs;
     // every call on a device shares its scratch: calls from several host
     // threads run one after the other
     std::lock_guard<std::mutex> lock(g_scratch_call_mu[dev]);

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -63,6 +64,10 @@ struct brkga_problem_s {
     brkga_engine_s* engine;
     brkga::ProblemDesc desc;  // device pointers
     std::vector<void*> owned;
+    // The data uploads are asynchronous (the legacy stream, then `ready`):
+    // every use of the data on the device first calls problem_ready.
+    hipEvent_t ready = nullptr;
+    std::atomic<bool> uploading{false};
 };
 
 namespace brkga {
@@ -99,6 +104,9 @@ struct brkga_pop_s {
 };
 
 namespace brkga {
+
+// Waits for the problem's data uploads (once; later calls return at once).
+brkga_status problem_ready(brkga_problem_s* pr);
 
 // Validates params against a problem and fills `d` (pointers excluded).
 brkga_status fill_inst_desc(const brkga_problem_s* pr, const brkga_params* q, InstDesc* d, int* nsga2);
