@@ -126,6 +126,10 @@ brkga_status brkga_engine_destroy(brkga_engine_t engine);
 
 /* ----------------------------------------------------------- problems --- */
 
+/* Every constructor copies the caller's arrays to the device asynchronously
+ * on the legacy stream; the arrays may be reused or freed as soon as the
+ * call returns, and the problem's first device use waits for the copy. */
+
 /* Euclidean TSP over n cities from FP32 coordinates xy[n][2]; fitness is
  * the closed-tour length, minimised.  Same decoder and edge formula as
  * TSPConfig (configs/tsp_config.hpp:180-225, 359-371), without the n x n

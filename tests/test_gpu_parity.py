@@ -51,6 +51,37 @@ def test_evaluate_and_decode_parity(engine, kind, n):
         assert np.array_equal(gf, of), f"fitness mismatch ({kind}, n={n}, ties={ties})"
 
 
+@pytest.mark.parametrize("kind,n", [("tsp", 12000), ("tsp_matrix", 1000), ("knap_greedy", 2000)])
+def test_problem_host_arrays_free_after_create(engine, kind, n):
+    """Problem data go up asynchronously; the caller may overwrite its host
+    arrays as soon as the create call returns (96 KB, 4 MB and two 8 KB
+    arrays here)."""
+    print(f"host arrays reused after create {kind} n={n}", flush=True)
+    if kind == "tsp":
+        xy = workloads.tsp_coords(n, n, 10000.0)
+        op = pyoracle.Problem.tsp_coords(xy.copy())
+        gp = brkga_amd.Problem.tsp_coords(engine, xy)
+        arrays = [xy]
+    elif kind == "tsp_matrix":
+        d = workloads.euclid_matrix(workloads.tsp_coords(n, n))
+        op = pyoracle.Problem.tsp_matrix(d.copy())
+        gp = brkga_amd.Problem.tsp_matrix(engine, d)
+        arrays = [d]
+    else:
+        w, v, cap = workloads.knapsack(n, n)
+        w, v = np.ascontiguousarray(w, np.int32), np.ascontiguousarray(v, np.int32)
+        op = pyoracle.Problem.knapsack(w.copy(), v.copy(), cap, True)
+        gp = brkga_amd.Problem.knapsack(engine, w, v, cap, True)
+        arrays = [w, v]
+    for a in arrays:
+        a[...] = 7
+    keys = random_keys(n, 32, op.chrom_len)
+    gf, gd = gpu_evaluate(gp, keys)
+    of, od = op.evaluate(keys)
+    assert np.array_equal(gd, od) and np.array_equal(gf, of)
+    gp.close()
+
+
 def test_problem_size_limits(engine):
     """Above 16000 genes, or beyond the decoder's LDS (TSP n = 16000 needs
     more than 160 KiB), creation fails with a clear error."""
